@@ -1,0 +1,50 @@
+"""Build the gfx950 shared library libgdremc.so in-tree with hipcc.
+
+The library holds the HIP kernels (csrc/gd_kernels.hip), the C ABI host code
+(csrc/gd_host.cpp) and the host preparation / parameter parsing
+(csrc/gd_prep.cpp).  All device code is compiled with contraction off
+(the canonical arithmetic relies on it) and without packed FP32 ops.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgdremc.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("GD_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+          "-Wall", "-Wno-unused-function"]
+DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    """Compile every source and link libgdremc.so; returns the library path."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src, dev in (("gd_kernels.hip", True), ("gd_host.cpp", False),
+                     ("gd_prep.cpp", False)):
+        obj = os.path.join(objdir, src + ".o")
+        cmd = [HIPCC] + COMMON + (DEVICE if dev else []) + [
+            "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,375 @@
+/*
+ * gd_canon.h -- the canonical FP32 arithmetic of the REMC docking step.
+ *
+ * Every floating-point operation that decides a Monte Carlo trajectory is
+ * written here once, in terms of IEEE-754 operations that are correctly
+ * rounded on both the x86 host and gfx950 (add, sub, mul, fma, div, sqrt,
+ * rint, compares) and integer bit manipulation.  Nothing calls a vendor
+ * math library: exp, log, sin and cos are built from those operations.
+ * Compiled with contraction off (-ffp-contract=off) on both sides, the same
+ * inputs therefore give the same bits on the CPU and on the GPU, which is
+ * what lets a CPU replay reproduce the GPU's accept/exchange decisions.
+ *
+ * Methods (written from the published algorithms, constants derived here):
+ *   exp  : Cody-Waite reduction x = n*ln2 + r with ln2 split into a float
+ *          and its float remainder, Taylor polynomial of degree 7 on
+ *          |r| <= ln2/2 (truncation error < 6e-9 relative), scale by 2^n
+ *          assembled from exponent bits.
+ *   log  : x = m*2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+ *          log m = 2 atanh(s) summed to s^11.
+ *   sin/cos : reduction by n = rint(x*2/pi) with pi/2 split in three
+ *          floats, Taylor polynomials on |r| <= pi/4, quadrant selection.
+ *   Philox4x32-10 : Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
+ *          as easy as 1, 2, 3", SC'11 (multipliers 0xD2511F53, 0xCD9E8D57,
+ *          Weyl key increments 0x9E3779B9, 0xBB67AE85).
+ *
+ * Energy forms follow the reference kernel gpudocksm-rem-1.2/src/
+ * calcenergy_d.cu:69-135 (pair terms), :142-149 (hydrophobic restraint),
+ * :186-202 (KDE), :240-268 (position restraints), :276-309 (distance,
+ * averaging, normalisation); the algebra is the same, the rounding order is
+ * the one written below.
+ */
+#ifndef GD_CANON_H
+#define GD_CANON_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GD_HD __host__ __device__ __forceinline__
+#else
+#define GD_HD static inline
+#endif
+
+/* ------------------------------------------------------------------ */
+/* bit casts                                                           */
+/* ------------------------------------------------------------------ */
+GD_HD float gd_as_float(uint32_t u)
+{
+  union { uint32_t u; float f; } v;
+  v.u = u;
+  return v.f;
+}
+
+GD_HD uint32_t gd_as_uint(float f)
+{
+  union { uint32_t u; float f; } v;
+  v.f = f;
+  return v.u;
+}
+
+GD_HD float gd_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+GD_HD float gd_rint(float x) { return __builtin_rintf(x); }
+GD_HD float gd_sqrt(float x) { return __builtin_sqrtf(x); }
+
+#define GD_INF_BITS 0x7f800000u
+
+/* ------------------------------------------------------------------ */
+/* exp                                                                  */
+/* ------------------------------------------------------------------ */
+/* Results for x > 88 are +inf and for x < -86 exactly 0, so no
+ * subnormal result is ever produced (both sides agree on those too). */
+GD_HD float gd_exp(float x)
+{
+  if (x != x)
+    return x;
+  if (x > 88.0f)
+    return gd_as_float(GD_INF_BITS);
+  if (x < -86.0f)
+    return 0.0f;
+  const float n = gd_rint(x * 0x1.715476p+0f);         /* x / ln2 */
+  float r = gd_fma(-n, 0x1.62e430p-1f, x);              /* ln2 (float) */
+  r = gd_fma(-n, -0x1.05c610p-29f, r);                   /* ln2 - float(ln2) */
+  float p = 0x1.a01a02p-13f;                             /* 1/7! */
+  p = gd_fma(p, r, 0x1.6c16c2p-10f);                     /* 1/6! */
+  p = gd_fma(p, r, 0x1.111112p-7f);                      /* 1/5! */
+  p = gd_fma(p, r, 0x1.555556p-5f);                      /* 1/4! */
+  p = gd_fma(p, r, 0x1.555556p-3f);                      /* 1/3! */
+  p = gd_fma(p, r, 0.5f);
+  p = gd_fma(p, r, 1.0f);
+  p = gd_fma(p, r, 1.0f);
+  const int ni = (int) n;                                /* in [-125, 127] */
+  return p * gd_as_float((uint32_t) (ni + 127) << 23);
+}
+
+/* ------------------------------------------------------------------ */
+/* log                                                                  */
+/* ------------------------------------------------------------------ */
+GD_HD float gd_log(float x)
+{
+  if (x != x)
+    return x;
+  if (x < 0.0f)
+    return gd_as_float(0x7fc00000u);
+  if (x == 0.0f)
+    return -gd_as_float(GD_INF_BITS);
+  if (x == gd_as_float(GD_INF_BITS))
+    return x;
+  int e = 0;
+  if (x < 0x1p-126f) {            /* subnormal: scale into the normal range */
+    x = x * 0x1p+24f;
+    e = -24;
+  }
+  const uint32_t b = gd_as_uint(x);
+  e += (int) ((b >> 23) & 0xffu) - 127;
+  float m = gd_as_float((b & 0x007fffffu) | 0x3f800000u);   /* [1, 2) */
+  if (m > 0x1.6a09e6p+0f) {                                  /* sqrt(2) */
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.0f;
+  const float s = f / (m + 1.0f);
+  const float s2 = s * s;
+  float p = 0x1.745d18p-4f;              /* 1/11 */
+  p = gd_fma(p, s2, 0x1.c71c72p-4f);     /* 1/9 */
+  p = gd_fma(p, s2, 0x1.24924ap-3f);     /* 1/7 */
+  p = gd_fma(p, s2, 0x1.99999ap-3f);     /* 1/5 */
+  p = gd_fma(p, s2, 0x1.555556p-2f);     /* 1/3 */
+  p = p * s2;
+  const float lm = gd_fma(2.0f * s, p, 2.0f * s);            /* 2 atanh(s) */
+  const float ef = (float) e;
+  return gd_fma(ef, 0x1.62e430p-1f, gd_fma(ef, -0x1.05c610p-29f, lm));
+}
+
+/* ------------------------------------------------------------------ */
+/* sin / cos                                                            */
+/* ------------------------------------------------------------------ */
+GD_HD void gd_sincos(float x, float *sp, float *cp)
+{
+  const float n = gd_rint(x * 0x1.45f306p-1f);          /* x * 2/pi */
+  float r = gd_fma(-n, 0x1.921fb6p+0f, x);
+  r = gd_fma(-n, -0x1.777a5cp-25f, r);
+  r = gd_fma(-n, -0x1.000000p-49f, r);
+  const float r2 = r * r;
+  /* sin r = r - r^3/3! + r^5/5! - r^7/7! + r^9/9! */
+  float ps = 0x1.71de3ap-19f;            /* 1/9! */
+  ps = gd_fma(ps, r2, -0x1.a01a02p-13f); /* -1/7! */
+  ps = gd_fma(ps, r2, 0x1.111112p-7f);   /* 1/5! */
+  ps = gd_fma(ps, r2, -0x1.555556p-3f);  /* -1/3! */
+  ps = ps * r2;
+  const float s = gd_fma(ps, r, r);
+  /* cos r = 1 - r^2/2! + r^4/4! - r^6/6! + r^8/8! - r^10/10! */
+  float pc = -0x1.27e4fcp-22f;           /* -1/10! */
+  pc = gd_fma(pc, r2, 0x1.a01a02p-16f);  /* 1/8! */
+  pc = gd_fma(pc, r2, -0x1.6c16c2p-10f); /* -1/6! */
+  pc = gd_fma(pc, r2, 0x1.555556p-5f);   /* 1/4! */
+  pc = gd_fma(pc, r2, -0.5f);
+  const float c = gd_fma(pc, r2, 1.0f);
+  const int q = ((int) n) & 3;
+  float so, co;
+  if (q == 0) { so = s;  co = c;  }
+  else if (q == 1) { so = c;  co = -s; }
+  else if (q == 2) { so = -s; co = -c; }
+  else { so = -c; co = s;  }
+  *sp = so;
+  *cp = co;
+}
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 and the (0,1] mapping                                  */
+/* ------------------------------------------------------------------ */
+typedef struct { uint32_t v[4]; } gd_u32x4;
+
+GD_HD gd_u32x4 gd_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                uint32_t k0, uint32_t k1)
+{
+  for (int i = 0; i < 10; ++i) {
+    const uint64_t p0 = (uint64_t) 0xD2511F53u * (uint64_t) c0;
+    const uint64_t p1 = (uint64_t) 0xCD9E8D57u * (uint64_t) c2;
+    const uint32_t hi0 = (uint32_t) (p0 >> 32), lo0 = (uint32_t) p0;
+    const uint32_t hi1 = (uint32_t) (p1 >> 32), lo1 = (uint32_t) p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  gd_u32x4 r;
+  r.v[0] = c0; r.v[1] = c1; r.v[2] = c2; r.v[3] = c3;
+  return r;
+}
+
+/* Top 24 bits plus one, times 2^-24: a value in (0, 1], exact in float.
+ * Same half-open side as the reference's curand_uniform. */
+GD_HD float gd_u01(uint32_t w) { return (float) ((w >> 8) + 1u) * 0x1p-24f; }
+
+/* Counter layout.  key = (seed, complex id).
+ *   MC step s of replica r: ctr (r, s, 0, 0) gives u0..u3, ctr (r, s, 1, 0)
+ *   gives u4..u7; u0..u5 perturb the six degrees of freedom, u6 is the
+ *   Metropolis draw.
+ *   Exchange number x of temperature chain c, temperature pair j:
+ *   ctr (c, x, 2, j >> 2), element j & 3. */
+#define GD_CTR_MOVE_A 0u
+#define GD_CTR_MOVE_B 1u
+#define GD_CTR_EXCHANGE 2u
+
+/* ------------------------------------------------------------------ */
+/* Rigid move (reference move_d.cu:79-132)                              */
+/* ------------------------------------------------------------------ */
+/* R = Rz(a) Ry(b) Rx(g) for (a, b, g) = (m[3], m[4], m[5]). */
+GD_HD void gd_rotation(const float m[6], float rot[9])
+{
+  float s1, c1, s2, c2, s3, c3;
+  gd_sincos(m[3], &s1, &c1);
+  gd_sincos(m[4], &s2, &c2);
+  gd_sincos(m[5], &s3, &c3);
+  rot[0] = c1 * c2;
+  rot[1] = c1 * s2 * s3 - c3 * s1;
+  rot[2] = s1 * s3 + c1 * c3 * s2;
+  rot[3] = c2 * s1;
+  rot[4] = c1 * c3 + s1 * s2 * s3;
+  rot[5] = c3 * s1 * s2 - c1 * s3;
+  rot[6] = -s2;
+  rot[7] = c2 * s3;
+  rot[8] = c2 * c3;
+}
+
+/* x' = R x + t + c, summed left to right as in the reference. */
+GD_HD void gd_transform(const float rot[9], const float m[6], const float c[3],
+                        float x, float y, float z, float *ox, float *oy, float *oz)
+{
+  *ox = rot[0] * x + rot[1] * y + rot[2] * z + m[0] + c[0];
+  *oy = rot[3] * x + rot[4] * y + rot[5] * z + m[1] + c[1];
+  *oz = rot[6] * x + rot[7] * y + rot[8] * z + m[2] + c[2];
+}
+
+/* ------------------------------------------------------------------ */
+/* Pair terms (reference calcenergy_d.cu:69-135)                        */
+/* ------------------------------------------------------------------ */
+/* Per (ligand type, point type) table entry. */
+typedef struct {
+  float p1a, p2a, pmf0, pmf1, hdb0, hdb1, pad0, pad1;
+} gd_pair_param;
+
+/* Packed per-point flags. */
+#define GD_PT_TYPE_MASK 0x1fu
+#define GD_PT_C2 0x20u          /* point class == 2 (PSP applies) */
+#define GD_PT_HPC 0x40u         /* (class 0 and residue GLY) or class 2 */
+#define GD_PT_SEQ_SHIFT 8
+
+/* Scalars shared by every pair. */
+typedef struct {
+  float lj0, lj1, el0, el1, a1, b1, inv_el1, pad;
+} gd_pair_scalars;
+
+/* Accumulators of one lane. */
+typedef struct {
+  float vdw, ele, pmf, psp, hdb;
+} gd_pair_acc;
+
+GD_HD float gd_dist2(float dx, float dy, float dz)
+{
+  return gd_fma(dz, dz, gd_fma(dy, dy, dx * dx));
+}
+
+/* hydrophobic contact kernel 1 - (3.5 u - 4.5 u^2 + 2.5 u^3 - 0.5 u^4),
+ * u = d^2 / 81, written as a polynomial in d^2 */
+GD_HD float gd_hpc_pair(float d2, float hpp)
+{
+  float t = gd_fma(d2, -0x1.8f1924p-27f, 0x1.3bb164p-18f);
+  t = gd_fma(d2, t, -0x1.67980ep-11f);
+  t = gd_fma(d2, t, 0x1.61f9aep-5f);
+  t = d2 * t;
+  return gd_fma(-hpp, t, hpp);
+}
+
+/* The five directly summed pair terms.  psp_row is the PSP table row of
+ * the ligand atom's type; it is read at index seq only for class-2 points
+ * inside the PMF radius (calcenergy_d.cu:123-126). */
+GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
+                         const gd_pair_scalars *sc, int is_c2,
+                         const float *psp_row, int seq, gd_pair_acc *acc)
+{
+  const float d = gd_sqrt(d2);
+  const float r2 = 1.0f / d2;
+  const float r1 = d * r2;
+  const float r4 = r2 * r2;
+  const float r6 = r4 * r2;
+  const float r9 = (r4 * r4) * r1;
+
+  /* soft L-J */
+  const float p1 = pp->p1a * r9;
+  const float p2 = pp->p2a * r6;
+  const float p4 = gd_fma(p1 * sc->lj0, gd_fma(sc->lj1, d2, 1.0f), 1.0f);
+  acc->vdw += (p1 - p2) / p4;
+
+  /* distance-dependent electrostatics */
+  const float s = sc->el1 * d;
+  const float g = (s < 1.0f) ? gd_fma(s * s, gd_fma(sc->b1, s, sc->a1), sc->el0)
+                             : r1 * sc->inv_el1;
+  acc->ele = gd_fma(lq_ele, g, acc->ele);
+
+  /* PMF contact */
+  const float dm = d - pp->pmf0;
+  const float ex = gd_exp(gd_fma(-0.5f, d, 6.0f) * dm);
+  acc->pmf += pp->pmf1 / (1.0f + ex);
+
+  /* pocket-specific */
+  if (is_c2 && dm <= 0.0f)
+    acc->psp += psp_row[seq];
+
+  /* hydrogen bond */
+  if (pp->hdb0 > 0.1f) {
+    const float h = (d - pp->hdb0) * pp->hdb1;
+    acc->hdb = gd_fma(pp->hdb1, gd_exp((-0.5f * h) * h), acc->hdb);
+  }
+}
+
+/* Raw (averaged, not normalised) terms in the parameter-file order used by
+ * the weight and normalisation files: vdw ele pmf hpc hdb dst psp kde lhm. */
+enum {
+  GD_F_VDW = 0, GD_F_ELE, GD_F_PMF, GD_F_HPC, GD_F_HDB, GD_F_DST, GD_F_PSP,
+  GD_F_KDE, GD_F_LHM, GD_F_NTERM
+};
+
+/* Output slots, same order as the reference Energy.e[] (dock.h:100-115). */
+enum {
+  GD_E_VDW = 0, GD_E_ELE, GD_E_PMF, GD_E_PSP, GD_E_HDB, GD_E_HPC, GD_E_KDE,
+  GD_E_LHM, GD_E_DST, GD_E_TOTAL, GD_E_N
+};
+
+/* slot index of each file-order term */
+#define GD_FILE_TO_SLOT_INIT {GD_E_VDW, GD_E_ELE, GD_E_PMF, GD_E_HPC, GD_E_HDB, \
+                              GD_E_DST, GD_E_PSP, GD_E_KDE, GD_E_LHM}
+
+/* term_order 0: parameter k applies to file-order term k (the documented
+ * order, cpu_mic calcenergy_cpu.C:220-239); 1: parameter k applies to
+ * slot k (rem-1.2 calcenergy_d.cu:301-309,349-358). */
+GD_HD void gd_finalize(const float sums[GD_F_NTERM], float lna_f,
+                       const float a[10], const float b[10], const float w[10],
+                       int term_order, float out[GD_E_N])
+{
+  float raw[GD_F_NTERM];
+  raw[GD_F_VDW] = sums[GD_F_VDW] / lna_f;
+  raw[GD_F_ELE] = sums[GD_F_ELE] / lna_f;
+  raw[GD_F_PMF] = sums[GD_F_PMF] / lna_f;
+  raw[GD_F_HPC] = sums[GD_F_HPC] / lna_f;
+  raw[GD_F_HDB] = sums[GD_F_HDB] / lna_f / 0x1.40d932p+1f * -1.0f;   /* sqrt(2 pi) */
+  raw[GD_F_DST] = sums[GD_F_DST];
+  raw[GD_F_PSP] = sums[GD_F_PSP] / lna_f;
+  raw[GD_F_KDE] = sums[GD_F_KDE] / lna_f;
+  raw[GD_F_LHM] = sums[GD_F_LHM];
+  const int f2s[GD_F_NTERM] = GD_FILE_TO_SLOT_INIT;
+  float slot[GD_E_N];
+  for (int k = 0; k < GD_F_NTERM; ++k)
+    slot[f2s[k]] = raw[k];
+  float total = 0.0f;
+  if (term_order == 0) {
+    for (int k = 0; k < GD_F_NTERM; ++k) {
+      const float v = gd_fma(a[k], raw[k], b[k]);
+      slot[f2s[k]] = v;
+      total = gd_fma(w[k], v, total);
+    }
+  } else {
+    for (int i = 0; i < GD_F_NTERM; ++i) {
+      const float v = gd_fma(a[i], slot[i], b[i]);
+      slot[i] = v;
+      total = gd_fma(w[i], v, total);
+    }
+  }
+  slot[GD_E_TOTAL] = total;
+  for (int i = 0; i < GD_E_N; ++i)
+    out[i] = slot[i];
+}
+
+#endif /* GD_CANON_H */

@@ -1,0 +1,674 @@
+/*
+ * gd_kernels.hip -- CDNA4 (gfx950) kernels of the REMC docking core.
+ *
+ *   gd_mc_kernel      fused Move + CalcEnergy + Accept for a run of MC
+ *                     steps (montecarlo_d.cu:59-106 with move_d.cu:73-134,
+ *                     calcenergy_d.cu:22-374, accept_d.cu:12-52), or the
+ *                     initial energies (MonteCarlo_Init_d, :12-55).
+ *   gd_exchange_kernel temperature swaps, one lane per chain
+ *                     (exchangereplicas_d.cu:7-136).
+ *   gd_score_kernel   energies of caller-given poses (FP32).
+ *   gd_rescore64_kernel FP64 energies of caller-given poses.
+ *
+ * Execution model: a workgroup is a task of up to GD_WPG replicas that share
+ * one protein conformer.  The workgroup stages that conformer's pocket
+ * points, the pair-parameter rows of the ligand's atom types and (when it
+ * fits) the type-sorted KDE points in LDS once; then each wavefront owns one
+ * replica for all steps of the launch.  Lanes run over pocket points (and
+ * over KDE points / ligand atoms in the smaller loops); the ligand atom of
+ * the outer loop is broadcast from the lane that holds it with v_readlane,
+ * so it sits in scalar registers.  Per-term sums are per-lane FP32
+ * accumulators reduced by an xor butterfly -- the order the CPU replay
+ * reproduces bit for bit.
+ */
+#include <hip/hip_runtime.h>
+#include "gd_device.h"
+
+#define GD_WPG 8  /* wavefronts per workgroup */
+
+__device__ __forceinline__ float gd_wave_sum(float v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    v += __shfl_xor(v, off, GD_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double gd_wave_sum64(double v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    v += __shfl_xor(v, off, GD_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ float gd_bcast(float v, int lane)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+/* LDS carve-up, offsets in bytes (host computes them with the same rule). */
+struct gd_lds_layout {
+  int off_eh, off_tab, off_kde, bytes;
+};
+
+/* ------------------------------------------------------------------------ */
+/* Workgroup staging                                                         */
+/* ------------------------------------------------------------------------ */
+template <bool KDE_LDS>
+__device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem,
+                                         const gd_lds_layout &L)
+{
+  float4 *s_pos = (float4 *) smem;
+  float2 *s_eh = (float2 *) (smem + L.off_eh);
+  float4 *s_tab = (float4 *) (smem + L.off_tab);
+  const float4 *g_pos = P.pt_pos + (size_t) prt * P.pnp;
+  const float2 *g_eh = P.pt_eh + (size_t) prt * P.pnp;
+  for (int i = threadIdx.x; i < P.pnp; i += blockDim.x) {
+    s_pos[i] = g_pos[i];
+    s_eh[i] = g_eh[i];
+  }
+  const float4 *g_tab = (const float4 *) P.tab;
+  const int ntab4 = P.n_rows * GD_MAXTP1_DEV * 2;
+  for (int i = threadIdx.x; i < ntab4; i += blockDim.x)
+    s_tab[i] = g_tab[i];
+  if (KDE_LDS) {
+    float4 *s_kde = (float4 *) (smem + L.off_kde);
+    for (int i = threadIdx.x; i < P.pnk; i += blockDim.x)
+      s_kde[i] = P.kde[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energy of one pose, one wavefront (calcenergy_d.cu:22-374).               */
+/* Lane j holds ligand atoms j and j + 64 in (ax0..az0) and (ax1..az1).      */
+/* On return out[] (slot order) is identical in every lane.                  */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ void gd_energy_wave(
+    const gd_pack &P, const float4 *__restrict__ s_pos,
+    const float2 *__restrict__ s_eh, const gd_pair_param *__restrict__ s_tab,
+    const float4 *__restrict__ kde, float ax0, float ay0, float az0, float ax1,
+    float ay1, float az1, const float cen[3], float out[GD_E_N])
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int lna = P.lna;
+  const int pnp = P.pnp;
+  const int nchunk = (pnp + GD_WAVE - 1) / GD_WAVE;
+
+  gd_pair_acc acc;
+  acc.vdw = 0.0f; acc.ele = 0.0f; acc.pmf = 0.0f; acc.psp = 0.0f; acc.hdb = 0.0f;
+  float ehpc = 0.0f;
+
+  /* pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) */
+  for (int l = 0; l < lna; ++l) {
+    const int src = l & (GD_WAVE - 1);
+    const bool hi = l >= GD_WAVE;
+    const float lx = gd_bcast(hi ? ax1 : ax0, src);
+    const float ly = gd_bcast(hi ? ay1 : ay0, src);
+    const float lz = gd_bcast(hi ? az1 : az0, src);
+    const int row = P.lig_row[l];
+    const int lt = P.lig_type[l];
+    const float lq = P.lig_q[l];
+    const gd_pair_param *trow = s_tab + row * GD_MAXTP1_DEV;
+    const float *psp_row = P.psp + (size_t) lt * P.n_res;
+    float hacc = 0.0f;
+    for (int k = 0; k < nchunk; ++k) {
+      const int p = lane + k * GD_WAVE;
+      if (p < pnp) {
+        const float4 q = s_pos[p];
+        const float2 eh = s_eh[p];
+        const uint32_t bits = __float_as_uint(q.w);
+        const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+        if ((bits & GD_PT_HPC) && d2 <= 81.0f)
+          hacc += gd_hpc_pair(d2, eh.y);
+        const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
+        gd_pair_terms(d2, lq * eh.x, &pp, &P.sc, (bits & GD_PT_C2) != 0, psp_row,
+                      (int) (bits >> GD_PT_SEQ_SHIFT), &acc);
+      }
+    }
+    const float hs = gd_wave_sum(hacc);
+    const float z = (hs - P.hpl0[lt]) / P.hpl1[lt];
+    ehpc += 0.5f * z * z - P.hpl2[lt];
+  }
+
+  /* KDE (calcenergy_d.cu:175-213) over the type-sorted segment */
+  float ekde = 0.0f;
+  for (int l = 0; l < lna; ++l) {
+    const int src = l & (GD_WAVE - 1);
+    const bool hi = l >= GD_WAVE;
+    const float lx = gd_bcast(hi ? ax1 : ax0, src);
+    const float ly = gd_bcast(hi ? ay1 : ay0, src);
+    const float lz = gd_bcast(hi ? az1 : az0, src);
+    const int row = P.lig_row[l];
+    const int s0 = P.kde_seg[2 * row];
+    const int s1 = P.kde_seg[2 * row + 1];
+    float v = 0.0f;
+    for (int i = s0 + lane; i < s1; i += GD_WAVE) {
+      const float4 q = kde[i];
+      const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+      v += gd_exp(P.kde2 * d2);
+    }
+    v = gd_wave_sum(v);
+    const int n = s1 - s0;
+    if (n > 0)
+      ekde += v / (float) n;
+  }
+  ekde = ekde / P.kde3;
+
+  /* position restraints (calcenergy_d.cu:229-270) */
+  float elhm = 0.0f;
+  for (int m = 0; m < P.pos; ++m) {
+    float s = 0.0f, c = 0.0f;
+    const float4 *tm = P.mcs + (size_t) m * lna;
+    if (lane < lna) {
+      const float4 t = tm[lane];
+      if (t.w != 0.0f) {
+        s += gd_dist2(ax0 - t.x, ay0 - t.y, az0 - t.z);
+        c += 1.0f;
+      }
+    }
+    if (lane + GD_WAVE < lna) {
+      const float4 t = tm[lane + GD_WAVE];
+      if (t.w != 0.0f) {
+        s += gd_dist2(ax1 - t.x, ay1 - t.y, az1 - t.z);
+        c += 1.0f;
+      }
+    }
+    s = gd_wave_sum(s);
+    c = gd_wave_sum(c);
+    if (c > 0.0f)
+      elhm += P.mcs_tcc[m] * gd_sqrt(s / c);
+  }
+  elhm = (P.pos > 0) ? gd_log(elhm / (float) P.pos) : 0.0f;
+
+  /* distance to the pocket centre (calcenergy_d.cu:276-283) */
+  const float edst = gd_sqrt(gd_dist2(cen[0] - P.pocket_center[0],
+                                      cen[1] - P.pocket_center[1],
+                                      cen[2] - P.pocket_center[2]));
+
+  float sums[GD_F_NTERM];
+  sums[GD_F_VDW] = gd_wave_sum(acc.vdw);
+  sums[GD_F_ELE] = gd_wave_sum(acc.ele);
+  sums[GD_F_PMF] = gd_wave_sum(acc.pmf);
+  sums[GD_F_HPC] = ehpc;
+  sums[GD_F_HDB] = gd_wave_sum(acc.hdb);
+  sums[GD_F_DST] = edst;
+  sums[GD_F_PSP] = gd_wave_sum(acc.psp);
+  sums[GD_F_KDE] = ekde;
+  sums[GD_F_LHM] = elhm;
+  gd_finalize(sums, (float) lna, P.a_para, P.b_para, P.w, P.term_order, out);
+}
+
+/* Move the conformer held by this lane (move_d.cu:86-132). */
+__device__ __forceinline__ void gd_place(const gd_pack &P, const float m[6],
+                                         float bx0, float by0, float bz0,
+                                         float bx1, float by1, float bz1,
+                                         float *ax0, float *ay0, float *az0,
+                                         float *ax1, float *ay1, float *az1,
+                                         float cen[3])
+{
+  float rot[9];
+  gd_rotation(m, rot);
+  gd_transform(rot, m, P.center, bx0, by0, bz0, ax0, ay0, az0);
+  gd_transform(rot, m, P.center, bx1, by1, bz1, ax1, ay1, az1);
+  cen[0] = P.center[0] + m[0];
+  cen[1] = P.center[1] + m[1];
+  cen[2] = P.center[2] + m[2];
+}
+
+/* ------------------------------------------------------------------------ */
+/* MC kernel                                                                 */
+/* mode 0: n_steps MC steps from global step s_begin                         */
+/* mode 1: initial energy of the current movematrix                          */
+/* mode 2: initial displacement by 44.5 * move_scale, forced accept          */
+/* ------------------------------------------------------------------------ */
+template <bool KDE_LDS>
+__global__ void __launch_bounds__(GD_WAVE *GD_WPG)
+gd_mc_kernel(gd_pack P, gd_state S, const gd_task *__restrict__ tasks,
+             const int *__restrict__ rep_lig, gd_lds_layout L, int s_begin,
+             int n_steps, uint32_t seed, uint32_t cid, int mode)
+{
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const gd_task task = tasks[blockIdx.x];
+  gd_stage<KDE_LDS>(P, task.prt, smem, L);
+  __syncthreads();
+
+  const int w = threadIdx.x / GD_WAVE;
+  if (w >= task.r_count)
+    return;  /* no barrier follows */
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int r = task.r_begin + w;
+
+  const float4 *s_pos = (const float4 *) smem;
+  const float2 *s_eh = (const float2 *) (smem + L.off_eh);
+  const gd_pair_param *s_tab = (const gd_pair_param *) (smem + L.off_tab);
+  const float4 *kde = KDE_LDS ? (const float4 *) (smem + L.off_kde) : P.kde;
+
+  /* conformer coordinates (centroid-relative), two atoms per lane */
+  const float4 *cx = P.lig_xyz + (size_t) rep_lig[r] * P.lna;
+  float bx0 = 0.0f, by0 = 0.0f, bz0 = 0.0f, bx1 = 0.0f, by1 = 0.0f, bz1 = 0.0f;
+  if (lane < P.lna) {
+    const float4 v = cx[lane];
+    bx0 = v.x; by0 = v.y; bz0 = v.z;
+  }
+  if (lane + GD_WAVE < P.lna) {
+    const float4 v = cx[lane + GD_WAVE];
+    bx1 = v.x; by1 = v.y; bz1 = v.z;
+  }
+
+  float m_old[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    m_old[k] = S.mv[r * 8 + k];
+
+  float ax0, ay0, az0, ax1, ay1, az1, cen[3];
+  float e_new[GD_E_N];
+
+  if (mode != 0) {
+    if (mode == 2) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        m_old[k] = P.move_scale[k] * 44.5f + m_old[k];
+    }
+    gd_place(P, m_old, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1,
+             &ay1, &az1, cen);
+    gd_energy_wave(P, s_pos, s_eh, s_tab, kde, ax0, ay0, az0, ax1, ay1, az1, cen,
+                   e_new);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        S.mv[r * 8 + k] = m_old[k];
+        S.mv_best[r * 8 + k] = m_old[k];
+      }
+#pragma unroll
+      for (int i = 0; i < GD_E_N; ++i) {
+        S.e[r * 10 + i] = e_new[i];
+        S.e_best[r * 10 + i] = e_new[i];
+      }
+      S.step_best[r] = -1;
+    }
+    return;
+  }
+
+  float e_old[GD_E_N];
+#pragma unroll
+  for (int i = 0; i < GD_E_N; ++i)
+    e_old[i] = S.e[r * 10 + i];
+  float best_total = S.e_best[r * 10 + GD_E_TOTAL];
+  const float mb = P.minus_beta[S.idx_tmp[r]];
+  int acs = 0;
+
+  for (int s = s_begin; s < s_begin + n_steps; ++s) {
+    const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
+    const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
+    const uint32_t wds[7] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3], rb.v[0], rb.v[1], rb.v[2]};
+    float m_new[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      m_new[k] = P.move_scale[k] * (2.0f * gd_u01(wds[k]) - 1.0f) + m_old[k];
+
+    gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1,
+             &ay1, &az1, cen);
+    gd_energy_wave(P, s_pos, s_eh, s_tab, kde, ax0, ay0, az0, ax1, ay1, az1, cen,
+                   e_new);
+
+    const float de = e_new[GD_E_TOTAL] - e_old[GD_E_TOTAL];
+    const bool accept = gd_u01(wds[6]) < gd_exp(de * mb);
+    if (accept) {
+      ++acs;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        m_old[k] = m_new[k];
+#pragma unroll
+      for (int i = 0; i < GD_E_N; ++i)
+        e_old[i] = e_new[i];
+      if (e_new[GD_E_TOTAL] < best_total) {
+        best_total = e_new[GD_E_TOTAL];
+        if (lane == 0) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            S.mv_best[r * 8 + k] = m_new[k];
+#pragma unroll
+          for (int i = 0; i < GD_E_N; ++i)
+            S.e_best[r * 10 + i] = e_new[i];
+          S.step_best[r] = s;
+        }
+      }
+    }
+  }
+
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      S.mv[r * 8 + k] = m_old[k];
+#pragma unroll
+    for (int i = 0; i < GD_E_N; ++i)
+      S.e[r * 10 + i] = e_old[i];
+    S.acs_mc[r] += acs;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Temperature exchange (exchangereplicas_d.cu:28-78), one lane per chain.  */
+/* ------------------------------------------------------------------------ */
+#define GD_MAXTMP_DEV 64
+
+__global__ void __launch_bounds__(256)
+gd_exchange_kernel(gd_pack P, gd_state S, int x, uint32_t seed, uint32_t cid)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_chain = P.n_prt * P.n_lig;
+  if (c >= n_chain)
+    return;
+  const int p = c / P.n_lig;
+  const int l = c - p * P.n_lig;
+  const int n_tmp = P.n_tmp;
+  const int base = n_tmp * P.n_lig * p + l;
+  int temps[GD_MAXTMP_DEV];
+  int orders[GD_MAXTMP_DEV];
+  float ener[GD_MAXTMP_DEV];
+  for (int t = 0; t < n_tmp; ++t) {
+    const int a = base + P.n_lig * t;
+    temps[t] = S.idx_tmp[a];
+    ener[t] = S.e[a * 10 + GD_E_TOTAL];
+    orders[temps[t]] = t;
+  }
+  const int mode = x & 1;
+  const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && mode == 1 ? 1 : 0);
+  gd_u32x4 u4;
+  for (int j = 0; j < maxpair; ++j) {
+    if ((j & 3) == 0)
+      u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2), seed, cid);
+    const int i1 = 2 * j + mode;
+    const int i2 = i1 + 1;
+    const int o1 = orders[i1];
+    const int o2 = orders[i2];
+    const float delta = (P.minus_beta[i1] - P.minus_beta[i2]) * (ener[o2] - ener[o1]);
+    if (gd_exp(delta) > gd_u01(u4.v[j & 3])) {
+      const int tt = temps[o1];
+      temps[o1] = temps[o2];
+      temps[o2] = tt;
+      S.acs_x[base + P.n_lig * o1] += 1;
+      S.acs_x[base + P.n_lig * o2] += 1;
+    }
+  }
+  for (int t = 0; t < n_tmp; ++t)
+    S.idx_tmp[base + P.n_lig * t] = temps[t];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energies of given poses (FP32).  Task r range indexes poses.             */
+/* ------------------------------------------------------------------------ */
+template <bool KDE_LDS>
+__global__ void __launch_bounds__(GD_WAVE *GD_WPG)
+gd_score_kernel(gd_pack P, const gd_task *__restrict__ tasks,
+                const int *__restrict__ pose_ids, const int *__restrict__ lig_conf,
+                const float *__restrict__ move, gd_lds_layout L,
+                float *__restrict__ energy)
+{
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const gd_task task = tasks[blockIdx.x];
+  gd_stage<KDE_LDS>(P, task.prt, smem, L);
+  __syncthreads();
+  const int w = threadIdx.x / GD_WAVE;
+  if (w >= task.r_count)
+    return;
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int i = pose_ids[task.r_begin + w];
+  const float4 *s_pos = (const float4 *) smem;
+  const float2 *s_eh = (const float2 *) (smem + L.off_eh);
+  const gd_pair_param *s_tab = (const gd_pair_param *) (smem + L.off_tab);
+  const float4 *kde = KDE_LDS ? (const float4 *) (smem + L.off_kde) : P.kde;
+
+  const float4 *cx = P.lig_xyz + (size_t) lig_conf[i] * P.lna;
+  float bx0 = 0.0f, by0 = 0.0f, bz0 = 0.0f, bx1 = 0.0f, by1 = 0.0f, bz1 = 0.0f;
+  if (lane < P.lna) {
+    const float4 v = cx[lane];
+    bx0 = v.x; by0 = v.y; bz0 = v.z;
+  }
+  if (lane + GD_WAVE < P.lna) {
+    const float4 v = cx[lane + GD_WAVE];
+    bx1 = v.x; by1 = v.y; bz1 = v.z;
+  }
+  float m[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    m[k] = move[i * 6 + k];
+  float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N];
+  gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
+           &az1, cen);
+  gd_energy_wave(P, s_pos, s_eh, s_tab, kde, ax0, ay0, az0, ax1, ay1, az1, cen, e);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < GD_E_N; ++k)
+      energy[i * GD_E_N + k] = e[k];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 re-score: the reference formulas (calcenergy_d.cu:69-309) in double, */
+/* one wavefront per pose, reading the pack straight from HBM.              */
+/* ------------------------------------------------------------------------ */
+__global__ void __launch_bounds__(GD_WAVE)
+gd_rescore64_kernel(gd_pack P, const int *__restrict__ lig_conf,
+                    const int *__restrict__ prt_conf, const float *__restrict__ move,
+                    int n_pose, double *__restrict__ energy)
+{
+  const int i = blockIdx.x;
+  if (i >= n_pose)
+    return;
+  const int lane = threadIdx.x;
+  const int lna = P.lna, pnp = P.pnp;
+  double m[6];
+  for (int k = 0; k < 6; ++k)
+    m[k] = (double) move[i * 6 + k];
+  const double s1 = sin(m[3]), c1 = cos(m[3]);
+  const double s2 = sin(m[4]), c2 = cos(m[4]);
+  const double s3 = sin(m[5]), c3 = cos(m[5]);
+  const double R[9] = {c1 * c2, c1 * s2 * s3 - c3 * s1, s1 * s3 + c1 * c3 * s2,
+                       c2 * s1, c1 * c3 + s1 * s2 * s3, c3 * s1 * s2 - c1 * s3,
+                       -s2,     c2 * s3,                c2 * c3};
+  const double cen[3] = {(double) P.center[0] + m[0], (double) P.center[1] + m[1],
+                         (double) P.center[2] + m[2]};
+  const float4 *cx = P.lig_xyz + (size_t) lig_conf[i] * lna;
+  const float4 *pp = P.pt_pos + (size_t) prt_conf[i] * pnp;
+  const float2 *pe = P.pt_eh + (size_t) prt_conf[i] * pnp;
+  const double lj0 = P.sc.lj0, lj1 = P.sc.lj1, el0 = P.sc.el0, el1 = P.sc.el1;
+  const double a1 = 4.0 - 3.0 * el0, b1 = 2.0 * el0 - 3.0;
+
+  double evdw = 0, eele = 0, epmf = 0, epsp = 0, ehdb = 0, ehpc = 0, ekde = 0, elhm = 0;
+  for (int l = 0; l < lna; ++l) {
+    const float4 o = cx[l];
+    const double lx = R[0] * o.x + R[1] * o.y + R[2] * o.z + cen[0];
+    const double ly = R[3] * o.x + R[4] * o.y + R[5] * o.z + cen[1];
+    const double lz = R[6] * o.x + R[7] * o.y + R[8] * o.z + cen[2];
+    const int row = P.lig_row[l];
+    const int lt = P.lig_type[l];
+    const double lq = P.lig_q[l];
+    const gd_pair_param *trow = P.tab + row * GD_MAXTP1_DEV;
+    double hpc1 = 0.0;
+    for (int p = lane; p < pnp; p += GD_WAVE) {
+      const float4 q = pp[p];
+      const float2 eh = pe[p];
+      const uint32_t bits = __float_as_uint(q.w);
+      const gd_pair_param tp = trow[bits & GD_PT_TYPE_MASK];
+      const double dx = lx - q.x, dy = ly - q.y, dz = lz - q.z;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double d = sqrt(d2);
+      if ((bits & GD_PT_HPC) && d2 <= 81.0) {
+        const double u = d2 / 81.0;
+        hpc1 += (double) eh.y * (1.0 - (3.5 * u - 4.5 * u * u + 2.5 * u * u * u - 0.5 * u * u * u * u));
+      }
+      const double p1 = (double) tp.p1a / (d2 * d2 * d2 * d2 * d);
+      const double p2 = (double) tp.p2a / (d2 * d2 * d2);
+      const double p4 = p1 * lj0 * (1.0 + lj1 * d2) + 1.0;
+      evdw += (p1 - p2) / p4;
+      const double s = el1 * d;
+      const double g = s < 1.0 ? el0 + a1 * s * s + b1 * s * s * s : 1.0 / s;
+      eele += lq * (double) eh.x * g;
+      const double dm = d - (double) tp.pmf0;
+      epmf += (double) tp.pmf1 / (1.0 + exp((-0.5 * d + 6.0) * dm));
+      if ((bits & GD_PT_C2) && dm <= 0.0)
+        epsp += (double) P.psp[(size_t) lt * P.n_res + (bits >> GD_PT_SEQ_SHIFT)];
+      if (tp.hdb0 > 0.1f) {
+        const double h = (d - (double) tp.hdb0) * (double) tp.hdb1;
+        ehdb += (double) tp.hdb1 * exp(-0.5 * h * h);
+      }
+    }
+    hpc1 = gd_wave_sum64(hpc1);
+    const double z = (hpc1 - (double) P.hpl0[lt]) / (double) P.hpl1[lt];
+    ehpc += 0.5 * z * z - (double) P.hpl2[lt];
+
+    const int k0 = P.kde_seg[2 * row], k1 = P.kde_seg[2 * row + 1];
+    double v = 0.0;
+    for (int k = k0 + lane; k < k1; k += GD_WAVE) {
+      const float4 q = P.kde[k];
+      const double ex = lx - q.x, ey = ly - q.y, ez = lz - q.z;
+      v += exp((double) P.kde2 * (ex * ex + ey * ey + ez * ez));
+    }
+    v = gd_wave_sum64(v);
+    if (k1 > k0)
+      ekde += v / (double) (k1 - k0);
+  }
+  evdw = gd_wave_sum64(evdw);
+  eele = gd_wave_sum64(eele);
+  epmf = gd_wave_sum64(epmf);
+  epsp = gd_wave_sum64(epsp);
+  ehdb = gd_wave_sum64(ehdb);
+  ekde = ekde / (double) P.kde3;
+
+  for (int mm = 0; mm < P.pos; ++mm) {
+    double s = 0.0, c = 0.0;
+    for (int l = lane; l < lna; l += GD_WAVE) {
+      const float4 t = P.mcs[(size_t) mm * lna + l];
+      if (t.w != 0.0f) {
+        const float4 o = cx[l];
+        const double lx = R[0] * o.x + R[1] * o.y + R[2] * o.z + cen[0];
+        const double ly = R[3] * o.x + R[4] * o.y + R[5] * o.z + cen[1];
+        const double lz = R[6] * o.x + R[7] * o.y + R[8] * o.z + cen[2];
+        const double dx = lx - t.x, dy = ly - t.y, dz = lz - t.z;
+        s += dx * dx + dy * dy + dz * dz;
+        c += 1.0;
+      }
+    }
+    s = gd_wave_sum64(s);
+    c = gd_wave_sum64(c);
+    if (c > 0.0)
+      elhm += (double) P.mcs_tcc[mm] * sqrt(s / c);
+  }
+  elhm = P.pos > 0 ? log(elhm / (double) P.pos) : 0.0;
+  const double ddx = cen[0] - P.pocket_center[0], ddy = cen[1] - P.pocket_center[1],
+               ddz = cen[2] - P.pocket_center[2];
+  const double edst = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+
+  if (lane == 0) {
+    const double fl = (double) lna;
+    double raw[9];
+    raw[GD_F_VDW] = evdw / fl;
+    raw[GD_F_ELE] = eele / fl;
+    raw[GD_F_PMF] = epmf / fl;
+    raw[GD_F_HPC] = ehpc / fl;
+    raw[GD_F_HDB] = -ehdb / fl / sqrt(2.0 * 3.14159265358979323846);
+    raw[GD_F_DST] = edst;
+    raw[GD_F_PSP] = epsp / fl;
+    raw[GD_F_KDE] = ekde / fl;
+    raw[GD_F_LHM] = elhm;
+    const int f2s[9] = GD_FILE_TO_SLOT_INIT;
+    double slot[10];
+    for (int k = 0; k < 9; ++k)
+      slot[f2s[k]] = raw[k];
+    double total = 0.0;
+    if (P.term_order == 0) {
+      for (int k = 0; k < 9; ++k) {
+        const double v = (double) P.a_para[k] * raw[k] + (double) P.b_para[k];
+        slot[f2s[k]] = v;
+        total += (double) P.w[k] * v;
+      }
+    } else {
+      for (int k = 0; k < 9; ++k) {
+        const double v = (double) P.a_para[k] * slot[k] + (double) P.b_para[k];
+        slot[k] = v;
+        total += (double) P.w[k] * v;
+      }
+    }
+    slot[9] = total;
+    for (int k = 0; k < 10; ++k)
+      energy[(size_t) i * 10 + k] = slot[k];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* host-callable launch wrappers                                             */
+/* ------------------------------------------------------------------------ */
+#include "gd_launch.h"
+
+hipError_t gd_launch_mc(const gd_pack &P, const gd_state &S, const gd_task *tasks,
+                        int n_tasks, const int *rep_lig, const gd_launch_cfg &C,
+                        int s_begin, int n_steps, uint32_t seed, uint32_t cid,
+                        int mode, hipStream_t st)
+{
+  gd_lds_layout L = {C.off_eh, C.off_tab, C.off_kde, C.lds_bytes};
+  if (C.kde_in_lds)
+    hipLaunchKernelGGL(gd_mc_kernel<true>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
+                       C.lds_bytes, st, P, S, tasks, rep_lig, L, s_begin, n_steps,
+                       seed, cid, mode);
+  else
+    hipLaunchKernelGGL(gd_mc_kernel<false>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
+                       C.lds_bytes, st, P, S, tasks, rep_lig, L, s_begin, n_steps,
+                       seed, cid, mode);
+  return hipGetLastError();
+}
+
+hipError_t gd_launch_exchange(const gd_pack &P, const gd_state &S, int x,
+                              uint32_t seed, uint32_t cid, hipStream_t st)
+{
+  const int n_chain = P.n_prt * P.n_lig;
+  const int blocks = (n_chain + 255) / 256;
+  hipLaunchKernelGGL(gd_exchange_kernel, dim3(blocks), dim3(256), 0, st, P, S, x,
+                     seed, cid);
+  return hipGetLastError();
+}
+
+hipError_t gd_launch_score(const gd_pack &P, const gd_task *tasks, int n_tasks,
+                           const int *pose_ids, const int *lig_conf,
+                           const float *move, const gd_launch_cfg &C,
+                           float *energy, hipStream_t st)
+{
+  gd_lds_layout L = {C.off_eh, C.off_tab, C.off_kde, C.lds_bytes};
+  if (C.kde_in_lds)
+    hipLaunchKernelGGL(gd_score_kernel<true>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
+                       C.lds_bytes, st, P, tasks, pose_ids, lig_conf, move, L, energy);
+  else
+    hipLaunchKernelGGL(gd_score_kernel<false>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
+                       C.lds_bytes, st, P, tasks, pose_ids, lig_conf, move, L, energy);
+  return hipGetLastError();
+}
+
+hipError_t gd_launch_rescore64(const gd_pack &P, const int *lig_conf,
+                               const int *prt_conf, const float *move, int n_pose,
+                               double *energy, hipStream_t st)
+{
+  if (n_pose <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(gd_rescore64_kernel, dim3(n_pose), dim3(GD_WAVE), 0, st, P,
+                     lig_conf, prt_conf, move, n_pose, energy);
+  return hipGetLastError();
+}
+
+hipError_t gd_set_kernel_attrs(int lds_bytes)
+{
+  hipError_t e;
+  e = hipFuncSetAttribute((const void *) gd_mc_kernel<true>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *) gd_mc_kernel<false>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *) gd_score_kernel<true>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *) gd_score_kernel<false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+}
+
+int gd_waves_per_group(void) { return GD_WPG; }

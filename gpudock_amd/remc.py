@@ -1,0 +1,411 @@
+"""ctypes mirror of include/gd_remc.h and thin Python helpers.
+
+The product is the C ABI in libgdremc.so; this module only marshals numpy
+arrays into the ABI structs for tests, the benchmark and Python callers.
+Every compute call goes to the HIP library; when the library is missing the
+import of `lib()` raises instead of falling back to anything.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgdremc.so")
+
+MAXTP1, MAXTP2, MAXTP3, MAXTP4, MAXWEI = 30, 24, 50, 20, 10
+MAX_LNA = 128
+MCS_INVALID = 728492.0
+TERM_ORDER_FILE, TERM_ORDER_SLOT = 0, 1
+
+# Energy slots (reference Energy.e[] order, dock.h:100-115)
+SLOTS = ("vdw", "ele", "pmf", "psp", "hdb", "hpc", "kde", "lhm", "dst", "total")
+# Parameter-file order (weights, nor_a, nor_b): vdw ele pmf hpc hdb dst psp kde lhm
+FILE_ORDER = ("vdw", "ele", "pmf", "hpc", "hdb", "dst", "psp", "kde", "lhm")
+FILE_TO_SLOT = [SLOTS.index(n) for n in FILE_ORDER]
+
+F = C.c_float
+FP = C.POINTER(C.c_float)
+IP = C.POINTER(C.c_int)
+
+
+class ComplexSize(C.Structure):
+    _fields_ = [(n, C.c_int) for n in
+                ("n_lig", "n_prt", "n_tmp", "n_rep", "lna", "pnp", "pnk", "pos")]
+
+
+class Ligand(C.Structure):
+    _fields_ = [("x", FP), ("y", FP), ("z", FP), ("t", IP), ("c", FP), ("n", IP),
+                ("center", F * 3)]
+
+
+class Protein(C.Structure):
+    _fields_ = [("x", FP), ("y", FP), ("z", FP), ("t", IP), ("c", IP), ("ele", FP),
+                ("seq3r", IP), ("c0_and_d12_or_c2", IP), ("hpp", FP),
+                ("pocket_center", F * 3)]
+
+
+class Psp(C.Structure):
+    _fields_ = [("psp", FP), ("n_res", C.c_int)]
+
+
+class Kde(C.Structure):
+    _fields_ = [("x", FP), ("y", FP), ("z", FP), ("t", IP)]
+
+
+class Mcs(C.Structure):
+    _fields_ = [("x", FP), ("y", FP), ("z", FP), ("tcc", FP), ("n_field", C.c_int)]
+
+
+T2x1 = F * MAXTP1 * MAXTP2
+
+
+class EnePara(C.Structure):
+    _fields_ = [("p1a", T2x1), ("p2a", T2x1), ("lj0", F), ("lj1", F), ("el0", F),
+                ("el1", F), ("ele", F * MAXTP3), ("a1", F), ("b1", F),
+                ("pmf0", T2x1), ("pmf1", T2x1), ("hdb0", T2x1), ("hdb1", T2x1),
+                ("hdb2", T2x1), ("hpp", F * MAXTP4), ("hpl0", F * MAXTP2),
+                ("hpl1", F * MAXTP2), ("hpl2", F * MAXTP2), ("kde2", F), ("kde3", F),
+                ("w", F * MAXWEI), ("a_para", F * MAXWEI), ("b_para", F * MAXWEI)]
+
+
+T1x2x2 = F * 2 * MAXTP2 * MAXTP1
+
+
+class EnePara0(C.Structure):
+    _fields_ = [("vdw", T1x2x2), ("ele", F * MAXTP3), ("pmf", T1x2x2),
+                ("hpp", F * MAXTP4), ("hpl", F * 2 * MAXTP2), ("hdb", T1x2x2),
+                ("lj", F * 3), ("el", F * 2), ("kde", F), ("w", F * MAXWEI),
+                ("a_para", F * MAXWEI), ("b_para", F * MAXWEI)]
+
+
+class Temp(C.Structure):
+    _fields_ = [("t", F), ("minus_beta", F), ("order", C.c_int)]
+
+
+class Replica(C.Structure):
+    _fields_ = [("idx_rep", C.c_int), ("idx_prt", C.c_int), ("idx_tmp", C.c_int),
+                ("idx_lig", C.c_int)]
+
+
+class McPara(C.Structure):
+    _fields_ = [("steps_total", C.c_int), ("steps_per_dump", C.c_int),
+                ("steps_per_exchange", C.c_int), ("move_scale", F * 6)]
+
+
+class McLog(C.Structure):
+    _fields_ = [("t0", C.c_double), ("t1", C.c_double), ("t2", C.c_double),
+                ("ac_mc", C.c_longlong), ("acs_mc", IP), ("ac_temp_exchg", C.c_longlong),
+                ("acs_temp_exchg", IP)]
+
+
+class Options(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("complex_id", C.c_uint32), ("exchange", C.c_int),
+                ("away", C.c_int), ("term_order", C.c_int), ("device", C.c_int)]
+
+
+class ReplicaResult(C.Structure):
+    _fields_ = [("move_final", F * 6), ("energy_final", F * MAXWEI),
+                ("move_best", F * 6), ("energy_best", F * MAXWEI),
+                ("energy_best_fp64", C.c_double * MAXWEI), ("step_best", C.c_int),
+                ("idx_tmp_final", C.c_int), ("acs_mc", C.c_int), ("acs_exchg", C.c_int)]
+
+
+_LIB = None
+
+SYMBOLS = {
+    "gd_run": (C.c_int, [C.POINTER(Ligand), C.POINTER(Protein), C.POINTER(Psp),
+                         C.POINTER(Kde), C.POINTER(Mcs), C.POINTER(EnePara),
+                         C.POINTER(Temp), C.POINTER(Replica), C.POINTER(McPara),
+                         C.POINTER(McLog), ComplexSize, C.POINTER(Options),
+                         C.POINTER(ReplicaResult)]),
+    "gd_score_poses": (C.c_int, [C.POINTER(Ligand), C.POINTER(Protein), C.POINTER(Psp),
+                                 C.POINTER(Kde), C.POINTER(Mcs), C.POINTER(EnePara),
+                                 ComplexSize, C.c_int, IP, IP, FP, C.c_int, C.c_int, FP,
+                                 C.POINTER(C.c_double)]),
+    "gd_session_create": (C.c_int, [C.POINTER(Ligand), C.POINTER(Protein),
+                                    C.POINTER(Psp), C.POINTER(Kde), C.POINTER(Mcs),
+                                    C.POINTER(EnePara), C.POINTER(Temp),
+                                    C.POINTER(Replica), C.POINTER(McPara), ComplexSize,
+                                    C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "gd_session_init": (C.c_int, [C.c_void_p]),
+    "gd_session_cycles": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_session_sync": (C.c_int, [C.c_void_p]),
+    "gd_session_steps_done": (C.c_longlong, [C.c_void_p]),
+    "gd_session_results": (C.c_int, [C.c_void_p, C.POINTER(McLog),
+                                     C.POINTER(ReplicaResult)]),
+    "gd_session_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_session_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]),
+    "gd_session_layout": (C.c_int, [C.c_void_p, IP, IP, IP, IP]),
+    "gd_session_destroy": (None, [C.c_void_p]),
+    "gd_optimize_enepara": (C.c_int, [C.POINTER(EnePara0), C.POINTER(EnePara)]),
+    "gd_set_temperature": (C.c_int, [C.POINTER(Temp), C.c_int, C.c_float, C.c_float]),
+    "gd_set_replica": (C.c_int, [C.POINTER(Replica), ComplexSize]),
+    "gd_load_enepara": (C.c_int, [C.c_char_p, C.POINTER(EnePara0)]),
+    "gd_load_weight": (C.c_int, [C.c_char_p, C.POINTER(EnePara0)]),
+    "gd_load_norpara": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(EnePara0)]),
+    "gd_last_error": (C.c_char_p, []),
+    "gd_device_count": (C.c_int, []),
+    "gd_version": (C.c_char_p, []),
+}
+
+
+def lib():
+    """Load libgdremc.so (built in-tree); raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run python -c 'import __graft_entry__ as g; "
+                "g.build()' (or python gpudock_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SYMBOLS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class GDError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise GDError(f"gd status {rc}: {lib().gd_last_error().decode()}")
+    return rc
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(FP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP)
+
+
+class Complex:
+    """One prepared complex as numpy arrays (the inputs of Run()).
+
+    Fields (shapes):
+      lig_xyz [n_lig, lna, 3] centroid-relative conformers; lig_t, lig_c, lig_n [lna];
+      lig_center [3]; prt_xyz [n_prt, pnp, 3]; prt_t, prt_c, prt_seq3r, prt_hpcflag
+      [n_prt, pnp] int; prt_ele, prt_hpp [n_prt, pnp]; pocket_center [3];
+      psp [24, n_res]; kde_xyz [pnk, 3]; kde_t [pnk]; mcs_xyz [pos, n_field, 3]
+      (MCS_INVALID where absent); mcs_tcc [pos]; enepara (EnePara ctypes struct).
+    """
+
+    def __init__(self, **kw):
+        self.lig_xyz = _f32(kw["lig_xyz"])
+        self.lig_t = _i32(kw["lig_t"])
+        self.lig_c = _f32(kw["lig_c"])
+        self.lig_n = _i32(kw.get("lig_n", np.arange(len(self.lig_t))))
+        self.lig_center = _f32(kw["lig_center"])
+        self.prt_xyz = _f32(kw["prt_xyz"])
+        self.prt_t = _i32(kw["prt_t"])
+        self.prt_c = _i32(kw["prt_c"])
+        self.prt_ele = _f32(kw["prt_ele"])
+        self.prt_seq3r = _i32(kw["prt_seq3r"])
+        self.prt_hpcflag = _i32(kw["prt_hpcflag"])
+        self.prt_hpp = _f32(kw["prt_hpp"])
+        self.pocket_center = _f32(kw["pocket_center"])
+        self.psp = _f32(kw["psp"])
+        self.kde_xyz = _f32(kw["kde_xyz"]).reshape(-1, 3)
+        self.kde_t = _i32(kw["kde_t"])
+        self.mcs_xyz = _f32(kw["mcs_xyz"])
+        self.mcs_tcc = _f32(kw["mcs_tcc"])
+        self.enepara = kw["enepara"]
+        self._keep = []
+        self._build()
+
+    @property
+    def size(self):
+        s = ComplexSize()
+        s.n_lig, s.lna = self.lig_xyz.shape[0], self.lig_xyz.shape[1]
+        s.n_prt, s.pnp = self.prt_xyz.shape[0], self.prt_xyz.shape[1]
+        s.pnk = self.kde_xyz.shape[0]
+        s.pos = self.mcs_xyz.shape[0]
+        s.n_tmp = 1
+        s.n_rep = s.n_lig * s.n_prt
+        return s
+
+    def sized(self, n_tmp):
+        s = self.size
+        s.n_tmp = n_tmp
+        s.n_rep = s.n_lig * s.n_prt * n_tmp
+        return s
+
+    def _build(self):
+        k = self._keep
+        lx = _f32(self.lig_xyz[..., 0].ravel())
+        ly = _f32(self.lig_xyz[..., 1].ravel())
+        lz = _f32(self.lig_xyz[..., 2].ravel())
+        k += [lx, ly, lz]
+        self.c_lig = Ligand(_fp(lx), _fp(ly), _fp(lz), _ip(self.lig_t), _fp(self.lig_c),
+                            _ip(self.lig_n), (F * 3)(*self.lig_center))
+        px = _f32(self.prt_xyz[..., 0].ravel())
+        py = _f32(self.prt_xyz[..., 1].ravel())
+        pz = _f32(self.prt_xyz[..., 2].ravel())
+        k += [px, py, pz]
+        self.c_prt = Protein(_fp(px), _fp(py), _fp(pz), _ip(self.prt_t), _ip(self.prt_c),
+                             _fp(self.prt_ele), _ip(self.prt_seq3r),
+                             _ip(self.prt_hpcflag), _fp(self.prt_hpp),
+                             (F * 3)(*self.pocket_center))
+        assert self.psp.shape[0] == MAXTP2
+        self.c_psp = Psp(_fp(self.psp), self.psp.shape[1])
+        kx = _f32(self.kde_xyz[:, 0])
+        ky = _f32(self.kde_xyz[:, 1])
+        kz = _f32(self.kde_xyz[:, 2])
+        k += [kx, ky, kz]
+        self.c_kde = Kde(_fp(kx), _fp(ky), _fp(kz), _ip(self.kde_t))
+        nf = self.mcs_xyz.shape[1] if self.mcs_xyz.ndim == 3 else 1
+        mx = _f32(self.mcs_xyz[..., 0].ravel()) if self.mcs_xyz.size else np.zeros(1, np.float32)
+        my = _f32(self.mcs_xyz[..., 1].ravel()) if self.mcs_xyz.size else np.zeros(1, np.float32)
+        mz = _f32(self.mcs_xyz[..., 2].ravel()) if self.mcs_xyz.size else np.zeros(1, np.float32)
+        tcc = self.mcs_tcc if self.mcs_tcc.size else np.zeros(1, np.float32)
+        k += [mx, my, mz, tcc]
+        self.c_mcs = Mcs(_fp(mx), _fp(my), _fp(mz), _fp(tcc), nf)
+
+    def args(self):
+        return (C.byref(self.c_lig), C.byref(self.c_prt), C.byref(self.c_psp),
+                C.byref(self.c_kde), C.byref(self.c_mcs), C.byref(self.enepara))
+
+
+def temperatures(n_tmp, floor, ceiling):
+    """SetTemperature (util.C:490-532) through the library."""
+    arr = (Temp * n_tmp)()
+    check(lib().gd_set_temperature(arr, n_tmp, floor, ceiling))
+    return arr
+
+
+def load_enepara0(ff_path, weight_path=None, nor_a=None, nor_b=None):
+    ep0 = EnePara0()
+    check(lib().gd_load_enepara(ff_path.encode(), C.byref(ep0)))
+    if weight_path:
+        check(lib().gd_load_weight(weight_path.encode(), C.byref(ep0)))
+    if nor_a and nor_b:
+        check(lib().gd_load_norpara(nor_a.encode(), nor_b.encode(), C.byref(ep0)))
+    return ep0
+
+
+def optimize_enepara(ep0):
+    ep = EnePara()
+    check(lib().gd_optimize_enepara(C.byref(ep0), C.byref(ep)))
+    return ep
+
+
+def default_options(seed=1, complex_id=0, exchange=1, away=0, term_order=0, device=0):
+    return Options(seed, complex_id, exchange, away, term_order, device)
+
+
+def mcpara(steps_total, steps_per_exchange, move_scale, steps_per_dump=1000):
+    return McPara(steps_total, steps_per_dump, steps_per_exchange, (F * 6)(*move_scale))
+
+
+def results_to_numpy(res):
+    n = len(res)
+    out = {
+        "move_final": np.array([list(r.move_final) for r in res], np.float32).reshape(n, 6),
+        "energy_final": np.array([list(r.energy_final) for r in res], np.float32).reshape(n, 10),
+        "move_best": np.array([list(r.move_best) for r in res], np.float32).reshape(n, 6),
+        "energy_best": np.array([list(r.energy_best) for r in res], np.float32).reshape(n, 10),
+        "energy_best_fp64": np.array([list(r.energy_best_fp64) for r in res]).reshape(n, 10),
+        "step_best": np.array([r.step_best for r in res], np.int32),
+        "idx_tmp_final": np.array([r.idx_tmp_final for r in res], np.int32),
+        "acs_mc": np.array([r.acs_mc for r in res], np.int32),
+        "acs_exchg": np.array([r.acs_exchg for r in res], np.int32),
+    }
+    return out
+
+
+def run(cx, temps, mc, opt=None, n_tmp=None):
+    """gd_run (drop-in for Run()); returns (results dict, McLog)."""
+    n_tmp = n_tmp or len(temps)
+    size = cx.sized(n_tmp)
+    res = (ReplicaResult * size.n_rep)()
+    acs = np.zeros(size.n_rep, np.int32)
+    acx = np.zeros(size.n_rep, np.int32)
+    log = McLog(0, 0, 0, 0, _ip(acs), 0, _ip(acx))
+    opt = opt or default_options()
+    check(lib().gd_run(*cx.args(), temps, None, C.byref(mc), C.byref(log), size,
+                       C.byref(opt), res))
+    return results_to_numpy(res), log
+
+
+def score_poses(cx, lig_conf, prt_conf, moves, term_order=0, device=0, fp64=True):
+    """gd_score_poses: FP32 canonical energies (and FP64) of poses, slot order."""
+    lig_conf = _i32(lig_conf)
+    prt_conf = _i32(prt_conf)
+    moves = _f32(moves).reshape(-1, 6)
+    n = len(lig_conf)
+    e = np.zeros((n, 10), np.float32)
+    e64 = np.zeros((n, 10), np.float64) if fp64 else None
+    check(lib().gd_score_poses(
+        *cx.args(), cx.size, n, _ip(lig_conf), _ip(prt_conf), _fp(moves), term_order,
+        device, _fp(e), e64.ctypes.data_as(C.POINTER(C.c_double)) if fp64 else None))
+    return (e, e64) if fp64 else e
+
+
+class Session:
+    """gd_session_*: complex and replica state resident on the GPU."""
+
+    def __init__(self, cx, temps, mc, opt=None):
+        self.cx, self.temps, self.mc = cx, temps, mc
+        self.opt = opt or default_options()
+        self.size = cx.sized(len(temps))
+        h = C.c_void_p()
+        check(lib().gd_session_create(*cx.args(), temps, None, C.byref(mc), self.size,
+                                      C.byref(self.opt), C.byref(h)))
+        self.h = h
+
+    def init(self):
+        check(lib().gd_session_init(self.h))
+
+    def cycles(self, n):
+        check(lib().gd_session_cycles(self.h, n))
+
+    def sync(self):
+        check(lib().gd_session_sync(self.h))
+
+    def set_timing(self, on):
+        check(lib().gd_session_set_timing(self.h, int(on)))
+
+    def timing(self):
+        a, b = C.c_double(), C.c_int()
+        c, d = C.c_double(), C.c_int()
+        check(lib().gd_session_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return a.value, b.value, c.value, d.value
+
+    def layout(self):
+        v = [C.c_int() for _ in range(4)]
+        check(lib().gd_session_layout(self.h, *[C.byref(x) for x in v]))
+        return {"lds_bytes": v[0].value, "kde_in_lds": v[1].value,
+                "n_tasks": v[2].value, "waves_per_group": v[3].value}
+
+    def results(self):
+        res = (ReplicaResult * self.size.n_rep)()
+        log = McLog()
+        check(lib().gd_session_results(self.h, C.byref(log), res))
+        return results_to_numpy(res), log
+
+    def close(self):
+        if self.h:
+            lib().gd_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

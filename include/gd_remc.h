@@ -1,0 +1,300 @@
+/*
+ * gd_remc.h -- C ABI of the MI355X replica-exchange Monte Carlo docking core.
+ *
+ * The plain per-complex entry point gd_run() replaces the reference's
+ *
+ *   void Run (const Ligand *, const Protein *, const Psp *, const Kde *,
+ *             const Mcs *, const EnePara *, const Temp *, const Replica *,
+ *             const McPara *, McLog *, const ComplexSize);
+ *
+ * (gpudocksm-rem-1.2/src/run.h:6-17, body run.cu:40-352).  The structs below
+ * carry the same fields as the reference's prepared host structs
+ * (gpudocksm-rem-1.2/src/dock.h), with pointer + count in place of the
+ * fixed capacities of size.h, so that every size the reference can hold
+ * and the larger configurations beyond MAXLIG/MAXREP fit.
+ *
+ * Conventions
+ *   - Plain pointers to host memory; nothing here takes ownership.
+ *   - Every function returns GD_OK (0) or a negative gd_status; the text of
+ *     the last error of the calling thread is gd_last_error().  Nothing
+ *     calls exit(), unlike the reference (cutil_yeah.h:26-42).
+ *   - Energies are float[GD_E_N] in the reference's Energy.e[] slot order
+ *     (dock.h:100-115): vdw ele pmf psp hdb hpc kde lhm dst total.
+ *   - Weights, normalisation a/b and the parameter files are in file order
+ *     vdw ele pmf hpc hdb dst psp kde lhm (see gd_options.term_order).
+ */
+#ifndef GD_REMC_H
+#define GD_REMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* capacities of the type tables (size.h:52-70) */
+#define GD_MAXTP1 30   /* protein effective-point types */
+#define GD_MAXTP2 24   /* ligand atom types */
+#define GD_MAXTP3 50   /* electrostatic point classes */
+#define GD_MAXTP4 20   /* residue types */
+#define GD_MAXWEI 10   /* energy terms + total */
+#define GD_MAX_LNA 128 /* ligand atoms per conformer held by one wavefront */
+#define GD_MCS_INVALID_COORD 728492.0f  /* size.h:97 */
+
+enum gd_status {
+  GD_OK = 0,
+  GD_ERR_ARG = -1,       /* invalid argument or size */
+  GD_ERR_HIP = -2,       /* HIP runtime error */
+  GD_ERR_NOMEM = -3,     /* device allocation failed / admission refused */
+  GD_ERR_NODEV = -4,     /* no usable GPU */
+  GD_ERR_IO = -5,        /* file could not be read or parsed */
+  GD_ERR_CANCELLED = -6  /* job cancelled (serving) */
+};
+
+enum gd_energy_slot {
+  GD_E_VDW_SLOT = 0, GD_E_ELE_SLOT, GD_E_PMF_SLOT, GD_E_PSP_SLOT, GD_E_HDB_SLOT,
+  GD_E_HPC_SLOT, GD_E_KDE_SLOT, GD_E_LHM_SLOT, GD_E_DST_SLOT, GD_E_TOTAL_SLOT
+};
+
+enum gd_term_order {
+  GD_TERM_ORDER_FILE = 0, /* parameter k applies to file-order term k */
+  GD_TERM_ORDER_SLOT = 1  /* rem-1.2 behaviour: parameter k applies to slot k
+                             (calcenergy_d.cu:301-309,349-358) */
+};
+
+/* ComplexSize, dock.h:77-90 */
+typedef struct gd_complex_size {
+  int n_lig, n_prt, n_tmp, n_rep;   /* n_rep = n_lig * n_prt * n_tmp */
+  int lna, pnp, pnk, pos;
+} gd_complex_size;
+
+/* Ligand ensemble: the used fields of Ligand (dock.h:191-216) after
+ * OptimizeLigand (util.C:199-226).  Coordinates are centroid-relative
+ * (coord_orig), conformer-major: atom a of conformer i at [i * lna + a].
+ * t, c and n are shared by all conformers, as the loader makes them
+ * (load.C:296-330). */
+typedef struct gd_ligand {
+  const float *x, *y, *z;  /* [n_lig * lna] */
+  const int *t;            /* [lna] atom type, 0 .. GD_MAXTP2-1 */
+  const float *c;          /* [lna] partial charge */
+  const int *n;            /* [lna] atom number; MCS field index is n + 1 */
+  float center[3];         /* coord_orig.center = pocket centre */
+} gd_ligand;
+
+/* Protein ensemble: Protein (dock.h:254-274) after OptimizeProtein
+ * (util.C:252-329).  Every array is [n_prt * pnp], conformer-major. */
+typedef struct gd_protein {
+  const float *x, *y, *z;
+  const int *t;                 /* point type, 0 .. GD_MAXTP1-1 */
+  const int *c;                 /* point class */
+  const float *ele;             /* ele_tab[t == 0 ? d + 30 : t] */
+  const int *seq3r;             /* residue index into the PSP table */
+  const int *c0_and_d12_or_c2;  /* (c == 0 && d == 12) || c == 2 */
+  const float *hpp;             /* hpp_tab[d] */
+  float pocket_center[3];
+} gd_protein;
+
+/* Psp (dock.h:289-295): psp[t * n_res + seq3r] for ligand type t. */
+typedef struct gd_psp {
+  const float *psp;
+  int n_res;
+} gd_psp;
+
+/* Kde (dock.h:317-325), [pnk] each */
+typedef struct gd_kde {
+  const float *x, *y, *z;
+  const int *t;
+} gd_kde;
+
+/* Mcs[pos] (dock.h:347-354): field f of restraint m at [m * n_field + f],
+ * GD_MCS_INVALID_COORD where absent (load.C:1014-1020). */
+typedef struct gd_mcs {
+  const float *x, *y, *z;
+  const float *tcc;   /* [pos] */
+  int n_field;
+} gd_mcs;
+
+/* EnePara (dock.h:386-421), the tables OptimizeEnepara builds
+ * (util.C:404-458), indexed [ligand type][point type]. */
+typedef struct gd_enepara {
+  float p1a[GD_MAXTP2][GD_MAXTP1];
+  float p2a[GD_MAXTP2][GD_MAXTP1];
+  float lj0, lj1;
+  float el0, el1;
+  float ele[GD_MAXTP3];
+  float a1, b1;
+  float pmf0[GD_MAXTP2][GD_MAXTP1];
+  float pmf1[GD_MAXTP2][GD_MAXTP1];
+  float hdb0[GD_MAXTP2][GD_MAXTP1];
+  float hdb1[GD_MAXTP2][GD_MAXTP1];
+  float hdb2[GD_MAXTP2][GD_MAXTP1];
+  float hpp[GD_MAXTP4];
+  float hpl0[GD_MAXTP2], hpl1[GD_MAXTP2], hpl2[GD_MAXTP2];
+  float kde2, kde3;
+  float w[GD_MAXWEI];
+  float a_para[GD_MAXWEI];
+  float b_para[GD_MAXWEI];
+} gd_enepara;
+
+/* EnePara0 (dock.h:363-380): raw parameter-file values, [point][ligand]. */
+typedef struct gd_enepara0 {
+  float vdw[GD_MAXTP1][GD_MAXTP2][2];
+  float ele[GD_MAXTP3];
+  float pmf[GD_MAXTP1][GD_MAXTP2][2];
+  float hpp[GD_MAXTP4];
+  float hpl[GD_MAXTP2][2];
+  float hdb[GD_MAXTP1][GD_MAXTP2][2];
+  float lj[3];
+  float el[2];
+  float kde;
+  float w[GD_MAXWEI];
+  float a_para[GD_MAXWEI];
+  float b_para[GD_MAXWEI];
+} gd_enepara0;
+
+/* Temp (dock.h:429-434) */
+typedef struct gd_temp {
+  float t;
+  float minus_beta;
+  int order;
+} gd_temp;
+
+/* Replica (dock.h:443-450); flat index n_tmp*n_lig*p + n_lig*t + l */
+typedef struct gd_replica {
+  int idx_rep, idx_prt, idx_tmp, idx_lig;
+} gd_replica;
+
+/* McPara (dock.h:460-470) without the output paths */
+typedef struct gd_mcpara {
+  int steps_total;
+  int steps_per_dump;
+  int steps_per_exchange;
+  float move_scale[6];   /* translation x y z (A), rotation x y z (rad):
+                            uniform half-widths (move_d.cu:79-84) */
+} gd_mcpara;
+
+/* McLog (dock.h:472-482).  acs_mc / acs_temp_exchg are [n_rep] arrays the
+ * caller provides (or NULL); they are accumulated into, as run.cu:284-296
+ * does.  t0 = seconds of MC compute, t1 = t0 plus result transfer. */
+typedef struct gd_mclog {
+  double t0, t1, t2;
+  long long ac_mc;
+  int *acs_mc;
+  long long ac_temp_exchg;
+  int *acs_temp_exchg;
+} gd_mclog;
+
+/* Settings the reference keeps in toggle.h or time-seeds; no counterpart
+ * argument in Run(). */
+typedef struct gd_options {
+  uint32_t seed;        /* Philox key word 0 (the reference seeds from time) */
+  uint32_t complex_id;  /* Philox key word 1 */
+  int exchange;         /* IS_EXCHANGE (toggle.h:12): swap temperatures */
+  int away;             /* IS_AWAY (toggle.h:15): start 44.5*move_scale away */
+  int term_order;       /* enum gd_term_order */
+  int device;           /* HIP device ordinal */
+} gd_options;
+
+/* Per-replica outcome (no counterpart: the reference leaves the final
+ * state on the device, montecarlo_d.cu:98-102, and never extracts a best
+ * pose, README.org:492-505). */
+typedef struct gd_replica_result {
+  float move_final[6];
+  float energy_final[GD_MAXWEI];
+  float move_best[6];
+  float energy_best[GD_MAXWEI];
+  double energy_best_fp64[GD_MAXWEI]; /* FP64 re-score of move_best */
+  int step_best;        /* MC step that produced move_best, -1 = start */
+  int idx_tmp_final;    /* temperature index held at the end */
+  int acs_mc;           /* accepted MC moves */
+  int acs_exchg;        /* accepted temperature swaps involving this slot */
+} gd_replica_result;
+
+/* ---------------------------------------------------------------------- */
+/* Plain per-complex run: drop-in for Run() (run.h:6-17).                   */
+/* results: [n_rep] or NULL.  opt: NULL = defaults (seed 1, complex 0,      */
+/* exchange on, away off, file term order, device 0).                       */
+/* ---------------------------------------------------------------------- */
+int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
+           const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *enepara,
+           const gd_temp *temp, const gd_replica *replica,
+           const gd_mcpara *mcpara, gd_mclog *mclog, gd_complex_size size,
+           const gd_options *opt, gd_replica_result *results);
+
+/* Energy of given poses on the GPU (CalcEnergy_d, calcenergy_d.cu:22-374,
+ * evaluated for caller-chosen movematrices).  pose i uses ligand conformer
+ * lig_conf[i], protein conformer prt_conf[i] and movematrix move[6*i..].
+ * energy: [n_pose * GD_MAXWEI] FP32; energy_fp64: same in FP64, or NULL. */
+int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
+                   const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                   const gd_enepara *enepara, gd_complex_size size,
+                   int n_pose, const int *lig_conf, const int *prt_conf,
+                   const float *move, int term_order, int device,
+                   float *energy, double *energy_fp64);
+
+/* ---------------------------------------------------------------------- */
+/* Session: the same run split into steps, with the complex and the replica */
+/* state resident on the GPU between calls (gd_run is create + init +       */
+/* cycles + results + destroy).  No counterpart in the reference, whose     */
+/* Run() does all of it in one call (run.cu:40-352).                        */
+/* ---------------------------------------------------------------------- */
+typedef struct gd_session gd_session;
+
+int gd_session_create(const gd_ligand *lig, const gd_protein *prt,
+                      const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                      const gd_enepara *enepara, const gd_temp *temp,
+                      const gd_replica *replica, const gd_mcpara *mcpara,
+                      gd_complex_size size, const gd_options *opt,
+                      gd_session **out);
+/* initial energies (MonteCarlo_Init_d, montecarlo_d.cu:12-55) */
+int gd_session_init(gd_session *s);
+/* n_cycles x (steps_per_exchange MC steps, then one exchange); queued on
+ * the session's stream, returns before the GPU finishes */
+int gd_session_cycles(gd_session *s, int n_cycles);
+int gd_session_sync(gd_session *s);
+/* MC steps done so far */
+long long gd_session_steps_done(const gd_session *s);
+/* copies the state back; mclog and results as in gd_run (either NULL) */
+int gd_session_results(gd_session *s, gd_mclog *mclog,
+                       gd_replica_result *results);
+/* kernel timing with HIP events on the session stream (off by default) */
+int gd_session_set_timing(gd_session *s, int on);
+int gd_session_timing(gd_session *s, double *mc_ms, int *mc_launches,
+                      double *x_ms, int *x_launches);
+/* LDS bytes per workgroup and whether KDE points are staged in LDS */
+int gd_session_layout(const gd_session *s, int *lds_bytes, int *kde_in_lds,
+                      int *n_tasks, int *waves_per_group);
+void gd_session_destroy(gd_session *s);
+
+/* ---------------------------------------------------------------------- */
+/* Host preparation (util.C:199-558)                                        */
+/* ---------------------------------------------------------------------- */
+/* OptimizeEnepara, util.C:404-458 */
+int gd_optimize_enepara(const gd_enepara0 *src, gd_enepara *dst);
+/* SetTemperature, util.C:490-532 */
+int gd_set_temperature(gd_temp *temp, int num_temp, float floor_temp,
+                       float ceiling_temp);
+/* SetReplica, util.C:537-558 (the replica table only) */
+int gd_set_replica(gd_replica *replica, gd_complex_size size);
+
+/* ---------------------------------------------------------------------- */
+/* Parameter files (load.C:1138-1419)                                       */
+/* ---------------------------------------------------------------------- */
+/* loadEnePara (load.C:1138-1324): gpudocksm.ff; zero-fills first. */
+int gd_load_enepara(const char *path, gd_enepara0 *enepara0);
+/* loadWeight (load.C:1389-1419) */
+int gd_load_weight(const char *path, gd_enepara0 *enepara0);
+/* loadNorPara (load.C:1326-1387) */
+int gd_load_norpara(const char *path_a, const char *path_b,
+                    gd_enepara0 *enepara0);
+
+/* ---------------------------------------------------------------------- */
+const char *gd_last_error(void);
+int gd_device_count(void);
+const char *gd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GD_REMC_H */

@@ -1,0 +1,814 @@
+/*
+ * gd_oracle.c -- CPU oracle of the REMC docking core.  TEST INFRASTRUCTURE:
+ * only the tests, the smoke check and the CPU-baseline leg of the benchmark
+ * load this library, as the checker; the product never links it.
+ *
+ * Three independent pieces:
+ *
+ * 1. Reference-order restatement (orc_energy_ref32 / orc_energy_ref64 /
+ *    orc_move_ref32 / orc_ref_run).  The formulas are those of the
+ *    reference kernel gpudocksm-rem-1.2/src/calcenergy_d.cu:69-309 written
+ *    in the loop order of the reference's CPU restatement
+ *    SimilarityMatirxGen/src/unused/cpu_mic/calcenergy_cpu.C:42-243
+ *    (ligand atom outer, pocket point inner, plain sequential sums, libm
+ *    expf/sqrtf/logf), term order of calcenergy_cpu.C:220-239.  The move
+ *    follows move_d.cu:79-132, accept accept_d.cu:18-51, exchange
+ *    exchangereplicas_d.cu:28-78, step order montecarlo_d.cu:81-102.  The
+ *    FP64 variant uses the same forms in double (the serial original,
+ *    gpudocksm-1.1-untested/src/complex.C:1058-1203, has the same algebra).
+ *    This is what per-pose energies are compared against.
+ *
+ * 2. Canonical replay (orc_replay_energy / orc_replay_run).  The same
+ *    algorithm evaluated with the canonical FP32 arithmetic of
+ *    gpudock_amd/csrc/gd_canon.h and the wave-of-64 summation order the
+ *    GPU kernel uses (lane j takes pocket points j, j+64, ...; per-lane
+ *    sums reduced by an xor butterfly 32,16,...,1).  Its purpose is
+ *    decision parity: given the same Philox stream it must reproduce the
+ *    GPU's Monte Carlo trajectory bit for bit.  The arithmetic spec is
+ *    shared on purpose; the formulas themselves are pinned by part 1, the
+ *    published golden vector and closed forms in the tests.
+ *
+ * 3. Philox4x32-10 and the (0,1] mapping are taken from gd_canon.h and
+ *    pinned by known-answer vectors generated from Random123's philox.h.
+ *
+ * Deliberate deviations shared with the product (both restate the same
+ * choice): an MCS restraint with no valid field is skipped (the reference
+ * computes tcc*sqrt(0/0) = NaN, calcenergy_d.cu:255-258); pos == 0 gives
+ * lhm 0 (the serial original guards it, complex.C:1195-1196, the GPU
+ * kernel takes log(0/0), calcenergy_d.cu:267-268).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "gd_remc.h"
+#include "gd_canon.h"
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+/* ====================================================================== */
+/* 1. reference-order restatement                                          */
+/* ====================================================================== */
+
+/* move_d.cu:86-132 */
+ORC_EXPORT void orc_move_ref32(const float m[6], const float *ox, const float *oy,
+                               const float *oz, int lna, const float c[3], float *nx,
+                               float *ny, float *nz, float cen[3])
+{
+  const float s1 = sinf(m[3]), c1 = cosf(m[3]);
+  const float s2 = sinf(m[4]), c2 = cosf(m[4]);
+  const float s3 = sinf(m[5]), c3 = cosf(m[5]);
+  float r[3][3];
+  r[0][0] = c1 * c2;
+  r[0][1] = c1 * s2 * s3 - c3 * s1;
+  r[0][2] = s1 * s3 + c1 * c3 * s2;
+  r[1][0] = c2 * s1;
+  r[1][1] = c1 * c3 + s1 * s2 * s3;
+  r[1][2] = c3 * s1 * s2 - c1 * s3;
+  r[2][0] = -1 * s2;
+  r[2][1] = c2 * s3;
+  r[2][2] = c2 * c3;
+  for (int l = 0; l < lna; ++l) {
+    const float x = ox[l], y = oy[l], z = oz[l];
+    nx[l] = r[0][0] * x + r[0][1] * y + r[0][2] * z + m[0] + c[0];
+    ny[l] = r[1][0] * x + r[1][1] * y + r[1][2] * z + m[1] + c[1];
+    nz[l] = r[2][0] * x + r[2][1] * y + r[2][2] * z + m[2] + c[2];
+  }
+  for (int i = 0; i < 3; ++i)
+    cen[i] = c[i] + m[i];
+}
+
+static const int kFileToSlot[9] = {GD_E_VDW, GD_E_ELE, GD_E_PMF, GD_E_HPC, GD_E_HDB,
+                                   GD_E_DST, GD_E_PSP, GD_E_KDE, GD_E_LHM};
+
+/* calcenergy_cpu.C:24-243 with the rem-1.2 table layout */
+ORC_EXPORT void orc_energy_ref32(const gd_ligand *lig, const gd_protein *prt,
+                                 const gd_psp *psp, const gd_kde *kde,
+                                 const gd_mcs *mcs, const gd_enepara *ep,
+                                 gd_complex_size sz, int prt_conf, const float *lx,
+                                 const float *ly, const float *lz, const float cen[3],
+                                 int term_order, float out[10])
+{
+  const int lna = sz.lna, pnp = sz.pnp;
+  const size_t po = (size_t) prt_conf * pnp;
+  float evdw = 0.0f, eele = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f, ehpc = 0.0f,
+        ekde = 0.0f, elhm = 0.0f;
+
+  for (int l = 0; l < lna; ++l) {
+    float hpc1 = 0.0f;
+    const int lig_t = lig->t[l];
+    for (int j = 0; j < pnp; ++j) {
+      const size_t p = po + j;
+      const int prt_t = prt->t[p];
+      const float dx = lx[l] - prt->x[p];
+      const float dy = ly[l] - prt->y[p];
+      const float dz = lz[l] - prt->z[p];
+      const float dst_pow2 = dx * dx + dy * dy + dz * dz;
+      const float dst_pow4 = dst_pow2 * dst_pow2;
+      const float dst = sqrtf(dst_pow2);
+
+      if (prt->c0_and_d12_or_c2[p] == 1 && dst_pow2 <= 81.0f) {
+        hpc1 += prt->hpp[p] *
+                (1.0f - (3.5f / 81.0f * dst_pow2 - 4.5f / 81.0f / 81.0f * dst_pow4 +
+                         2.5f / 81.0f / 81.0f / 81.0f * dst_pow4 * dst_pow2 -
+                         0.5f / 81.0f / 81.0f / 81.0f / 81.0f * dst_pow4 * dst_pow4));
+      }
+
+      const float p1 = ep->p1a[lig_t][prt_t] / (dst_pow4 * dst_pow4 * dst);
+      const float p2 = ep->p2a[lig_t][prt_t] / (dst_pow4 * dst_pow2);
+      const float p4 = p1 * ep->lj0 * (1.0f + ep->lj1 * dst_pow2) + 1.0f;
+      evdw += (p1 - p2) / p4;
+
+      const float s1 = ep->el1 * dst;
+      float g1;
+      if (s1 < 1)
+        g1 = ep->el0 + ep->a1 * s1 * s1 + ep->b1 * s1 * s1 * s1;
+      else
+        g1 = 1.0f / s1;
+      eele += lig->c[l] * prt->ele[p] * g1;
+
+      const float dst_minus_pmf0 = dst - ep->pmf0[lig_t][prt_t];
+      epmf += ep->pmf1[lig_t][prt_t] /
+              (1.0f + expf((-0.5f * dst + 6.0f) * dst_minus_pmf0));
+
+      if (prt->c[p] == 2 && dst_minus_pmf0 <= 0)
+        epsp += psp->psp[(size_t) lig_t * psp->n_res + prt->seq3r[p]];
+
+      const float hdb0 = ep->hdb0[lig_t][prt_t];
+      if (hdb0 > 0.1f) {
+        const float hdb1 = ep->hdb1[lig_t][prt_t];
+        const float hdb3 = (dst - hdb0) * hdb1;
+        ehdb += hdb1 * expf(-0.5f * hdb3 * hdb3);
+      }
+    }
+    const float hpc2 = (hpc1 - ep->hpl0[lig_t]) / ep->hpl1[lig_t];
+    ehpc += 0.5f * hpc2 * hpc2 - ep->hpl2[lig_t];
+  }
+
+  for (int l = 0; l < lna; ++l) {
+    float kde_val = 0.0f, kde_sz = 0.0f;
+    const int lig_t = lig->t[l];
+    for (int k = 0; k < sz.pnk; ++k) {
+      if (lig_t == kde->t[k]) {
+        const float dx = lx[l] - kde->x[k];
+        const float dy = ly[l] - kde->y[k];
+        const float dz = lz[l] - kde->z[k];
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        kde_val += expf(ep->kde2 * d2);
+        kde_sz += 1.0f;
+      }
+    }
+    if (kde_sz != 0.0f)
+      ekde += (kde_val / kde_sz);
+  }
+  ekde = ekde / ep->kde3;
+
+  for (int m = 0; m < sz.pos; ++m) {
+    float lhm_val = 0.0f, lhm_sz = 0.0f;
+    for (int l = 0; l < lna; ++l) {
+      const size_t f = (size_t) m * mcs->n_field + lig->n[l] + 1;
+      if (mcs->x[f] != GD_MCS_INVALID_COORD) {
+        const float dx = lx[l] - mcs->x[f];
+        const float dy = ly[l] - mcs->y[f];
+        const float dz = lz[l] - mcs->z[f];
+        lhm_val += dx * dx + dy * dy + dz * dz;
+        lhm_sz += 1.0f;
+      }
+    }
+    if (lhm_sz != 0.0f)
+      elhm += mcs->tcc[m] * sqrtf(lhm_val / lhm_sz);
+  }
+
+  const float dx = cen[0] - prt->pocket_center[0];
+  const float dy = cen[1] - prt->pocket_center[1];
+  const float dz = cen[2] - prt->pocket_center[2];
+  const float edst = sqrtf(dx * dx + dy * dy + dz * dz);
+
+  float e[9];
+  e[0] = evdw / lna;
+  e[1] = eele / lna;
+  e[2] = epmf / lna;
+  e[3] = ehpc / lna;
+  e[4] = ehdb / lna / sqrtf(2.0f * 3.1415926535f) * -1.0f;
+  e[5] = edst;
+  e[6] = epsp / lna;
+  e[7] = ekde / lna;
+  e[8] = sz.pos > 0 ? logf(elhm / sz.pos) : 0.0f;
+
+  float slot[10];
+  for (int k = 0; k < 9; ++k)
+    slot[kFileToSlot[k]] = e[k];
+  float etotal = 0.0f;
+  if (term_order == 0) {
+    for (int k = 0; k < 9; ++k) {
+      e[k] = ep->a_para[k] * e[k] + ep->b_para[k];
+      slot[kFileToSlot[k]] = e[k];
+    }
+    for (int k = 0; k < 9; ++k)
+      etotal += ep->w[k] * e[k];
+  } else {
+    for (int i = 0; i < 9; ++i)
+      slot[i] = ep->a_para[i] * slot[i] + ep->b_para[i];
+    for (int i = 0; i < 9; ++i)
+      etotal += ep->w[i] * slot[i];
+  }
+  slot[9] = etotal;
+  memcpy(out, slot, sizeof slot);
+}
+
+/* The same in double; coordinates are moved in double from the float
+ * conformer and movematrix. */
+ORC_EXPORT void orc_energy_ref64(const gd_ligand *lig, const gd_protein *prt,
+                                 const gd_psp *psp, const gd_kde *kde,
+                                 const gd_mcs *mcs, const gd_enepara *ep,
+                                 gd_complex_size sz, int lig_conf, int prt_conf,
+                                 const float mv[6], int term_order, double out[10])
+{
+  const int lna = sz.lna, pnp = sz.pnp;
+  const size_t po = (size_t) prt_conf * pnp, lo = (size_t) lig_conf * lna;
+  double m[6];
+  for (int k = 0; k < 6; ++k)
+    m[k] = mv[k];
+  const double s1 = sin(m[3]), c1 = cos(m[3]), s2 = sin(m[4]), c2 = cos(m[4]),
+               s3 = sin(m[5]), c3 = cos(m[5]);
+  const double r[9] = {c1 * c2, c1 * s2 * s3 - c3 * s1, s1 * s3 + c1 * c3 * s2,
+                       c2 * s1, c1 * c3 + s1 * s2 * s3, c3 * s1 * s2 - c1 * s3,
+                       -s2,     c2 * s3,                c2 * c3};
+  double cen[3];
+  for (int i = 0; i < 3; ++i)
+    cen[i] = (double) lig->center[i] + m[i];
+  double *lx = (double *) malloc(sizeof(double) * 3 * lna);
+  double *ly = lx + lna, *lz = ly + lna;
+  for (int l = 0; l < lna; ++l) {
+    const double x = lig->x[lo + l], y = lig->y[lo + l], z = lig->z[lo + l];
+    lx[l] = r[0] * x + r[1] * y + r[2] * z + cen[0];
+    ly[l] = r[3] * x + r[4] * y + r[5] * z + cen[1];
+    lz[l] = r[6] * x + r[7] * y + r[8] * z + cen[2];
+  }
+  const double el0 = ep->el0, el1 = ep->el1, a1 = 4.0 - 3.0 * el0, b1 = 2.0 * el0 - 3.0;
+  double evdw = 0, eele = 0, epmf = 0, epsp = 0, ehdb = 0, ehpc = 0, ekde = 0, elhm = 0;
+  for (int l = 0; l < lna; ++l) {
+    double hpc1 = 0.0;
+    const int lt = lig->t[l];
+    for (int j = 0; j < pnp; ++j) {
+      const size_t p = po + j;
+      const int pt = prt->t[p];
+      const double dx = lx[l] - prt->x[p], dy = ly[l] - prt->y[p], dz = lz[l] - prt->z[p];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double d = sqrt(d2);
+      if (prt->c0_and_d12_or_c2[p] == 1 && d2 <= 81.0) {
+        const double u = d2 / 81.0;
+        hpc1 += (double) prt->hpp[p] *
+                (1.0 - 0.5 * (7.0 * u - 9.0 * u * u + 5.0 * u * u * u - u * u * u * u));
+      }
+      const double p1 = (double) ep->p1a[lt][pt] / pow(d, 9);
+      const double p2 = (double) ep->p2a[lt][pt] / pow(d, 6);
+      const double p4 = p1 * (double) ep->lj0 * (1.0 + (double) ep->lj1 * d2) + 1.0;
+      evdw += (p1 - p2) / p4;
+      const double s = el1 * d;
+      const double g = s < 1.0 ? el0 + a1 * s * s + b1 * s * s * s : 1.0 / s;
+      eele += (double) lig->c[l] * (double) prt->ele[p] * g;
+      const double dm = d - (double) ep->pmf0[lt][pt];
+      epmf += (double) ep->pmf1[lt][pt] / (1.0 + exp((-0.5 * d + 6.0) * dm));
+      if (prt->c[p] == 2 && dm <= 0)
+        epsp += psp->psp[(size_t) lt * psp->n_res + prt->seq3r[p]];
+      if (ep->hdb0[lt][pt] > 0.1f) {
+        const double h = (d - (double) ep->hdb0[lt][pt]) * (double) ep->hdb1[lt][pt];
+        ehdb += (double) ep->hdb1[lt][pt] * exp(-0.5 * h * h);
+      }
+    }
+    const double z = (hpc1 - (double) ep->hpl0[lt]) / (double) ep->hpl1[lt];
+    ehpc += 0.5 * z * z - (double) ep->hpl2[lt];
+  }
+  for (int l = 0; l < lna; ++l) {
+    double v = 0.0;
+    int n = 0;
+    for (int k = 0; k < sz.pnk; ++k)
+      if (lig->t[l] == kde->t[k]) {
+        const double dx = lx[l] - kde->x[k], dy = ly[l] - kde->y[k], dz = lz[l] - kde->z[k];
+        v += exp((double) ep->kde2 * (dx * dx + dy * dy + dz * dz));
+        ++n;
+      }
+    if (n)
+      ekde += v / n;
+  }
+  ekde /= (double) ep->kde3;
+  for (int mm = 0; mm < sz.pos; ++mm) {
+    double s = 0.0;
+    int c = 0;
+    for (int l = 0; l < lna; ++l) {
+      const size_t f = (size_t) mm * mcs->n_field + lig->n[l] + 1;
+      if (mcs->x[f] != GD_MCS_INVALID_COORD) {
+        const double dx = lx[l] - mcs->x[f], dy = ly[l] - mcs->y[f], dz = lz[l] - mcs->z[f];
+        s += dx * dx + dy * dy + dz * dz;
+        ++c;
+      }
+    }
+    if (c)
+      elhm += (double) mcs->tcc[mm] * sqrt(s / c);
+  }
+  const double ddx = cen[0] - prt->pocket_center[0], ddy = cen[1] - prt->pocket_center[1],
+               ddz = cen[2] - prt->pocket_center[2];
+  double e[9];
+  e[0] = evdw / lna;
+  e[1] = eele / lna;
+  e[2] = epmf / lna;
+  e[3] = ehpc / lna;
+  e[4] = -ehdb / lna / sqrt(2.0 * 3.14159265358979323846);
+  e[5] = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+  e[6] = epsp / lna;
+  e[7] = ekde / lna;
+  e[8] = sz.pos > 0 ? log(elhm / sz.pos) : 0.0;
+  double slot[10];
+  for (int k = 0; k < 9; ++k)
+    slot[kFileToSlot[k]] = e[k];
+  double etotal = 0.0;
+  if (term_order == 0) {
+    for (int k = 0; k < 9; ++k) {
+      e[k] = (double) ep->a_para[k] * e[k] + (double) ep->b_para[k];
+      slot[kFileToSlot[k]] = e[k];
+      etotal += (double) ep->w[k] * e[k];
+    }
+  } else {
+    for (int i = 0; i < 9; ++i) {
+      slot[i] = (double) ep->a_para[i] * slot[i] + (double) ep->b_para[i];
+      etotal += (double) ep->w[i] * slot[i];
+    }
+  }
+  slot[9] = etotal;
+  memcpy(out, slot, sizeof slot);
+  free(lx);
+}
+
+/* Convenience: reference-order FP32 energy of a pose given by conformer
+ * indices and a movematrix. */
+ORC_EXPORT void orc_pose_ref32(const gd_ligand *lig, const gd_protein *prt,
+                               const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                               const gd_enepara *ep, gd_complex_size sz, int lig_conf,
+                               int prt_conf, const float mv[6], int term_order,
+                               float out[10])
+{
+  const int lna = sz.lna;
+  const size_t lo = (size_t) lig_conf * lna;
+  float *buf = (float *) malloc(sizeof(float) * 3 * lna);
+  float cen[3];
+  orc_move_ref32(mv, lig->x + lo, lig->y + lo, lig->z + lo, lna, lig->center, buf,
+                 buf + lna, buf + 2 * lna, cen);
+  orc_energy_ref32(lig, prt, psp, kde, mcs, ep, sz, prt_conf, buf, buf + lna,
+                   buf + 2 * lna, cen, term_order, out);
+  free(buf);
+}
+
+/* Reference-order REMC on the CPU (the CPU baseline): the Metropolis and
+ * exchange rules of the reference on the Philox stream, OpenMP over
+ * replicas as montecarlo_cpu.C:34-55 does.  Energies come from
+ * orc_energy_ref32, so trajectories are statistically, not bitwise, equal
+ * to the GPU's.  Runs steps [0, n_steps) from the loaded pose (IS_AWAY 0);
+ * exchange after every steps_per_exchange steps when n_tmp > 1 and
+ * exchange != 0.  acs: [n_rep] accepted moves; e_final: [n_rep * 10]. */
+ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
+                           const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                           const gd_enepara *ep, const gd_temp *temp,
+                           gd_complex_size sz, const float move_scale[6], int n_steps,
+                           int steps_per_exchange, int exchange, uint32_t seed,
+                           uint32_t cid, int term_order, int n_threads, int *acs,
+                           int *acs_x, int *idx_tmp_out, float *e_final,
+                           float *mv_final)
+{
+  const int n = sz.n_rep, lna = sz.lna;
+  float *mv = (float *) calloc((size_t) n * 6, sizeof(float));
+  float *e = (float *) calloc((size_t) n * 10, sizeof(float));
+  int *it = (int *) malloc(sizeof(int) * n);
+  for (int r = 0; r < n; ++r) {
+    it[r] = (r / sz.n_lig) % sz.n_tmp;
+    acs[r] = 0;
+    if (acs_x)
+      acs_x[r] = 0;
+  }
+#ifdef _OPENMP
+  if (n_threads > 0)
+    omp_set_num_threads(n_threads);
+#else
+  (void) n_threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < n; ++r) {
+    const int l = r % sz.n_lig, p = r / (sz.n_tmp * sz.n_lig);
+    orc_pose_ref32(lig, prt, psp, kde, mcs, ep, sz, l, p, mv + (size_t) r * 6,
+                   term_order, e + (size_t) r * 10);
+  }
+  for (int s0 = 0; s0 < n_steps; s0 += steps_per_exchange) {
+    const int s1 = s0 + steps_per_exchange < n_steps ? s0 + steps_per_exchange : n_steps;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < n; ++r) {
+      const int l = r % sz.n_lig, p = r / (sz.n_tmp * sz.n_lig);
+      const size_t lo = (size_t) l * lna;
+      float *buf = (float *) malloc(sizeof(float) * 3 * lna);
+      float cen[3], mn[6], en[10];
+      float *mo = mv + (size_t) r * 6, *eo = e + (size_t) r * 10;
+      const float mb = temp[it[r]].minus_beta;
+      for (int s = s0; s < s1; ++s) {
+        const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
+        const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
+        const uint32_t w[7] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3], rb.v[0], rb.v[1], rb.v[2]};
+        for (int k = 0; k < 6; ++k)
+          mn[k] = move_scale[k] * (2.0f * gd_u01(w[k]) - 1.0f) + mo[k];
+        orc_move_ref32(mn, lig->x + lo, lig->y + lo, lig->z + lo, lna, lig->center, buf,
+                       buf + lna, buf + 2 * lna, cen);
+        orc_energy_ref32(lig, prt, psp, kde, mcs, ep, sz, p, buf, buf + lna,
+                         buf + 2 * lna, cen, term_order, en);
+        const float de = en[9] - eo[9];
+        if (gd_u01(w[6]) < expf(de * mb)) {
+          memcpy(mo, mn, sizeof mn);
+          memcpy(eo, en, sizeof en);
+          acs[r]++;
+        }
+      }
+      free(buf);
+    }
+    if (exchange && sz.n_tmp > 1) {
+      const int x = s0 / steps_per_exchange;
+      const int n_chain = sz.n_prt * sz.n_lig;
+      for (int c = 0; c < n_chain; ++c) {
+        const int p = c / sz.n_lig, l = c % sz.n_lig;
+        const int base = sz.n_tmp * sz.n_lig * p + l;
+        int temps[64], orders[64];
+        float ener[64];
+        for (int t = 0; t < sz.n_tmp; ++t) {
+          temps[t] = it[base + sz.n_lig * t];
+          ener[t] = e[(size_t) (base + sz.n_lig * t) * 10 + 9];
+          orders[temps[t]] = t;
+        }
+        const int mode = x & 1;
+        const int maxpair = sz.n_tmp / 2 - ((sz.n_tmp % 2 == 0) && mode == 1);
+        for (int j = 0; j < maxpair; ++j) {
+          const gd_u32x4 u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x,
+                                               GD_CTR_EXCHANGE, (uint32_t) (j >> 2), seed, cid);
+          const int i1 = 2 * j + mode, i2 = i1 + 1;
+          const int o1 = orders[i1], o2 = orders[i2];
+          const float delta = (temp[i1].minus_beta - temp[i2].minus_beta) * (ener[o2] - ener[o1]);
+          if (expf(delta) > gd_u01(u4.v[j & 3])) {
+            const int tt = temps[o1];
+            temps[o1] = temps[o2];
+            temps[o2] = tt;
+            if (acs_x) {
+              acs_x[base + sz.n_lig * o1]++;
+              acs_x[base + sz.n_lig * o2]++;
+            }
+          }
+        }
+        for (int t = 0; t < sz.n_tmp; ++t)
+          it[base + sz.n_lig * t] = temps[t];
+      }
+    }
+  }
+  if (e_final)
+    memcpy(e_final, e, sizeof(float) * 10 * n);
+  if (mv_final)
+    memcpy(mv_final, mv, sizeof(float) * 6 * n);
+  if (idx_tmp_out)
+    memcpy(idx_tmp_out, it, sizeof(int) * n);
+  free(mv);
+  free(e);
+  free(it);
+  return 0;
+}
+
+/* ====================================================================== */
+/* 2. canonical replay                                                     */
+/* ====================================================================== */
+#define W 64
+
+static float bfly(float *v)
+{
+  float t[W];
+  for (int off = W / 2; off >= 1; off >>= 1) {
+    for (int j = 0; j < W; ++j)
+      t[j] = v[j] + v[j ^ off];
+    memcpy(v, t, sizeof t);
+  }
+  return v[0];
+}
+
+typedef struct {
+  const gd_ligand *lig;
+  const gd_protein *prt;
+  const gd_psp *psp;
+  const gd_kde *kde;
+  const gd_mcs *mcs;
+  const gd_enepara *ep;
+  gd_complex_size sz;
+  int term_order;
+  gd_pair_scalars sc;
+  /* KDE points per ligand atom type in original order */
+  int kseg[GD_MAXTP2 + 1];
+  float *kx, *ky, *kz;
+} rp_ctx;
+
+static void rp_init(rp_ctx *c, const gd_ligand *lig, const gd_protein *prt,
+                    const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                    const gd_enepara *ep, gd_complex_size sz, int term_order)
+{
+  c->lig = lig; c->prt = prt; c->psp = psp; c->kde = kde; c->mcs = mcs; c->ep = ep;
+  c->sz = sz; c->term_order = term_order;
+  c->sc.lj0 = ep->lj0; c->sc.lj1 = ep->lj1; c->sc.el0 = ep->el0; c->sc.el1 = ep->el1;
+  c->sc.a1 = ep->a1; c->sc.b1 = ep->b1; c->sc.inv_el1 = 1.0f / ep->el1; c->sc.pad = 0.0f;
+  const int n = sz.pnk > 0 ? sz.pnk : 1;
+  c->kx = (float *) malloc(sizeof(float) * 3 * n);
+  c->ky = c->kx + n;
+  c->kz = c->ky + n;
+  int cnt = 0;
+  for (int t = 0; t < GD_MAXTP2; ++t) {
+    c->kseg[t] = cnt;
+    for (int k = 0; k < sz.pnk; ++k)
+      if (kde->t[k] == t) {
+        c->kx[cnt] = kde->x[k];
+        c->ky[cnt] = kde->y[k];
+        c->kz[cnt] = kde->z[k];
+        ++cnt;
+      }
+  }
+  c->kseg[GD_MAXTP2] = cnt;
+}
+
+static void rp_free(rp_ctx *c) { free(c->kx); }
+
+/* canonical energy of atoms (ax, ay, az)[lna] placed with centre cen */
+static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const float *ay,
+                      const float *az, const float cen[3], float out[10])
+{
+  const gd_complex_size sz = c->sz;
+  const int lna = sz.lna, pnp = sz.pnp;
+  const size_t po = (size_t) prt_conf * pnp;
+  const gd_enepara *ep = c->ep;
+  const gd_protein *prt = c->prt;
+  gd_pair_acc acc[W];
+  memset(acc, 0, sizeof acc);
+  float ehpc = 0.0f;
+  for (int l = 0; l < lna; ++l) {
+    const float lx = ax[l], ly = ay[l], lz = az[l];
+    const int lt = c->lig->t[l];
+    const float lq = c->lig->c[l];
+    const float *psp_row = c->psp->psp + (size_t) lt * c->psp->n_res;
+    float hacc[W];
+    for (int j = 0; j < W; ++j) {
+      hacc[j] = 0.0f;
+      for (int p = j; p < pnp; p += W) {
+        const size_t q = po + p;
+        const float d2 = gd_dist2(lx - prt->x[q], ly - prt->y[q], lz - prt->z[q]);
+        if (prt->c0_and_d12_or_c2[q] && d2 <= 81.0f)
+          hacc[j] += gd_hpc_pair(d2, prt->hpp[q]);
+        const int pt = prt->t[q];
+        gd_pair_param pp;
+        pp.p1a = ep->p1a[lt][pt]; pp.p2a = ep->p2a[lt][pt];
+        pp.pmf0 = ep->pmf0[lt][pt]; pp.pmf1 = ep->pmf1[lt][pt];
+        pp.hdb0 = ep->hdb0[lt][pt]; pp.hdb1 = ep->hdb1[lt][pt];
+        pp.pad0 = 0.0f; pp.pad1 = 0.0f;
+        const int is_c2 = prt->c[q] == 2;
+        gd_pair_terms(d2, lq * prt->ele[q], &pp, &c->sc, is_c2, psp_row,
+                      is_c2 ? prt->seq3r[q] : 0, &acc[j]);
+      }
+    }
+    const float hs = bfly(hacc);
+    const float z = (hs - ep->hpl0[lt]) / ep->hpl1[lt];
+    ehpc += 0.5f * z * z - ep->hpl2[lt];
+  }
+
+  float ekde = 0.0f;
+  for (int l = 0; l < lna; ++l) {
+    const int lt = c->lig->t[l];
+    const int s0 = c->kseg[lt], s1 = c->kseg[lt + 1];
+    float v[W];
+    for (int j = 0; j < W; ++j) {
+      v[j] = 0.0f;
+      for (int i = s0 + j; i < s1; i += W)
+        v[j] += gd_exp(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
+                                           az[l] - c->kz[i]));
+    }
+    const float vs = bfly(v);
+    const int n = s1 - s0;
+    if (n > 0)
+      ekde += vs / (float) n;
+  }
+  ekde = ekde / ep->kde3;
+
+  float elhm = 0.0f;
+  for (int m = 0; m < sz.pos; ++m) {
+    float s[W], cn[W];
+    for (int j = 0; j < W; ++j) {
+      s[j] = 0.0f;
+      cn[j] = 0.0f;
+      for (int a = j; a < lna && a < j + 2 * W; a += W) {
+        const size_t f = (size_t) m * c->mcs->n_field + c->lig->n[a] + 1;
+        if (c->mcs->x[f] != GD_MCS_INVALID_COORD) {
+          s[j] += gd_dist2(ax[a] - c->mcs->x[f], ay[a] - c->mcs->y[f], az[a] - c->mcs->z[f]);
+          cn[j] += 1.0f;
+        }
+      }
+    }
+    const float ss = bfly(s), cs = bfly(cn);
+    if (cs > 0.0f)
+      elhm += c->mcs->tcc[m] * gd_sqrt(ss / cs);
+  }
+  elhm = sz.pos > 0 ? gd_log(elhm / (float) sz.pos) : 0.0f;
+
+  const float edst = gd_sqrt(gd_dist2(cen[0] - prt->pocket_center[0],
+                                      cen[1] - prt->pocket_center[1],
+                                      cen[2] - prt->pocket_center[2]));
+  float t[W];
+  float sums[GD_F_NTERM];
+  for (int j = 0; j < W; ++j) t[j] = acc[j].vdw;
+  sums[GD_F_VDW] = bfly(t);
+  for (int j = 0; j < W; ++j) t[j] = acc[j].ele;
+  sums[GD_F_ELE] = bfly(t);
+  for (int j = 0; j < W; ++j) t[j] = acc[j].pmf;
+  sums[GD_F_PMF] = bfly(t);
+  sums[GD_F_HPC] = ehpc;
+  for (int j = 0; j < W; ++j) t[j] = acc[j].hdb;
+  sums[GD_F_HDB] = bfly(t);
+  sums[GD_F_DST] = edst;
+  for (int j = 0; j < W; ++j) t[j] = acc[j].psp;
+  sums[GD_F_PSP] = bfly(t);
+  sums[GD_F_KDE] = ekde;
+  sums[GD_F_LHM] = elhm;
+  gd_finalize(sums, (float) lna, ep->a_para, ep->b_para, ep->w, c->term_order, out);
+}
+
+static void rp_place(const rp_ctx *c, int lig_conf, const float m[6], float *ax, float *ay,
+                     float *az, float cen[3])
+{
+  const int lna = c->sz.lna;
+  const size_t lo = (size_t) lig_conf * lna;
+  float rot[9];
+  gd_rotation(m, rot);
+  for (int a = 0; a < lna; ++a)
+    gd_transform(rot, m, c->lig->center, c->lig->x[lo + a], c->lig->y[lo + a],
+                 c->lig->z[lo + a], &ax[a], &ay[a], &az[a]);
+  for (int i = 0; i < 3; ++i)
+    cen[i] = c->lig->center[i] + m[i];
+}
+
+/* Canonical FP32 energies of given poses, slot order, [n_pose * 10]. */
+ORC_EXPORT void orc_replay_energy(const gd_ligand *lig, const gd_protein *prt,
+                                  const gd_psp *psp, const gd_kde *kde,
+                                  const gd_mcs *mcs, const gd_enepara *ep,
+                                  gd_complex_size sz, int n_pose, const int *lig_conf,
+                                  const int *prt_conf, const float *move, int term_order,
+                                  float *out)
+{
+  rp_ctx c;
+  rp_init(&c, lig, prt, psp, kde, mcs, ep, sz, term_order);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n_pose; ++i) {
+    float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3];
+    rp_place(&c, lig_conf[i], move + (size_t) i * 6, ax, ay, az, cen);
+    rp_energy(&c, prt_conf[i], ax, ay, az, cen, out + (size_t) i * 10);
+  }
+  rp_free(&c);
+}
+
+/* Canonical replay of gd_run: the same initialisation, MC steps, exchange
+ * cadence and counter keys.  Fills results[n_rep] (energy_best_fp64 is
+ * left zero).  steps_total need not be a multiple of steps_per_exchange;
+ * the last, shorter cycle is followed by an exchange like the others. */
+ORC_EXPORT int orc_replay_run(const gd_ligand *lig, const gd_protein *prt,
+                              const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                              const gd_enepara *ep, const gd_temp *temp,
+                              const gd_replica *replica, const gd_mcpara *mc,
+                              gd_complex_size sz, const gd_options *opt,
+                              gd_replica_result *res)
+{
+  rp_ctx c;
+  rp_init(&c, lig, prt, psp, kde, mcs, ep, sz, opt->term_order);
+  const int n = sz.n_rep;
+  int *lc = (int *) malloc(sizeof(int) * n), *pc = (int *) malloc(sizeof(int) * n);
+  int *it = (int *) malloc(sizeof(int) * n);
+  for (int r = 0; r < n; ++r) {
+    if (replica) {
+      pc[r] = replica[r].idx_prt; it[r] = replica[r].idx_tmp; lc[r] = replica[r].idx_lig;
+    } else {
+      pc[r] = r / (sz.n_tmp * sz.n_lig);
+      it[r] = (r / sz.n_lig) % sz.n_tmp;
+      lc[r] = r % sz.n_lig;
+    }
+    memset(&res[r], 0, sizeof res[r]);
+  }
+  /* initial state (MonteCarlo_Init_d) */
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < n; ++r) {
+    float m[6] = {0, 0, 0, 0, 0, 0};
+    if (opt->away)
+      for (int k = 0; k < 6; ++k)
+        m[k] = mc->move_scale[k] * 44.5f + m[k];
+    float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3], e[10];
+    rp_place(&c, lc[r], m, ax, ay, az, cen);
+    rp_energy(&c, pc[r], ax, ay, az, cen, e);
+    memcpy(res[r].move_final, m, sizeof m);
+    memcpy(res[r].move_best, m, sizeof m);
+    memcpy(res[r].energy_final, e, sizeof e);
+    memcpy(res[r].energy_best, e, sizeof e);
+    res[r].step_best = -1;
+  }
+  const int spe = mc->steps_per_exchange;
+  int x = 0;
+  for (int s0 = 0; s0 < mc->steps_total; s0 += spe, ++x) {
+    const int s1 = s0 + spe < mc->steps_total ? s0 + spe : mc->steps_total;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < n; ++r) {
+      gd_replica_result *R = &res[r];
+      const float mb = temp[it[r]].minus_beta;
+      float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3], en[10], mn[6];
+      for (int s = s0; s < s1; ++s) {
+        const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u,
+                                             opt->seed, opt->complex_id);
+        const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u,
+                                             opt->seed, opt->complex_id);
+        const uint32_t w[7] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3], rb.v[0], rb.v[1], rb.v[2]};
+        for (int k = 0; k < 6; ++k)
+          mn[k] = mc->move_scale[k] * (2.0f * gd_u01(w[k]) - 1.0f) + R->move_final[k];
+        rp_place(&c, lc[r], mn, ax, ay, az, cen);
+        rp_energy(&c, pc[r], ax, ay, az, cen, en);
+        const float de = en[GD_E_TOTAL] - R->energy_final[GD_E_TOTAL];
+        if (gd_u01(w[6]) < gd_exp(de * mb)) {
+          R->acs_mc++;
+          memcpy(R->move_final, mn, sizeof mn);
+          memcpy(R->energy_final, en, sizeof en);
+          if (en[GD_E_TOTAL] < R->energy_best[GD_E_TOTAL]) {
+            memcpy(R->move_best, mn, sizeof mn);
+            memcpy(R->energy_best, en, sizeof en);
+            R->step_best = s;
+          }
+        }
+      }
+    }
+    if (opt->exchange && sz.n_tmp > 1) {
+      const int n_chain = sz.n_prt * sz.n_lig;
+      for (int ch = 0; ch < n_chain; ++ch) {
+        const int p = ch / sz.n_lig, l = ch % sz.n_lig;
+        const int base = sz.n_tmp * sz.n_lig * p + l;
+        int temps[64], orders[64];
+        float ener[64];
+        for (int t = 0; t < sz.n_tmp; ++t) {
+          temps[t] = it[base + sz.n_lig * t];
+          ener[t] = res[base + sz.n_lig * t].energy_final[GD_E_TOTAL];
+          orders[temps[t]] = t;
+        }
+        const int mode = x & 1;
+        const int maxpair = sz.n_tmp / 2 - ((sz.n_tmp % 2 == 0) && mode == 1);
+        gd_u32x4 u4;
+        for (int j = 0; j < maxpair; ++j) {
+          if ((j & 3) == 0)
+            u4 = gd_philox4x32_10((uint32_t) ch, (uint32_t) x, GD_CTR_EXCHANGE,
+                                  (uint32_t) (j >> 2), opt->seed, opt->complex_id);
+          const int i1 = 2 * j + mode, i2 = i1 + 1;
+          const int o1 = orders[i1], o2 = orders[i2];
+          const float delta = (temp[i1].minus_beta - temp[i2].minus_beta) * (ener[o2] - ener[o1]);
+          if (gd_exp(delta) > gd_u01(u4.v[j & 3])) {
+            const int tt = temps[o1];
+            temps[o1] = temps[o2];
+            temps[o2] = tt;
+            res[base + sz.n_lig * o1].acs_exchg++;
+            res[base + sz.n_lig * o2].acs_exchg++;
+          }
+        }
+        for (int t = 0; t < sz.n_tmp; ++t)
+          it[base + sz.n_lig * t] = temps[t];
+      }
+    }
+  }
+  for (int r = 0; r < n; ++r)
+    res[r].idx_tmp_final = it[r];
+  free(lc);
+  free(pc);
+  free(it);
+  rp_free(&c);
+  return 0;
+}
+
+/* ====================================================================== */
+/* math/RNG probes for the known-answer and accuracy tests                 */
+/* ====================================================================== */
+ORC_EXPORT void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                           uint32_t k1, uint32_t out[4])
+{
+  const gd_u32x4 r = gd_philox4x32_10(c0, c1, c2, c3, k0, k1);
+  memcpy(out, r.v, sizeof r.v);
+}
+
+ORC_EXPORT void orc_canon_math(int fn, int n, const float *x, float *y, float *y2)
+{
+  for (int i = 0; i < n; ++i) {
+    if (fn == 0)
+      y[i] = gd_exp(x[i]);
+    else if (fn == 1)
+      y[i] = gd_log(x[i]);
+    else if (fn == 2)
+      gd_sincos(x[i], &y[i], &y2[i]);
+    else if (fn == 3)
+      y[i] = gd_u01(gd_as_uint(x[i]));
+  }
+}

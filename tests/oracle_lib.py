@@ -1,0 +1,121 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker; nothing in the product imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from gpudock_amd import remc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR, "all"], check=True)
+        L = C.CDLL(ORACLE_SO)
+        cx6 = [C.POINTER(remc.Ligand), C.POINTER(remc.Protein), C.POINTER(remc.Psp),
+               C.POINTER(remc.Kde), C.POINTER(remc.Mcs), C.POINTER(remc.EnePara)]
+        FP, IP = remc.FP, remc.IP
+        L.orc_pose_ref32.argtypes = cx6 + [remc.ComplexSize, C.c_int, C.c_int, FP, C.c_int, FP]
+        L.orc_energy_ref64.argtypes = cx6 + [remc.ComplexSize, C.c_int, C.c_int, FP, C.c_int,
+                                             C.POINTER(C.c_double)]
+        L.orc_move_ref32.argtypes = [FP, FP, FP, FP, C.c_int, FP, FP, FP, FP, FP]
+        L.orc_replay_energy.argtypes = cx6 + [remc.ComplexSize, C.c_int, IP, IP, FP, C.c_int, FP]
+        L.orc_replay_run.argtypes = cx6 + [C.POINTER(remc.Temp), C.POINTER(remc.Replica),
+                                           C.POINTER(remc.McPara), remc.ComplexSize,
+                                           C.POINTER(remc.Options),
+                                           C.POINTER(remc.ReplicaResult)]
+        L.orc_ref_run.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP, C.c_int,
+                                        C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int,
+                                        C.c_int, IP, IP, IP, FP, FP]
+        L.orc_philox.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]
+        L.orc_canon_math.argtypes = [C.c_int, C.c_int, FP, FP, FP]
+        _L = L
+    return _L
+
+
+def _fp(a):
+    return a.ctypes.data_as(remc.FP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(remc.IP)
+
+
+def pose_ref32(cx, lig_conf, prt_conf, moves, term_order=0):
+    moves = np.ascontiguousarray(moves, np.float32).reshape(-1, 6)
+    out = np.zeros((len(moves), 10), np.float32)
+    for i, m in enumerate(moves):
+        m = np.ascontiguousarray(m)
+        o = out[i]
+        lib().orc_pose_ref32(*cx.args(), cx.size, int(lig_conf[i]), int(prt_conf[i]),
+                             _fp(m), term_order, _fp(o))
+    return out
+
+
+def pose_ref64(cx, lig_conf, prt_conf, moves, term_order=0):
+    moves = np.ascontiguousarray(moves, np.float32).reshape(-1, 6)
+    out = np.zeros((len(moves), 10), np.float64)
+    for i, m in enumerate(moves):
+        m = np.ascontiguousarray(m)
+        o = np.zeros(10, np.float64)
+        lib().orc_energy_ref64(*cx.args(), cx.size, int(lig_conf[i]), int(prt_conf[i]),
+                               _fp(m), term_order, o.ctypes.data_as(C.POINTER(C.c_double)))
+        out[i] = o
+    return out
+
+
+def replay_energy(cx, lig_conf, prt_conf, moves, term_order=0):
+    lig_conf = np.ascontiguousarray(lig_conf, np.int32)
+    prt_conf = np.ascontiguousarray(prt_conf, np.int32)
+    moves = np.ascontiguousarray(moves, np.float32).reshape(-1, 6)
+    out = np.zeros((len(moves), 10), np.float32)
+    lib().orc_replay_energy(*cx.args(), cx.size, len(moves), _ip(lig_conf), _ip(prt_conf),
+                            _fp(moves), term_order, _fp(out))
+    return out
+
+
+def replay_run(cx, temps, mc, opt):
+    size = cx.sized(len(temps))
+    res = (remc.ReplicaResult * size.n_rep)()
+    lib().orc_replay_run(*cx.args(), temps, None, C.byref(mc), size, C.byref(opt), res)
+    return remc.results_to_numpy(res)
+
+
+def ref_run(cx, temps, scale, n_steps, spe, exchange=1, seed=1, cid=0, term_order=0,
+            threads=0):
+    size = cx.sized(len(temps))
+    n = size.n_rep
+    acs = np.zeros(n, np.int32)
+    acx = np.zeros(n, np.int32)
+    it = np.zeros(n, np.int32)
+    e = np.zeros((n, 10), np.float32)
+    mv = np.zeros((n, 6), np.float32)
+    sc = np.ascontiguousarray(scale, np.float32)
+    lib().orc_ref_run(*cx.args(), temps, size, _fp(sc), n_steps, spe, exchange, seed, cid,
+                      term_order, threads, _ip(acs), _ip(acx), _ip(it), _fp(e), _fp(mv))
+    return dict(acs_mc=acs, acs_exchg=acx, idx_tmp_final=it, energy_final=e, move_final=mv)
+
+
+def philox(ctr, key):
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox(*[int(v) for v in ctr], *[int(v) for v in key], out)
+    return list(out)
+
+
+def canon_math(fn, x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x)
+    y2 = np.zeros_like(x)
+    lib().orc_canon_math(fn, len(x), _fp(x), _fp(y), _fp(y2))
+    return (y, y2) if fn == 2 else y

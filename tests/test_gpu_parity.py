@@ -1,0 +1,166 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Per pose: GPU FP32 energies equal the canonical replay bit for bit and
+match the reference-order FP32 restatement within 1e-5 relative (floor 1e-5
+absolute for terms near zero, both taken against max(|E64|, 1)); the GPU
+FP64 re-score matches the oracle's FP64 within 1e-9 relative.
+Per run: with the same Philox stream, every accept and exchange decision
+and every final energy equals the replay's (bitwise)."""
+import math
+
+import numpy as np
+import pytest
+
+import fixtures
+import oracle_lib as O
+from gpudock_amd import remc, synth
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5   # FP32 per-term tolerance against the reference-order oracle
+
+
+def _poses(rng, n, n_lig, n_prt, tsig=0.7, rsig=1.0):
+    lc = rng.integers(0, n_lig, n).astype(np.int32)
+    pc = rng.integers(0, n_prt, n).astype(np.int32)
+    mv = np.concatenate([rng.normal(0, tsig, (n, 3)), rng.normal(0, rsig, (n, 3))], 1)
+    return lc, pc, mv.astype(np.float32)
+
+
+def _check_pose_parity(cx, lc, pc, mv, term_order=0):
+    e, e64 = remc.score_poses(cx, lc, pc, mv, term_order=term_order)
+    rp = O.replay_energy(cx, lc, pc, mv, term_order=term_order)
+    np.testing.assert_array_equal(e, rp)               # canonical: bit-exact
+    r32 = O.pose_ref32(cx, lc, pc, mv, term_order=term_order)
+    r64 = O.pose_ref64(cx, lc, pc, mv, term_order=term_order)
+    scale = np.maximum(np.abs(r64), 1.0)
+    assert np.max(np.abs(e - r32) / scale) <= REL
+    np.testing.assert_allclose(e64, r64, rtol=1e-9, atol=1e-9)
+    return e, r32, r64
+
+
+def test_g1_on_gpu():
+    cx, _ = fixtures.load_1a07c1()
+    e, e64 = remc.score_poses(cx, [0], [0], np.zeros((1, 6)))
+    np.testing.assert_allclose(e[0], fixtures.g1_slots(), rtol=0, atol=5.001e-5)
+    np.testing.assert_allclose(e64[0], fixtures.g1_slots(), rtol=0, atol=1.5e-4)
+
+
+def test_pose_parity_1a07c1():
+    cx, _ = fixtures.load_1a07c1()
+    rng = np.random.default_rng(10)
+    lc, pc, mv = _poses(rng, 200, 21, 3, tsig=0.5, rsig=0.5)
+    _check_pose_parity(cx, lc, pc, mv)
+    _check_pose_parity(cx, lc, pc, mv, term_order=remc.TERM_ORDER_SLOT)
+
+
+def test_pose_parity_c0():
+    cx, temps, mc, cfg = synth.make_config("c0")
+    rng = np.random.default_rng(11)
+    lc, pc, mv = _poses(rng, 300, 4, 2)
+    _check_pose_parity(cx, lc, pc, mv)
+
+
+@pytest.mark.parametrize("lna,pnp,pnk,pos", [
+    (1, 1, 0, 0), (1, 64, 5, 1), (63, 65, 64, 3), (64, 63, 1, 11), (65, 129, 130, 2),
+    (128, 200, 700, 11), (24, 1000, 0, 0), (7, 300, 300, 0)])
+def test_pose_parity_edge_sizes(lna, pnp, pnk, pos):
+    cx = synth.make_complex(pnp=pnp, radius=max(6.0, 3.0 * lna ** (1 / 3) + 4),
+                            lna=lna, n_lig=2, n_prt=2, pnk=pnk, pos=pos, seed=lna + pnp)
+    rng = np.random.default_rng(lna * 1000 + pnp)
+    lc, pc, mv = _poses(rng, 40, 2, 2, tsig=0.4, rsig=0.6)
+    _check_pose_parity(cx, lc, pc, mv)
+
+
+def test_pose_parity_c1_shape():
+    """configs[1] complex (pnp 800, lna 48, 2000 KDE points), a sample of
+    poses across all 64 ligand x 16 protein conformers."""
+    cx, temps, mc, cfg = synth.make_config("c1")
+    rng = np.random.default_rng(12)
+    lc, pc, mv = _poses(rng, 256, 64, 16)
+    _check_pose_parity(cx, lc, pc, mv)
+
+
+def test_pose_parity_c4_shape():
+    """configs[4] complex (pnp 4000, lna 128, 5000 KDE points): KDE does not
+    fit LDS beside the pocket and is read from HBM."""
+    cx, temps, mc, cfg = synth.make_config("c4", n_lig=8)
+    rng = np.random.default_rng(13)
+    lc, pc, mv = _poses(rng, 24, 8, 1, tsig=0.3, rsig=0.3)
+    _check_pose_parity(cx, lc, pc, mv)
+
+
+def _compare_runs(g, r):
+    for k in ("move_final", "energy_final", "move_best", "energy_best", "step_best",
+              "idx_tmp_final", "acs_mc", "acs_exchg"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+
+
+def test_run_matches_replay_c0_full():
+    """configs[0] in full: 8 T x 4 ligand x 2 protein conformers, 1000 steps,
+    exchange every 10: every decision equals the replay's."""
+    cx, temps, mc, cfg = synth.make_config("c0")
+    opt = remc.default_options(seed=1)
+    g, log = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    assert g["acs_mc"].sum() > 0 and g["acs_exchg"].sum() > 0
+    assert log.ac_mc == g["acs_mc"].sum()
+    # best-pose FP64 re-score equals the oracle's FP64 of the same pose
+    n_lig, n_tmp = 4, 8
+    idx = np.arange(len(g["acs_mc"]))
+    lc, pc = idx % n_lig, idx // (n_tmp * n_lig)
+    r64 = O.pose_ref64(cx, lc, pc, g["move_best"])
+    np.testing.assert_allclose(g["energy_best_fp64"], r64, rtol=1e-9, atol=1e-9)
+    scale = np.maximum(np.abs(r64), 1.0)
+    assert np.max(np.abs(g["energy_best"] - r64) / scale) < REL
+
+
+def test_decision_parity_1e4_steps():
+    """10^4 MC steps on 8 temperatures with exchange every 10 steps: the GPU
+    trajectory (accept counts, exchange counts, final state bits) equals the
+    CPU replay's on the same Philox stream."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=1, steps=10000)
+    opt = remc.default_options(seed=1)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    assert g["acs_exchg"].sum() > 0
+
+
+@pytest.mark.parametrize("n_tmp,exchange,away,term_order,steps,spe", [
+    (1, 1, 0, 0, 57, 10), (5, 1, 0, 0, 200, 7), (4, 0, 1, 0, 100, 10),
+    (6, 1, 1, 1, 120, 3), (3, 1, 0, 1, 33, 1)])
+def test_run_variants(n_tmp, exchange, away, term_order, steps, spe):
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=2, n_tmp=n_tmp,
+                                           steps=steps, spe=spe)
+    opt = remc.default_options(seed=3, complex_id=5, exchange=exchange, away=away,
+                               term_order=term_order)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+
+
+def test_session_equals_run():
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=1, steps=60)
+    opt = remc.default_options(seed=9)
+    g, _ = remc.run(cx, temps, mc, opt)
+    s = remc.Session(cx, temps, mc, opt)
+    s.init()
+    s.cycles(2)
+    s.cycles(4)
+    s.sync()
+    h, _ = s.results()
+    s.close()
+    _compare_runs(g, h)
+
+
+def test_acceptance_rises_with_temperature():
+    """MC acceptance grows with T (README.org:295-346 records the same trend)."""
+    cx, temps, mc, cfg = synth.make_config("c0", steps=400)
+    opt = remc.default_options(seed=2, exchange=0)
+    g, _ = remc.run(cx, temps, mc, opt)
+    n_lig, n_prt, n_tmp = 4, 2, 8
+    acs = g["acs_mc"].reshape(n_prt, n_tmp, n_lig).sum(axis=(0, 2))
+    assert acs[-1] > acs[0]
+    assert np.corrcoef(np.arange(n_tmp), acs)[0, 1] > 0.8

@@ -1,0 +1,99 @@
+"""Host preparation, parameter parsing and the C ABI surface (no GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fixtures
+from gpudock_amd import remc, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "gd_remc.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = remc.lib()
+    names = _header_functions()
+    assert "gd_run" in names and "gd_score_poses" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert b"gfx950" in remc.lib().gd_version()
+
+
+def test_enepara_tables_match_reference_prep():
+    """gd_load_enepara + gd_load_weight + gd_load_norpara + gd_optimize_enepara
+    on the reference's parameter files give the tables the reference's own
+    loadEnePara/OptimizeEnepara produced (bitwise)."""
+    _, r = fixtures.load_1a07c1()
+    ep0, ep = synth.enepara_from_params()
+    for name, key in (("p1a", "ep_p1a"), ("p2a", "ep_p2a"), ("pmf0", "ep_pmf0"),
+                      ("pmf1", "ep_pmf1"), ("hdb0", "ep_hdb0"), ("hdb1", "ep_hdb1"),
+                      ("hdb2", "ep_hdb2")):
+        got = np.ctypeslib.as_array(getattr(ep, name)).ravel()
+        want = r[key]
+        # hdb1/hdb2 of absent pairs are 1/0: compare with inf/nan equal
+        np.testing.assert_array_equal(got, want, err_msg=name)
+    sc = [ep.lj0, ep.lj1, ep.el0, ep.el1, ep.a1, ep.b1, ep.kde2, ep.kde3]
+    np.testing.assert_array_equal(np.float32(sc), r["ep_scalars"])
+    for name, key in (("ele", "ep_ele"), ("hpp", "ep_hpp"), ("hpl0", "ep_hpl0"),
+                      ("hpl1", "ep_hpl1"), ("hpl2", "ep_hpl2"), ("a_para", "ep_a"),
+                      ("b_para", "ep_b")):
+        np.testing.assert_array_equal(np.ctypeslib.as_array(getattr(ep, name)), r[key],
+                                      err_msg=name)
+    # weights: the 9 file values (the reference's 10th slot is never used)
+    np.testing.assert_array_equal(np.ctypeslib.as_array(ep.w)[:9], r["ep_w"][:9])
+
+
+def test_set_temperature_matches_reference():
+    _, r = fixtures.load_1a07c1()
+    t = remc.temperatures(1, 0.044, 0.036)
+    assert np.float32(t[0].minus_beta) == r["temp_minus_beta"][0]
+
+
+def test_set_replica_layout():
+    size = remc.ComplexSize(3, 2, 4, 24, 5, 10, 0, 0)
+    arr = (remc.Replica * 24)()
+    remc.check(remc.lib().gd_set_replica(arr, size))
+    for p in range(2):
+        for t in range(4):
+            for l in range(3):
+                f = 4 * 3 * p + 3 * t + l
+                assert (arr[f].idx_rep, arr[f].idx_prt, arr[f].idx_tmp, arr[f].idx_lig) == (f, p, t, l)
+
+
+def test_bad_inputs_are_rejected_without_a_gpu():
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=1, n_tmp=2, steps=10)
+    size = cx.sized(2)
+    bad = remc.ComplexSize(size.n_lig, size.n_prt, size.n_tmp, size.n_rep, 0, size.pnp,
+                           size.pnk, size.pos)
+    rc = remc.lib().gd_run(*cx.args(), temps, None, C.byref(mc), None, bad, None, None)
+    assert rc == -1
+    assert b"lna" in remc.lib().gd_last_error()
+    cx.lig_t[0] = 40
+    rc = remc.lib().gd_run(*cx.args(), temps, None, C.byref(mc), None, size, None, None)
+    assert rc == -1 and b"type" in remc.lib().gd_last_error()
+
+
+def test_load_enepara_missing_file():
+    ep0 = remc.EnePara0()
+    rc = remc.lib().gd_load_enepara(b"/nonexistent/gpudocksm.ff", C.byref(ep0))
+    assert rc == -5
+
+
+@pytest.mark.skipif(remc.lib().gd_device_count() > 0, reason="a GPU is visible")
+def test_no_gpu_is_an_error_not_a_fallback():
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=1, n_tmp=2, steps=10)
+    with pytest.raises(remc.GDError, match="no HIP device"):
+        remc.run(cx, temps, mc)
+    with pytest.raises(remc.GDError, match="no HIP device"):
+        remc.score_poses(cx, [0], [0], np.zeros((1, 6)))
