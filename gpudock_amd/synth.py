@@ -83,6 +83,8 @@ def _min_nonbonded(x):
 
 def _torsion_conformer(rng, x, clash=2.5, n_tors=3):
     n = len(x)
+    if n < 4:
+        return x.copy()
     for _ in range(1000):
         y = x.copy()
         for _ in range(n_tors):
