@@ -23,6 +23,9 @@
  *          as easy as 1, 2, 3", SC'11 (multipliers 0xD2511F53, 0xCD9E8D57,
  *          Weyl key increments 0x9E3779B9, 0xBB67AE85).
  *
+ *   1/sqrt, 1/x : integer-bit seeds (0x5f3759df - i/2, 0x7ef311c3 - i) and
+ *          three Newton iterations each, all in fma.
+ *
  * Energy forms follow the reference kernel gpudocksm-rem-1.2/src/
  * calcenergy_d.cu:69-135 (pair terms), :142-149 (hydrophobic restraint),
  * :186-202 (KDE), :240-268 (position restraints), :276-309 (distance,
@@ -63,6 +66,30 @@ GD_HD float gd_sqrt(float x) { return __builtin_sqrtf(x); }
 
 #define GD_INF_BITS 0x7f800000u
 
+/* 1/sqrt(x), x positive normal: seed from the exponent/mantissa bits
+ * (0x5f3759df - bits/2, relative error <= 3.5e-2), then three Newton steps
+ * y <- y + y (1/2 - x/2 y^2) with fma (error ~1e-7, rounding-limited). */
+GD_HD float gd_rsqrt(float x)
+{
+  float y = gd_as_float(0x5f3759dfu - (gd_as_uint(x) >> 1));
+  const float h = 0.5f * x;
+  y = gd_fma(y, gd_fma(-h, y * y, 0.5f), y);
+  y = gd_fma(y, gd_fma(-h, y * y, 0.5f), y);
+  y = gd_fma(y, gd_fma(-h, y * y, 0.5f), y);
+  return y;
+}
+
+/* 1/b, b positive normal below 2^126: seed 0x7ef311c3 - bits (relative
+ * error <= 5.1e-2), then three Newton steps y <- y + y (1 - b y). */
+GD_HD float gd_rcp(float b)
+{
+  float y = gd_as_float(0x7ef311c3u - gd_as_uint(b));
+  y = gd_fma(y, gd_fma(-b, y, 1.0f), y);
+  y = gd_fma(y, gd_fma(-b, y, 1.0f), y);
+  y = gd_fma(y, gd_fma(-b, y, 1.0f), y);
+  return y;
+}
+
 /* ------------------------------------------------------------------ */
 /* exp                                                                  */
 /* ------------------------------------------------------------------ */
@@ -89,6 +116,27 @@ GD_HD float gd_exp(float x)
   p = gd_fma(p, r, 1.0f);
   const int ni = (int) n;                                /* in [-125, 127] */
   return p * gd_as_float((uint32_t) (ni + 127) << 23);
+}
+
+/* gd_exp for the energy loops, where the argument is never NaN and never
+ * above 88 (every PMF, H-bond and KDE exponent is <= 18): the same value
+ * as gd_exp there, computed without branches. */
+GD_HD float gd_exp_hot(float x)
+{
+  const float xc = __builtin_fmaxf(x, -86.0f);
+  const float n = gd_rint(xc * 0x1.715476p+0f);
+  float r = gd_fma(-n, 0x1.62e430p-1f, xc);
+  r = gd_fma(-n, -0x1.05c610p-29f, r);
+  float p = 0x1.a01a02p-13f;
+  p = gd_fma(p, r, 0x1.6c16c2p-10f);
+  p = gd_fma(p, r, 0x1.111112p-7f);
+  p = gd_fma(p, r, 0x1.555556p-5f);
+  p = gd_fma(p, r, 0x1.555556p-3f);
+  p = gd_fma(p, r, 0.5f);
+  p = gd_fma(p, r, 1.0f);
+  p = gd_fma(p, r, 1.0f);
+  const float v = p * gd_as_float((uint32_t) ((int) n + 127) << 23);
+  return (x < -86.0f) ? 0.0f : v;
 }
 
 /* ------------------------------------------------------------------ */
@@ -236,10 +284,15 @@ GD_HD void gd_transform(const float rot[9], const float m[6], const float c[3],
 /* ------------------------------------------------------------------ */
 /* Pair terms (reference calcenergy_d.cu:69-135)                        */
 /* ------------------------------------------------------------------ */
-/* Per (ligand type, point type) table entry. */
-typedef struct {
+/* Per (ligand type, point type) table entry.  A table row holds
+ * GD_ROW_STRIDE entries: the 30 point types and two all-zero entries used
+ * by padding points. */
+typedef struct __attribute__((aligned(16))) {
   float p1a, p2a, pmf0, pmf1, hdb0, hdb1, pad0, pad1;
 } gd_pair_param;
+
+#define GD_ROW_STRIDE 32
+#define GD_PAD_TYPE 30   /* point type of padding points: all-zero entry */
 
 /* Packed per-point flags. */
 #define GD_PT_TYPE_MASK 0x1fu
@@ -273,25 +326,26 @@ GD_HD float gd_hpc_pair(float d2, float hpp)
   return gd_fma(-hpp, t, hpp);
 }
 
-/* The five directly summed pair terms.  psp_row is the PSP table row of
- * the ligand atom's type; it is read at index seq only for class-2 points
- * inside the PMF radius (calcenergy_d.cu:123-126). */
+/* The five directly summed pair terms.  psp_val is the PSP table value
+ * psp[ligand type][residue] of the point (any value for points that are
+ * not class 2); it is added for class-2 points inside the PMF radius
+ * (calcenergy_d.cu:123-126). */
 GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
-                         const gd_pair_scalars *sc, int is_c2,
-                         const float *psp_row, int seq, gd_pair_acc *acc)
+                         const gd_pair_scalars *sc, int is_c2, float psp_val,
+                         gd_pair_acc *acc)
 {
-  const float d = gd_sqrt(d2);
-  const float r2 = 1.0f / d2;
-  const float r1 = d * r2;
+  /* all inverse powers from one reciprocal square root */
+  const float r1 = gd_rsqrt(d2);
+  const float d = d2 * r1;
+  const float r2 = r1 * r1;
   const float r4 = r2 * r2;
   const float r6 = r4 * r2;
   const float r9 = (r4 * r4) * r1;
 
-  /* soft L-J */
+  /* soft L-J numerator and denominator */
   const float p1 = pp->p1a * r9;
   const float p2 = pp->p2a * r6;
   const float p4 = gd_fma(p1 * sc->lj0, gd_fma(sc->lj1, d2, 1.0f), 1.0f);
-  acc->vdw += (p1 - p2) / p4;
 
   /* distance-dependent electrostatics */
   const float s = sc->el1 * d;
@@ -299,20 +353,33 @@ GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
                              : r1 * sc->inv_el1;
   acc->ele = gd_fma(lq_ele, g, acc->ele);
 
-  /* PMF contact */
+  /* PMF contact denominator */
   const float dm = d - pp->pmf0;
-  const float ex = gd_exp(gd_fma(-0.5f, d, 6.0f) * dm);
-  acc->pmf += pp->pmf1 / (1.0f + ex);
+  const float q1 = 1.0f + gd_exp_hot(gd_fma(-0.5f, d, 6.0f) * dm);
+
+  /* one reciprocal serves both quotients: 1/(p4 q1) */
+  const float rr = gd_rcp(p4 * q1);
+  acc->vdw = gd_fma((p1 - p2) * q1, rr, acc->vdw);
+  acc->pmf = gd_fma(pp->pmf1 * p4, rr, acc->pmf);
 
   /* pocket-specific */
   if (is_c2 && dm <= 0.0f)
-    acc->psp += psp_row[seq];
+    acc->psp += psp_val;
 
   /* hydrogen bond */
   if (pp->hdb0 > 0.1f) {
     const float h = (d - pp->hdb0) * pp->hdb1;
-    acc->hdb = gd_fma(pp->hdb1, gd_exp((-0.5f * h) * h), acc->hdb);
+    acc->hdb = gd_fma(pp->hdb1, gd_exp_hot((-0.5f * h) * h), acc->hdb);
   }
+}
+
+/* Order of the pocket points inside the canonical sums: stable by
+ * (point type, class-2 flag, hydrophobic flag), so that the lanes of one
+ * wavefront mostly see one point type (coherent branches, broadcast table
+ * reads). */
+GD_HD int gd_point_sort_key(int type, int is_c2, int hpc)
+{
+  return type * 4 + (is_c2 ? 2 : 0) + (hpc ? 1 : 0);
 }
 
 /* Raw (averaged, not normalised) terms in the parameter-file order used by

@@ -22,12 +22,13 @@ typedef struct gd_pack {
   int n_rows;            /* distinct ligand atom types = table rows */
   int n_res;             /* PSP residues */
   int term_order;
-  int pad0;
+  int pnp_pad;           /* pnp rounded up to a multiple of 64 */
 
-  /* pocket, per protein conformer: [n_prt][pnp] */
+  /* pocket, per protein conformer: [n_prt][pnp_pad]; points pnp..pnp_pad-1
+   * are padding that contributes exactly zero to every term */
   const float4 *pt_pos;  /* x, y, z, bits(type | flags | seq << 8) */
   const float2 *pt_eh;   /* ele, hpp */
-  /* pair table, [n_rows][GD_MAXTP1_DEV] */
+  /* pair table, [n_rows][GD_ROW_STRIDE] */
   const gd_pair_param *tab;
   /* ligand: conformer coordinates [n_lig][lna] (x, y, z, 0) */
   const float4 *lig_xyz;

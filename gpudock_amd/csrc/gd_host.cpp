@@ -194,11 +194,12 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
   }
   P.n_rows = (int) row_type.size();
 
-  std::vector<gd_pair_param> tab((size_t) P.n_rows * GD_MAXTP1);
+  std::vector<gd_pair_param> tab((size_t) P.n_rows * GD_ROW_STRIDE);
+  memset(tab.data(), 0, tab.size() * sizeof(gd_pair_param));
   for (int r = 0; r < P.n_rows; ++r) {
     const int lt = row_type[r];
     for (int pt = 0; pt < GD_MAXTP1; ++pt) {
-      gd_pair_param &q = tab[(size_t) r * GD_MAXTP1 + pt];
+      gd_pair_param &q = tab[(size_t) r * GD_ROW_STRIDE + pt];
       q.p1a = ep->p1a[lt][pt];
       q.p2a = ep->p2a[lt][pt];
       q.pmf0 = ep->pmf0[lt][pt];
@@ -210,11 +211,38 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
     }
   }
 
-  /* pocket points, per conformer */
-  const size_t np = (size_t) sz.n_prt * sz.pnp;
+  /* pocket points, per conformer, in the canonical order (stable by
+   * gd_point_sort_key) */
+  const int pnp_pad = (sz.pnp + GD_WAVE - 1) / GD_WAVE * GD_WAVE;
+  P.pnp_pad = pnp_pad;
+  const size_t np = (size_t) sz.n_prt * pnp_pad;
   std::vector<float4> pt_pos(np);
   std::vector<float2> pt_eh(np);
-  for (size_t i = 0; i < np; ++i) {
+  {
+    /* padding: a far point of the all-zero table entry, no flags, ele 0 */
+    const uint32_t pbits = GD_PAD_TYPE;
+    float pw;
+    memcpy(&pw, &pbits, 4);
+    for (size_t i = 0; i < np; ++i) {
+      pt_pos[i] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, pw);
+      pt_eh[i] = make_float2(0.0f, 0.0f);
+    }
+  }
+  std::vector<int> perm(sz.pnp);
+  for (int cf = 0; cf < sz.n_prt; ++cf) {
+    const size_t base = (size_t) cf * sz.pnp;
+    const size_t obase = (size_t) cf * pnp_pad;
+    for (int j = 0; j < sz.pnp; ++j)
+      perm[j] = j;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+      return gd_point_sort_key(prt->t[base + a], prt->c[base + a] == 2,
+                               prt->c0_and_d12_or_c2[base + a] != 0) <
+             gd_point_sort_key(prt->t[base + b], prt->c[base + b] == 2,
+                               prt->c0_and_d12_or_c2[base + b] != 0);
+    });
+    for (int j = 0; j < sz.pnp; ++j) {
+    const size_t i = base + perm[j];
+    const size_t o = obase + j;
     uint32_t bits = (uint32_t) prt->t[i] & GD_PT_TYPE_MASK;
     if (prt->c[i] == 2)
       bits |= GD_PT_C2;
@@ -223,8 +251,9 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
     bits |= (uint32_t) (prt->c[i] == 2 ? prt->seq3r[i] : 0) << GD_PT_SEQ_SHIFT;
     float w;
     memcpy(&w, &bits, 4);
-    pt_pos[i] = make_float4(prt->x[i], prt->y[i], prt->z[i], w);
-    pt_eh[i] = make_float2(prt->ele[i], prt->hpp[i]);
+    pt_pos[o] = make_float4(prt->x[i], prt->y[i], prt->z[i], w);
+    pt_eh[o] = make_float2(prt->ele[i], prt->hpp[i]);
+    }
   }
 
   /* ligand conformers */
@@ -332,9 +361,9 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
   HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
   gd_launch_cfg &C = K.C;
   auto al16 = [](int v) { return (v + 15) & ~15; };
-  C.off_eh = al16(sz.pnp * 16);
-  C.off_tab = al16(C.off_eh + sz.pnp * 8);
-  C.off_kde = al16(C.off_tab + P.n_rows * GD_MAXTP1 * (int) sizeof(gd_pair_param));
+  C.off_eh = al16(pnp_pad * 16);
+  C.off_tab = al16(C.off_eh + pnp_pad * 8);
+  C.off_kde = al16(C.off_tab + P.n_rows * GD_ROW_STRIDE * (int) sizeof(gd_pair_param));
   const int with_kde = C.off_kde + pnk_used * 16;
   if (with_kde <= max_lds) {
     C.kde_in_lds = 1;

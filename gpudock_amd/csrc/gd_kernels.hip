@@ -62,14 +62,14 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem,
   float4 *s_pos = (float4 *) smem;
   float2 *s_eh = (float2 *) (smem + L.off_eh);
   float4 *s_tab = (float4 *) (smem + L.off_tab);
-  const float4 *g_pos = P.pt_pos + (size_t) prt * P.pnp;
-  const float2 *g_eh = P.pt_eh + (size_t) prt * P.pnp;
-  for (int i = threadIdx.x; i < P.pnp; i += blockDim.x) {
+  const float4 *g_pos = P.pt_pos + (size_t) prt * P.pnp_pad;
+  const float2 *g_eh = P.pt_eh + (size_t) prt * P.pnp_pad;
+  for (int i = threadIdx.x; i < P.pnp_pad; i += blockDim.x) {
     s_pos[i] = g_pos[i];
     s_eh[i] = g_eh[i];
   }
   const float4 *g_tab = (const float4 *) P.tab;
-  const int ntab4 = P.n_rows * GD_MAXTP1_DEV * 2;
+  const int ntab4 = P.n_rows * GD_ROW_STRIDE * 2;
   for (int i = threadIdx.x; i < ntab4; i += blockDim.x)
     s_tab[i] = g_tab[i];
   if (KDE_LDS) {
@@ -92,8 +92,7 @@ __device__ __forceinline__ void gd_energy_wave(
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
-  const int pnp = P.pnp;
-  const int nchunk = (pnp + GD_WAVE - 1) / GD_WAVE;
+  const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
 
   gd_pair_acc acc;
   acc.vdw = 0.0f; acc.ele = 0.0f; acc.pmf = 0.0f; acc.psp = 0.0f; acc.hdb = 0.0f;
@@ -109,22 +108,22 @@ __device__ __forceinline__ void gd_energy_wave(
     const int row = P.lig_row[l];
     const int lt = P.lig_type[l];
     const float lq = P.lig_q[l];
-    const gd_pair_param *trow = s_tab + row * GD_MAXTP1_DEV;
+    const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
     const float *psp_row = P.psp + (size_t) lt * P.n_res;
     float hacc = 0.0f;
     for (int k = 0; k < nchunk; ++k) {
       const int p = lane + k * GD_WAVE;
-      if (p < pnp) {
-        const float4 q = s_pos[p];
-        const float2 eh = s_eh[p];
-        const uint32_t bits = __float_as_uint(q.w);
-        const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-        if ((bits & GD_PT_HPC) && d2 <= 81.0f)
-          hacc += gd_hpc_pair(d2, eh.y);
-        const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
-        gd_pair_terms(d2, lq * eh.x, &pp, &P.sc, (bits & GD_PT_C2) != 0, psp_row,
-                      (int) (bits >> GD_PT_SEQ_SHIFT), &acc);
-      }
+      const float4 q = s_pos[p];
+      const uint32_t bits = __float_as_uint(q.w);
+      const bool c2 = (bits & GD_PT_C2) != 0;
+      /* issue the PSP gather early; it is consumed at the end of the pair */
+      const float pv = c2 ? psp_row[bits >> GD_PT_SEQ_SHIFT] : 0.0f;
+      const float2 eh = s_eh[p];
+      const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+      if ((bits & GD_PT_HPC) && d2 <= 81.0f)
+        hacc += gd_hpc_pair(d2, eh.y);
+      const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
+      gd_pair_terms(d2, lq * eh.x, &pp, &P.sc, c2, pv, &acc);
     }
     const float hs = gd_wave_sum(hacc);
     const float z = (hs - P.hpl0[lt]) / P.hpl1[lt];
@@ -146,7 +145,7 @@ __device__ __forceinline__ void gd_energy_wave(
     for (int i = s0 + lane; i < s1; i += GD_WAVE) {
       const float4 q = kde[i];
       const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      v += gd_exp(P.kde2 * d2);
+      v += gd_exp_hot(P.kde2 * d2);
     }
     v = gd_wave_sum(v);
     const int n = s1 - s0;
@@ -471,8 +470,8 @@ gd_rescore64_kernel(gd_pack P, const int *__restrict__ lig_conf,
   const double cen[3] = {(double) P.center[0] + m[0], (double) P.center[1] + m[1],
                          (double) P.center[2] + m[2]};
   const float4 *cx = P.lig_xyz + (size_t) lig_conf[i] * lna;
-  const float4 *pp = P.pt_pos + (size_t) prt_conf[i] * pnp;
-  const float2 *pe = P.pt_eh + (size_t) prt_conf[i] * pnp;
+  const float4 *pp = P.pt_pos + (size_t) prt_conf[i] * P.pnp_pad;
+  const float2 *pe = P.pt_eh + (size_t) prt_conf[i] * P.pnp_pad;
   const double lj0 = P.sc.lj0, lj1 = P.sc.lj1, el0 = P.sc.el0, el1 = P.sc.el1;
   const double a1 = 4.0 - 3.0 * el0, b1 = 2.0 * el0 - 3.0;
 
@@ -485,7 +484,7 @@ gd_rescore64_kernel(gd_pack P, const int *__restrict__ lig_conf,
     const int row = P.lig_row[l];
     const int lt = P.lig_type[l];
     const double lq = P.lig_q[l];
-    const gd_pair_param *trow = P.tab + row * GD_MAXTP1_DEV;
+    const gd_pair_param *trow = P.tab + row * GD_ROW_STRIDE;
     double hpc1 = 0.0;
     for (int p = lane; p < pnp; p += GD_WAVE) {
       const float4 q = pp[p];

@@ -507,7 +507,37 @@ typedef struct {
   /* KDE points per ligand atom type in original order */
   int kseg[GD_MAXTP2 + 1];
   float *kx, *ky, *kz;
+  /* canonical point order per protein conformer (stable by sort key) */
+  int *perm;
 } rp_ctx;
+
+static const gd_protein *g_sort_prt;
+static size_t g_sort_base;
+
+static int point_key(int i)
+{
+  const gd_protein *p = g_sort_prt;
+  const size_t q = g_sort_base + i;
+  return gd_point_sort_key(p->t[q], p->c[q] == 2, p->c0_and_d12_or_c2[q] != 0);
+}
+
+/* stable insertion-merge sort of indices by point_key (n is small) */
+static void stable_sort_points(int *idx, int n)
+{
+  int *tmp = (int *) malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int w = 1; w < n; w *= 2) {
+    for (int lo = 0; lo < n; lo += 2 * w) {
+      int mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+      int a = lo, b = mid, k = lo;
+      while (a < mid && b < hi)
+        tmp[k++] = point_key(idx[b]) < point_key(idx[a]) ? idx[b++] : idx[a++];
+      while (a < mid) tmp[k++] = idx[a++];
+      while (b < hi) tmp[k++] = idx[b++];
+    }
+    memcpy(idx, tmp, sizeof(int) * n);
+  }
+  free(tmp);
+}
 
 static void rp_init(rp_ctx *c, const gd_ligand *lig, const gd_protein *prt,
                     const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
@@ -533,9 +563,22 @@ static void rp_init(rp_ctx *c, const gd_ligand *lig, const gd_protein *prt,
       }
   }
   c->kseg[GD_MAXTP2] = cnt;
+  c->perm = (int *) malloc(sizeof(int) * (size_t) sz.n_prt * sz.pnp);
+  for (int cf = 0; cf < sz.n_prt; ++cf) {
+    int *pp = c->perm + (size_t) cf * sz.pnp;
+    for (int j = 0; j < sz.pnp; ++j)
+      pp[j] = j;
+    g_sort_prt = prt;
+    g_sort_base = (size_t) cf * sz.pnp;
+    stable_sort_points(pp, sz.pnp);
+  }
 }
 
-static void rp_free(rp_ctx *c) { free(c->kx); }
+static void rp_free(rp_ctx *c)
+{
+  free(c->kx);
+  free(c->perm);
+}
 
 /* canonical energy of atoms (ax, ay, az)[lna] placed with centre cen */
 static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const float *ay,
@@ -558,7 +601,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     for (int j = 0; j < W; ++j) {
       hacc[j] = 0.0f;
       for (int p = j; p < pnp; p += W) {
-        const size_t q = po + p;
+        const size_t q = po + c->perm[po + p];
         const float d2 = gd_dist2(lx - prt->x[q], ly - prt->y[q], lz - prt->z[q]);
         if (prt->c0_and_d12_or_c2[q] && d2 <= 81.0f)
           hacc[j] += gd_hpc_pair(d2, prt->hpp[q]);
@@ -569,8 +612,8 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         pp.hdb0 = ep->hdb0[lt][pt]; pp.hdb1 = ep->hdb1[lt][pt];
         pp.pad0 = 0.0f; pp.pad1 = 0.0f;
         const int is_c2 = prt->c[q] == 2;
-        gd_pair_terms(d2, lq * prt->ele[q], &pp, &c->sc, is_c2, psp_row,
-                      is_c2 ? prt->seq3r[q] : 0, &acc[j]);
+        const float pv = is_c2 ? psp_row[prt->seq3r[q]] : 0.0f;
+        gd_pair_terms(d2, lq * prt->ele[q], &pp, &c->sc, is_c2, pv, &acc[j]);
       }
     }
     const float hs = bfly(hacc);
@@ -586,7 +629,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     for (int j = 0; j < W; ++j) {
       v[j] = 0.0f;
       for (int i = s0 + j; i < s1; i += W)
-        v[j] += gd_exp(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
+        v[j] += gd_exp_hot(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
                                            az[l] - c->kz[i]));
     }
     const float vs = bfly(v);
@@ -810,5 +853,9 @@ ORC_EXPORT void orc_canon_math(int fn, int n, const float *x, float *y, float *y
       gd_sincos(x[i], &y[i], &y2[i]);
     else if (fn == 3)
       y[i] = gd_u01(gd_as_uint(x[i]));
+    else if (fn == 4)
+      y[i] = gd_rsqrt(x[i]);
+    else if (fn == 5)
+      y[i] = gd_rcp(x[i]);
   }
 }

@@ -136,6 +136,15 @@ def test_canonical_sincos_accuracy():
     assert O.canon_math(2, np.zeros(1, np.float32))[1][0] == 1.0
 
 
+def test_canonical_rsqrt_rcp_accuracy():
+    rng = np.random.default_rng(5)
+    x = np.exp(rng.uniform(-60, 60, 300000)).astype(np.float32)
+    x = np.concatenate([x, np.array([1.0, 2.0, 4.0, 0.25, 81.0], np.float32)])
+    xd = x.astype(np.float64)
+    assert np.max(_ulp_err(O.canon_math(4, x), 1.0 / np.sqrt(xd))) < 2.0
+    assert np.max(_ulp_err(O.canon_math(5, x), 1.0 / xd)) < 2.0
+
+
 # ---------------------------------------------------------- closed forms
 def _single_pair_complex(d, lig_t=3, prt_t=5, cls=2, hpcflag=1, q=0.4, ele=-0.3,
                          psp_val=-0.7, kde_at=None, mcs_at=None):
