@@ -14,7 +14,9 @@
  *   exp  : Cody-Waite reduction x = n*ln2 + r with ln2 split into a float
  *          and its float remainder, Taylor polynomial of degree 7 on
  *          |r| <= ln2/2 (truncation error < 6e-9 relative), scale by 2^n
- *          assembled from exponent bits.
+ *          assembled from exponent bits; a shorter branch-free variant
+ *          (one-constant reduction, fitted degree-6 polynomial) serves the
+ *          energy loops.
  *   log  : x = m*2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
  *          log m = 2 atanh(s) summed to s^11.
  *   sin/cos : reduction by n = rint(x*2/pi) with pi/2 split in three
@@ -118,21 +120,21 @@ GD_HD float gd_exp(float x)
   return p * gd_as_float((uint32_t) (ni + 127) << 23);
 }
 
-/* gd_exp for the energy loops, where the argument is never NaN and never
- * above 88 (every PMF, H-bond and KDE exponent is <= 18): the same value
- * as gd_exp there, computed without branches. */
+/* exp for the energy loops, where the argument is never NaN and never
+ * above 88 (every PMF, H-bond and KDE exponent is <= 18): branch-free, a
+ * one-constant Cody-Waite step (|r| error <= 2.4e-7 at |x| = 86) and a
+ * degree-6 polynomial fitted here for minimax relative error on
+ * |r| <= ln2/2 (6.5e-8 in FP32); total error below 4 ulp. */
 GD_HD float gd_exp_hot(float x)
 {
   const float xc = __builtin_fmaxf(x, -86.0f);
   const float n = gd_rint(xc * 0x1.715476p+0f);
-  float r = gd_fma(-n, 0x1.62e430p-1f, xc);
-  r = gd_fma(-n, -0x1.05c610p-29f, r);
-  float p = 0x1.a01a02p-13f;
-  p = gd_fma(p, r, 0x1.6c16c2p-10f);
-  p = gd_fma(p, r, 0x1.111112p-7f);
-  p = gd_fma(p, r, 0x1.555556p-5f);
-  p = gd_fma(p, r, 0x1.555556p-3f);
-  p = gd_fma(p, r, 0.5f);
+  const float r = gd_fma(-n, 0x1.62e430p-1f, xc);
+  float p = 0x1.6ab926p-10f;
+  p = gd_fma(p, r, 0x1.126d64p-7f);
+  p = gd_fma(p, r, 0x1.55589cp-5f);
+  p = gd_fma(p, r, 0x1.55540ap-3f);
+  p = gd_fma(p, r, 0x1.fffffap-2f);
   p = gd_fma(p, r, 1.0f);
   p = gd_fma(p, r, 1.0f);
   const float v = p * gd_as_float((uint32_t) ((int) n + 127) << 23);
@@ -362,9 +364,8 @@ GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
   acc->vdw = gd_fma((p1 - p2) * q1, rr, acc->vdw);
   acc->pmf = gd_fma(pp->pmf1 * p4, rr, acc->pmf);
 
-  /* pocket-specific */
-  if (is_c2 && dm <= 0.0f)
-    acc->psp += psp_val;
+  /* pocket-specific (adding 0 leaves the sum's bits unchanged) */
+  acc->psp += (is_c2 && dm <= 0.0f) ? psp_val : 0.0f;
 
   /* hydrogen bond */
   if (pp->hdb0 > 0.1f) {

@@ -38,6 +38,7 @@ typedef struct gd_pack {
   /* KDE points sorted by type, and per-row segments [n_rows + 1] */
   const float4 *kde;
   const int *kde_seg;    /* start of row r's points at kde_seg[2r], end kde_seg[2r+1] */
+  const float *kde_inv_n;/* [n_rows] 1.0f / (float) segment length (0 if empty) */
   /* position restraints: targets per (m, atom) [pos][lna] (x, y, z, valid) */
   const float4 *mcs;
   const float *mcs_tcc;  /* [pos] */
@@ -45,6 +46,7 @@ typedef struct gd_pack {
   const float *psp;
   /* per type (24 entries) */
   const float *hpl0, *hpl1, *hpl2;
+  const float *inv_hpl1;    /* 1.0f / hpl1 */
   /* temperatures */
   const float *minus_beta;  /* [n_tmp] */
 

@@ -158,8 +158,8 @@ static int check_inputs(const gd_ligand *lig, const gd_protein *prt, const gd_ps
 struct Pack {
   gd_pack P;
   gd_launch_cfg C;
-  DevBuf pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_seg, mcs,
-      mcs_tcc, psp, hpl, minus_beta;
+  DevBuf pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_seg, kde_inv,
+      mcs, mcs_tcc, psp, hpl, minus_beta;
 };
 
 static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
@@ -272,6 +272,11 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
     kde_seg[2 * r + 1] = (int) kde_pts.size();
   }
   const int pnk_used = (int) kde_pts.size();
+  std::vector<float> kde_inv(P.n_rows);
+  for (int r = 0; r < P.n_rows; ++r) {
+    const int n = kde_seg[2 * r + 1] - kde_seg[2 * r];
+    kde_inv[r] = n > 0 ? 1.0f / (float) n : 0.0f;
+  }
 
   /* MCS targets per (restraint, ligand atom) */
   std::vector<float4> mcs_t((size_t) sz.pos * sz.lna);
@@ -289,11 +294,12 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
 
   std::vector<float> psp_v((size_t) GD_MAXTP2 * psp->n_res);
   memcpy(psp_v.data(), psp->psp, psp_v.size() * sizeof(float));
-  std::vector<float> hpl(3 * GD_MAXTP2);
+  std::vector<float> hpl(4 * GD_MAXTP2);
   for (int t = 0; t < GD_MAXTP2; ++t) {
     hpl[t] = ep->hpl0[t];
     hpl[GD_MAXTP2 + t] = ep->hpl1[t];
     hpl[2 * GD_MAXTP2 + t] = ep->hpl2[t];
+    hpl[3 * GD_MAXTP2 + t] = 1.0f / ep->hpl1[t];
   }
   std::vector<float> mb(std::max(sz.n_tmp, 1), 0.0f);
   if (temp)
@@ -310,6 +316,7 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
   HIP_TRY(K.lig_q.upload(lig_q));
   HIP_TRY(K.kde.upload(kde_pts));
   HIP_TRY(K.kde_seg.upload(kde_seg));
+  HIP_TRY(K.kde_inv.upload(kde_inv));
   HIP_TRY(K.mcs.upload(mcs_t));
   HIP_TRY(K.mcs_tcc.upload(tcc));
   HIP_TRY(K.psp.upload(psp_v));
@@ -325,12 +332,14 @@ static int build_pack(Pack &K, const gd_ligand *lig, const gd_protein *prt,
   P.lig_q = K.lig_q.as<float>();
   P.kde = K.kde.as<float4>();
   P.kde_seg = K.kde_seg.as<int>();
+  P.kde_inv_n = K.kde_inv.as<float>();
   P.mcs = K.mcs.as<float4>();
   P.mcs_tcc = K.mcs_tcc.as<float>();
   P.psp = K.psp.as<float>();
   P.hpl0 = K.hpl.as<float>();
   P.hpl1 = P.hpl0 + GD_MAXTP2;
   P.hpl2 = P.hpl0 + 2 * GD_MAXTP2;
+  P.inv_hpl1 = P.hpl0 + 3 * GD_MAXTP2;
   P.minus_beta = K.minus_beta.as<float>();
 
   P.sc.lj0 = ep->lj0;

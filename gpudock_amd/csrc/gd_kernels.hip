@@ -26,12 +26,34 @@
 
 #define GD_WPG 8  /* wavefronts per workgroup */
 
+__device__ __forceinline__ float gd_bcast_early(float v, int lane)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float gd_dpp(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF,
+                                                    false));
+}
+
+/* Sum over the 64 lanes with the pairing tree of an xor butterfly
+ * (1, 2, 4, 8, 16, 32): DPP quad_perm [1,0,3,2] and [2,3,0,1] pair lanes
+ * j^1 and j^2, row_half_mirror pairs the two quads of each 8-lane half-row,
+ * row_mirror the two halves of each 16-lane row; the four row sums are
+ * then added as (r0 + r1) + (r2 + r3).  Every partial sum is the same
+ * float addition as in the butterfly (addition commutes), so the result is
+ * bit-identical to it; it is returned uniform. */
 __device__ __forceinline__ float gd_wave_sum(float v)
 {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
-    v += __shfl_xor(v, off, GD_WAVE);
-  return v;
+  v += gd_dpp<0xB1>(v);
+  v += gd_dpp<0x4E>(v);
+  v += gd_dpp<0x141>(v);
+  v += gd_dpp<0x140>(v);
+  const float r0 = gd_bcast_early(v, 0), r1 = gd_bcast_early(v, 16);
+  const float r2 = gd_bcast_early(v, 32), r3 = gd_bcast_early(v, 48);
+  return (r0 + r1) + (r2 + r3);
 }
 
 __device__ __forceinline__ double gd_wave_sum64(double v)
@@ -126,7 +148,7 @@ __device__ __forceinline__ void gd_energy_wave(
       gd_pair_terms(d2, lq * eh.x, &pp, &P.sc, c2, pv, &acc);
     }
     const float hs = gd_wave_sum(hacc);
-    const float z = (hs - P.hpl0[lt]) / P.hpl1[lt];
+    const float z = (hs - P.hpl0[lt]) * P.inv_hpl1[lt];
     ehpc += 0.5f * z * z - P.hpl2[lt];
   }
 
@@ -148,9 +170,8 @@ __device__ __forceinline__ void gd_energy_wave(
       v += gd_exp_hot(P.kde2 * d2);
     }
     v = gd_wave_sum(v);
-    const int n = s1 - s0;
-    if (n > 0)
-      ekde += v / (float) n;
+    if (s1 > s0)
+      ekde += v * P.kde_inv_n[row];
   }
   ekde = ekde / P.kde3;
 

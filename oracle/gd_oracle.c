@@ -22,7 +22,7 @@
  *    algorithm evaluated with the canonical FP32 arithmetic of
  *    gpudock_amd/csrc/gd_canon.h and the wave-of-64 summation order the
  *    GPU kernel uses (lane j takes pocket points j, j+64, ...; per-lane
- *    sums reduced by an xor butterfly 32,16,...,1).  Its purpose is
+ *    sums reduced by the pairing tree of an xor butterfly 1,2,4,...,32).  Its purpose is
  *    decision parity: given the same Philox stream it must reproduce the
  *    GPU's Monte Carlo trajectory bit for bit.  The arithmetic spec is
  *    shared on purpose; the formulas themselves are pinned by part 1, the
@@ -483,10 +483,12 @@ ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
 /* ====================================================================== */
 #define W 64
 
+/* pairing tree of the GPU wave sum: lanes j and j^1 first, then j^2, j^4,
+ * j^8, j^16, j^32 */
 static float bfly(float *v)
 {
   float t[W];
-  for (int off = W / 2; off >= 1; off >>= 1) {
+  for (int off = 1; off < W; off <<= 1) {
     for (int j = 0; j < W; ++j)
       t[j] = v[j] + v[j ^ off];
     memcpy(v, t, sizeof t);
@@ -617,7 +619,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
       }
     }
     const float hs = bfly(hacc);
-    const float z = (hs - ep->hpl0[lt]) / ep->hpl1[lt];
+    const float z = (hs - ep->hpl0[lt]) * (1.0f / ep->hpl1[lt]);
     ehpc += 0.5f * z * z - ep->hpl2[lt];
   }
 
@@ -635,7 +637,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     const float vs = bfly(v);
     const int n = s1 - s0;
     if (n > 0)
-      ekde += vs / (float) n;
+      ekde += vs * (1.0f / (float) n);
   }
   ekde = ekde / ep->kde3;
 
@@ -857,5 +859,7 @@ ORC_EXPORT void orc_canon_math(int fn, int n, const float *x, float *y, float *y
       y[i] = gd_rsqrt(x[i]);
     else if (fn == 5)
       y[i] = gd_rcp(x[i]);
+    else if (fn == 6)
+      y[i] = gd_exp_hot(x[i]);
   }
 }
