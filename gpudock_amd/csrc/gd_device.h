@@ -2,9 +2,11 @@
  * gd_device.h -- device-side data layout shared by the HIP kernels and the
  * host code that builds it.
  *
- * One "complex pack" holds everything the Monte Carlo kernels read for one
- * complex; it is built once per gd_run() call (or once per admitted job in
- * the batched and serving paths) and stays resident in HBM.
+ * A complex (one docking job: ligand ensemble, protein ensemble, pocket
+ * tables, KDE, MCS, PSP, temperature ladder and its replica state) is
+ * described by one gd_pack that lives in HBM.  The MC, exchange and
+ * advance kernels take an array of packs, so one launch serves one complex
+ * (the plain Run() path) or many (batched and served jobs) alike.
  */
 #ifndef GD_DEVICE_H
 #define GD_DEVICE_H
@@ -14,7 +16,20 @@
 
 #define GD_WAVE 64
 
-/* Per-complex constants and device pointers (kernel argument). */
+/* Replica state of one complex in HBM, structure of arrays over its n_rep
+ * replicas (local replica index r). */
+typedef struct gd_state {
+  float *mv;        /* [n_rep][8]  movematrix_old (6 used) */
+  float *e;         /* [n_rep][10] energy_old */
+  float *mv_best;   /* [n_rep][8] */
+  float *e_best;    /* [n_rep][10] */
+  int *step_best;   /* [n_rep] */
+  int *idx_tmp;     /* [n_rep] temperature index held by the slot */
+  int *acs_mc;      /* [n_rep] */
+  int *acs_x;       /* [n_rep] */
+} gd_state;
+
+/* One complex. */
 typedef struct gd_pack {
   /* sizes */
   int n_lig, n_prt, n_tmp, n_rep;
@@ -23,6 +38,19 @@ typedef struct gd_pack {
   int n_res;             /* PSP residues */
   int term_order;
   int pnp_pad;           /* pnp rounded up to a multiple of 64 */
+
+  /* run control */
+  int steps_total;       /* MC steps of the job */
+  int spe;               /* steps per exchange */
+  int exchange;          /* temperature swaps on/off */
+  int away;              /* start 44.5 * move_scale away (IS_AWAY) */
+  uint32_t seed, cid;    /* Philox key */
+  int *step;             /* [2] device counters: next MC step, exchanges done */
+
+  /* LDS carve-up for this complex (bytes) */
+  int off_eh, off_tab, off_kde, lds_bytes;
+  int kde_in_lds;
+  int pad0;
 
   /* pocket, per protein conformer: [n_prt][pnp_pad]; points pnp..pnp_pad-1
    * are padding that contributes exactly zero to every term */
@@ -35,50 +63,54 @@ typedef struct gd_pack {
   const int *lig_row;    /* [lna] table row of each atom */
   const int *lig_type;   /* [lna] atom type */
   const float *lig_q;    /* [lna] charge */
-  /* KDE points sorted by type, and per-row segments [n_rows + 1] */
+  /* KDE points grouped by table row; row r spans [kde_seg[2r], kde_seg[2r+1]) */
   const float4 *kde;
-  const int *kde_seg;    /* start of row r's points at kde_seg[2r], end kde_seg[2r+1] */
+  const int *kde_seg;
   const float *kde_inv_n;/* [n_rows] 1.0f / (float) segment length (0 if empty) */
   /* position restraints: targets per (m, atom) [pos][lna] (x, y, z, valid) */
   const float4 *mcs;
   const float *mcs_tcc;  /* [pos] */
   /* PSP table [GD_MAXTP2][n_res] */
   const float *psp;
-  /* per type (24 entries) */
+  /* per ligand type (24 entries each) */
   const float *hpl0, *hpl1, *hpl2;
-  const float *inv_hpl1;    /* 1.0f / hpl1 */
+  const float *inv_hpl1; /* 1.0f / hpl1 */
   /* temperatures */
   const float *minus_beta;  /* [n_tmp] */
+  /* replicas */
+  const int *rep_lig;    /* [n_rep] ligand conformer */
+  const int *rep_prt;    /* [n_rep] protein conformer */
+  gd_state S;
 
   gd_pair_scalars sc;
   float kde2, kde3;
   float center[3];          /* ligand coord_orig.center */
-  float pocket_center[3];   /* same for every protein conformer */
+  float pocket_center[3];
   float move_scale[6];
   float a_para[10], b_para[10], w[10];
 } gd_pack;
 
-#define GD_MAXTP1_DEV 30
-
-/* Replica state in HBM, structure of arrays over [n_rep]. */
-typedef struct gd_state {
-  float *mv;        /* [n_rep][8]  movematrix_old (6 used) */
-  float *e;         /* [n_rep][10] energy_old */
-  float *mv_best;   /* [n_rep][8] */
-  float *e_best;    /* [n_rep][10] */
-  int *step_best;   /* [n_rep] */
-  int *idx_tmp;     /* [n_rep] temperature index held by the slot */
-  int *acs_mc;      /* [n_rep] */
-  int *acs_x;       /* [n_rep] */
-} gd_state;
-
 /* One workgroup task: up to waves-per-group replicas of one protein
- * conformer. */
+ * conformer of one complex. */
 typedef struct gd_task {
+  int cx;
   int prt;
   int r_begin;
   int r_count;
-  int pad;
 } gd_task;
+
+/* One temperature chain: (complex, protein conformer * n_lig + ligand conformer). */
+typedef struct gd_chain {
+  int cx;
+  int c;
+} gd_chain;
+
+/* Per-wave LDS slot holding the replica's uniform state between steps:
+ * m_old[6] e_old[10] m_new[6] e_new[10] */
+#define GD_WSLOT_FLOATS 32
+#define GD_WS_MOLD 0
+#define GD_WS_EOLD 6
+#define GD_WS_MNEW 16
+#define GD_WS_ENEW 22
 
 #endif /* GD_DEVICE_H */

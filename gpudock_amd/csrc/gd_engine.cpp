@@ -1,0 +1,1218 @@
+/*
+ * gd_engine.cpp -- the resident-complex engine behind every entry point.
+ *
+ * An engine owns one HIP stream, one device arena and a set of resident
+ * complexes.  A cycle is three launches on the stream -- the MC kernel over
+ * all tasks of all complexes, the exchange kernel over all temperature
+ * chains, the advance kernel -- with each complex's step index kept in
+ * device memory, so the cycle can be captured once into a hipGraph and
+ * replayed.  Complexes are admitted and retired between cycles without
+ * waiting for queued work: their memory comes from the arena (no
+ * hipMalloc/hipFree), a new complex is uploaded into a block no queued
+ * launch refers to, and a block (or task list, or graph) that queued
+ * launches still refer to is released only after an event recorded behind
+ * them has completed.
+ *
+ *   gd_run            = engine with one complex, run to completion
+ *                       (the reference's Run(), run.cu:40-352)
+ *   gd_session_*      = engine with one complex, driven by the caller
+ *   gd_engine_*       = the engine itself (batched complexes)
+ *   gd_batch_run      = engine with many complexes, autotuned, graph-run
+ *   gd_server_*       = engine fed by a request loop (gd_server.cpp)
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "gd_internal.h"
+#include "gd_engine.h"
+
+namespace {
+template <class T> size_t vbytes(const std::vector<T> &v) { return v.size() * sizeof(T); }
+}  // namespace
+
+/* ------------------------------------------------------------------------ */
+/* engine                                                                    */
+/* ------------------------------------------------------------------------ */
+int gd_engine::init(const gd_engine_config *cfg)
+{
+  gd_engine_config c;
+  memset(&c, 0, sizeof c);
+  if (cfg)
+    c = *cfg;
+  if (c.arena_bytes == 0)
+    c.arena_bytes = (size_t) 1 << 30;
+  const int ndev = gd_device_count();
+  if (ndev < 1)
+    return gd_fail(GD_ERR_NODEV, "no HIP device visible");
+  if (c.device < 0 || c.device >= ndev)
+    return gd_fail(GD_ERR_ARG, "device %d of %d", c.device, ndev);
+  conf = c;
+  lc.wpg = (c.wpg == 4 || c.wpg == 16) ? c.wpg : 8;
+  lc.persistent = c.persistent ? 1 : 0;
+  lc.stage_bytes = 16;
+  GD_HIP_TRY(hipSetDevice(c.device));
+  GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                   c.device));
+  GD_HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
+  GD_HIP_TRY(arena.init(c.arena_bytes));
+  GD_HIP_TRY(pinned.init(c.staging_bytes ? c.staging_bytes : ((size_t) 64 << 20), true));
+  GD_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  /* control block: the persistent work-queue counter */
+  ctl_off = arena.alloc(256, true);
+  if (ctl_off == (size_t) -1)
+    return gd_fail(GD_ERR_NOMEM, "arena too small");
+  std::vector<char> zeros(256, 0);
+  GD_HIP_TRY(hipMemcpy(arena.base() + ctl_off, zeros.data(), 256, hipMemcpyHostToDevice));
+  return GD_OK;
+}
+
+gd_engine::~gd_engine()
+{
+  (void) hipSetDevice(conf.device);
+  if (stream)
+    (void) hipStreamSynchronize(stream);
+  drop_graph();
+  for (auto &d : deferred) {
+    if (d.gexec) (void) hipGraphExecDestroy(d.gexec);
+    if (d.graph) (void) hipGraphDestroy(d.graph);
+    if (d.ev) (void) hipEventDestroy(d.ev);
+  }
+  deferred.clear();
+  for (auto &e : ev_mc) (void) hipEventDestroy(e);
+  for (auto &e : ev_x) (void) hipEventDestroy(e);
+  for (auto *s : slots)
+    delete s;
+  if (stream)
+    (void) hipStreamDestroy(stream);
+}
+
+int *gd_engine::queue_ptr() { return (int *) (arena.base() + ctl_off); }
+
+void gd_engine::defer(size_t off, size_t bytes, hipGraph_t g, hipGraphExec_t x, bool host)
+{
+  Deferred d{nullptr, off, bytes, g, x, host};
+  if (hipEventCreateWithFlags(&d.ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventRecord(d.ev, stream) != hipSuccess) {
+    /* no event: release once the stream has drained */
+    if (d.ev)
+      (void) hipEventDestroy(d.ev);
+    (void) hipStreamSynchronize(stream);
+    if (bytes)
+      (host ? pinned : arena).free(off, bytes);
+    if (x) (void) hipGraphExecDestroy(x);
+    if (g) (void) hipGraphDestroy(g);
+    return;
+  }
+  deferred.push_back(d);
+}
+
+void gd_engine::reap()
+{
+  size_t k = 0;
+  for (size_t i = 0; i < deferred.size(); ++i) {
+    Deferred &d = deferred[i];
+    if (hipEventQuery(d.ev) == hipSuccess) {
+      if (d.bytes)
+        (d.host ? pinned : arena).free(d.off, d.bytes);
+      if (d.gexec) (void) hipGraphExecDestroy(d.gexec);
+      if (d.graph) (void) hipGraphDestroy(d.graph);
+      (void) hipEventDestroy(d.ev);
+    } else {
+      deferred[k++] = d;
+    }
+  }
+  deferred.resize(k);
+}
+
+void gd_engine::drop_graph()
+{
+  if (gexec || graph)
+    defer(0, 0, graph, gexec);
+  gexec = nullptr;
+  graph = nullptr;
+}
+
+/* Arena block of `bytes` (complex blocks low, small task lists and scratch
+ * high); when the arena is full, wait for queued work and release what it
+ * held before giving up. */
+static size_t arena_take(gd_engine &E, size_t bytes, bool high = false)
+{
+  size_t off = E.arena.alloc(bytes, high);
+  if (off == (size_t) -1 && !E.deferred.empty()) {
+    (void) hipStreamSynchronize(E.stream);
+    E.reap();
+    off = E.arena.alloc(bytes, high);
+  }
+  return off;
+}
+
+int gd_engine::upload(char *dev, const void *src, size_t bytes)
+{
+  if (!bytes)
+    return GD_OK;
+  size_t off = pinned.alloc(bytes);
+  if (off == (size_t) -1 && !deferred.empty()) {
+    reap();
+    off = pinned.alloc(bytes);
+  }
+  if (off != (size_t) -1) {
+    memcpy(pinned.base() + off, src, bytes);
+    hipError_t e = hipMemcpyAsync(dev, pinned.base() + off, bytes, hipMemcpyHostToDevice,
+                                  stream);
+    defer(off, bytes, nullptr, nullptr, true);
+    GD_HIP_TRY(e);
+    return GD_OK;
+  }
+  /* larger than the staging area: pageable copy, then wait for it */
+  GD_HIP_TRY(hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, stream));
+  GD_HIP_TRY(hipStreamSynchronize(stream));
+  return GD_OK;
+}
+
+int gd_engine::upload_image(GdImage &I, size_t off, GdBlock &b)
+{
+  const size_t bytes = gd_image_footprint(I);
+  char *dev = arena.base() + off;
+  size_t hoff = pinned.alloc(bytes);
+  if (hoff == (size_t) -1 && !deferred.empty()) {
+    reap();
+    hoff = pinned.alloc(bytes);
+  }
+  if (hoff != (size_t) -1) {
+    gd_pack_image(I, pinned.base() + hoff, dev, b);
+    hipError_t e = hipMemcpyAsync(dev, pinned.base() + hoff, bytes, hipMemcpyHostToDevice,
+                                  stream);
+    defer(hoff, bytes, nullptr, nullptr, true);
+    GD_HIP_TRY(e);
+    return GD_OK;
+  }
+  std::vector<char> host(bytes);
+  gd_pack_image(I, host.data(), dev, b);
+  GD_HIP_TRY(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, stream));
+  GD_HIP_TRY(hipStreamSynchronize(stream));
+  return GD_OK;
+}
+
+/* New device copies of the pack, task and chain arrays for the current
+ * members (fresh blocks: queued launches keep reading the old ones, which
+ * are released behind them). */
+int gd_engine::rebuild()
+{
+  drop_graph();
+  std::vector<gd_pack> packs;
+  std::vector<gd_task> tasks;
+  std::vector<gd_chain> chains;
+  std::vector<double> cost;
+  int stage = 16;
+  for (auto *s : slots) {
+    const int cx = (int) packs.size();
+    s->cx_index = cx;
+    const gd_pack &P = s->img->P;
+    packs.push_back(P);
+    const size_t t0 = tasks.size();
+    gd_complex_tasks(*s->img, cx, lc.wpg, tasks);
+    for (size_t t = t0; t < tasks.size(); ++t)
+      cost.push_back((double) tasks[t].r_count * P.lna * P.pnp_pad);
+    const int nc = P.n_prt * P.n_lig;
+    for (int c = 0; c < nc; ++c)
+      chains.push_back(gd_chain{cx, c});
+    stage = std::max(stage, P.lds_bytes);
+  }
+  /* longest tasks first: the queue hands out big tasks before small ones */
+  std::vector<int> order(tasks.size());
+  for (size_t i = 0; i < order.size(); ++i)
+    order[i] = (int) i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<gd_task> sorted(tasks.size());
+  for (size_t i = 0; i < order.size(); ++i)
+    sorted[i] = tasks[order[i]];
+
+  if (packs_bytes) defer(packs_off, packs_bytes);
+  if (tasks_bytes) defer(tasks_off, tasks_bytes);
+  if (chains_bytes) defer(chains_off, chains_bytes);
+  packs_bytes = tasks_bytes = chains_bytes = 0;
+  packs_off = tasks_off = chains_off = 0;
+  auto place = [&](size_t &off, size_t &cap, const void *src, size_t bytes) -> int {
+    if (!bytes)
+      return GD_OK;
+    const size_t want = GdArena::round(bytes);
+    off = arena_take(*this, want, true);
+    if (off == (size_t) -1)
+      return gd_fail(GD_ERR_NOMEM, "arena too small for the task lists");
+    cap = want;
+    return upload(arena.base() + off, src, bytes);
+  };
+  int rc = place(packs_off, packs_bytes, packs.data(), vbytes(packs));
+  if (!rc) rc = place(tasks_off, tasks_bytes, sorted.data(), vbytes(sorted));
+  if (!rc) rc = place(chains_off, chains_bytes, chains.data(), vbytes(chains));
+  if (rc)
+    return rc;
+  n_cx = (int) packs.size();
+  n_tasks = (int) sorted.size();
+  n_chain = (int) chains.size();
+  lc.stage_bytes = stage;
+  GD_HIP_TRY(gd_set_kernel_attrs(stage));
+  int per_cu = 0;
+  GD_HIP_TRY(gd_mc_occupancy(lc.wpg, stage, &per_cu));
+  resident = std::max(1, per_cu) * n_cu;
+  dirty = false;
+  ++n_rebuilds;
+  return GD_OK;
+}
+
+int gd_engine::ensure_built()
+{
+  if (!dirty)
+    return GD_OK;
+  GD_HIP_TRY(hipSetDevice(conf.device));
+  return rebuild();
+}
+
+int gd_engine::grid() const
+{
+  if (!lc.persistent)
+    return n_tasks;
+  return std::min(n_tasks, resident);
+}
+
+int gd_engine::build_image(GdImage &I, const gd_job &job) const
+{
+  gd_options o;
+  if (job.opt) {
+    o = *job.opt;
+  } else {
+    memset(&o, 0, sizeof o);
+    o.seed = 1;
+    o.exchange = 1;
+  }
+  const gd_complex_size &sz = job.size;
+  int rc = gd_check_inputs(job.lig, job.prt, job.psp, job.kde, job.mcs, job.enepara, sz);
+  if (rc)
+    return rc;
+  if (!job.temp || !job.mcpara)
+    return gd_fail(GD_ERR_ARG, "null temp/mcpara");
+  if (sz.n_tmp < 1 || sz.n_tmp > 64)
+    return gd_fail(GD_ERR_ARG, "n_tmp=%d outside [1, 64]", sz.n_tmp);
+  if ((long long) sz.n_rep != (long long) sz.n_lig * sz.n_prt * sz.n_tmp)
+    return gd_fail(GD_ERR_ARG, "n_rep=%d != n_lig*n_prt*n_tmp", sz.n_rep);
+  const gd_mcpara *mc = job.mcpara;
+  if (mc->steps_total < 0 || mc->steps_per_exchange < 1)
+    return gd_fail(GD_ERR_ARG, "steps_total=%d steps_per_exchange=%d", mc->steps_total,
+                   mc->steps_per_exchange);
+  return gd_build_image(I, job.lig, job.prt, job.psp, job.kde, job.mcs, job.enepara, job.temp,
+                        job.replica, mc, sz, o, max_lds, conf.kde_policy);
+}
+
+int gd_engine::add(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
+                   const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *ep,
+                   const gd_temp *temp, const gd_replica *replica, const gd_mcpara *mc,
+                   const gd_complex_size &sz, const gd_options &o, int *handle)
+{
+  gd_job job;
+  memset(&job, 0, sizeof job);
+  job.lig = lig; job.prt = prt; job.psp = psp; job.kde = kde; job.mcs = mcs;
+  job.enepara = ep; job.temp = temp; job.replica = replica; job.mcpara = mc;
+  job.size = sz; job.opt = &o;
+  std::unique_ptr<GdImage> img(new GdImage());
+  int rc = build_image(*img, job);
+  if (rc)
+    return rc;
+  return add_image(img, handle);
+}
+
+int gd_engine::add_image(std::unique_ptr<GdImage> &img, int *handle)
+{
+  GD_HIP_TRY(hipSetDevice(conf.device));
+  if (!img)
+    return gd_fail(GD_ERR_ARG, "no image");
+  const size_t bytes = gd_image_footprint(*img);
+  size_t off = arena_take(*this, bytes);
+  if (off == (size_t) -1 && slots.empty() && (packs_bytes || tasks_bytes || chains_bytes)) {
+    /* nothing resident: the task lists of retired complexes go first */
+    rebuild();
+    (void) hipStreamSynchronize(stream);
+    reap();
+    off = arena_take(*this, bytes);
+  }
+  if (off == (size_t) -1)
+    return gd_fail(GD_ERR_NOMEM, "complex needs %zu B; arena has %zu of %zu B in use", bytes,
+                   arena.used(), arena.size());
+  std::unique_ptr<GdSlot> s(new GdSlot());
+  s->img = std::move(img);
+  GdImage &I = *s->img;
+  int rc = upload_image(I, off, s->blk);
+  if (rc) {
+    arena.free(off, bytes);
+    return rc;
+  }
+  s->off = off;
+  s->bytes = bytes;
+  s->pack_dev = (gd_pack *) (arena.base() + off + s->blk.pack);
+  s->handle = next_handle++;
+  s->total_steps = I.P.steps_total;
+  s->spe = I.P.spe;
+  /* initial energies of this complex alone (grid form over its own pack),
+   * queued behind the cycles already on the stream */
+  std::vector<gd_task> t0;
+  gd_complex_tasks(I, 0, lc.wpg, t0);
+  const size_t tb = GdArena::round(vbytes(t0));
+  const size_t toff = arena_take(*this, tb, true);
+  if (toff == (size_t) -1) {
+    arena.free(off, bytes);
+    return gd_fail(GD_ERR_NOMEM, "arena too small for the initial-energy tasks");
+  }
+  rc = upload(arena.base() + toff, t0.data(), vbytes(t0));
+  if (rc) {
+    arena.free(toff, tb);
+    arena.free(off, bytes);
+    return rc;
+  }
+  hipError_t he = hipSuccess;
+  gd_launch_cfg ic = lc;
+  ic.persistent = 0;
+  ic.stage_bytes = std::max(lc.stage_bytes, I.P.lds_bytes);
+  if (he == hipSuccess)
+    he = gd_set_kernel_attrs(ic.stage_bytes);
+  if (he == hipSuccess)
+    he = gd_launch_mc(ic, s->pack_dev, (const gd_task *) (arena.base() + toff),
+                      (int) t0.size(), nullptr, (int) t0.size(), 1, stream);
+  defer(toff, tb);
+  if (he != hipSuccess) {
+    defer(off, bytes);
+    return gd_fail(GD_ERR_HIP, "admission: %s", hipGetErrorString(he));
+  }
+  *handle = s->handle;
+  slots.push_back(s.release());
+  dirty = true;
+  return GD_OK;
+}
+
+GdSlot *gd_engine::find(int handle)
+{
+  for (auto *s : slots)
+    if (s->handle == handle)
+      return s;
+  return nullptr;
+}
+
+int gd_engine::remove(int handle)
+{
+  for (size_t i = 0; i < slots.size(); ++i) {
+    if (slots[i]->handle == handle) {
+      GD_HIP_TRY(hipSetDevice(conf.device));
+      defer(slots[i]->off, slots[i]->bytes);
+      delete slots[i];
+      slots.erase(slots.begin() + i);
+      dirty = true;
+      return GD_OK;
+    }
+  }
+  return gd_fail(GD_ERR_ARG, "no complex with handle %d", handle);
+}
+
+int gd_engine::enqueue_cycle()
+{
+  const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
+  const gd_task *tasks = (const gd_task *) (arena.base() + tasks_off);
+  const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
+  int *queue = lc.persistent ? queue_ptr() : nullptr;
+  GD_HIP_TRY(gd_launch_mc(lc, packs, tasks, n_tasks, queue, grid(), 0, stream));
+  GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, stream));
+  GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), stream));
+  return GD_OK;
+}
+
+int gd_engine::cycles(int n)
+{
+  if (n < 0)
+    return gd_fail(GD_ERR_ARG, "negative cycle count");
+  int rc = ensure_built();
+  if (rc)
+    return rc;
+  if (n_tasks == 0)
+    return GD_OK;
+  GD_HIP_TRY(hipSetDevice(conf.device));
+  for (int c = 0; c < n; ++c) {
+    if (timing) {
+      const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
+      const gd_task *tasks = (const gd_task *) (arena.base() + tasks_off);
+      const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
+      int *queue = lc.persistent ? queue_ptr() : nullptr;
+      hipEvent_t a, b, x0, x1;
+      GD_HIP_TRY(hipEventCreate(&a));
+      GD_HIP_TRY(hipEventCreate(&b));
+      GD_HIP_TRY(hipEventCreate(&x0));
+      GD_HIP_TRY(hipEventCreate(&x1));
+      GD_HIP_TRY(hipEventRecord(a, stream));
+      GD_HIP_TRY(gd_launch_mc(lc, packs, tasks, n_tasks, queue, grid(), 0, stream));
+      GD_HIP_TRY(hipEventRecord(b, stream));
+      GD_HIP_TRY(hipEventRecord(x0, stream));
+      GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, stream));
+      GD_HIP_TRY(hipEventRecord(x1, stream));
+      GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), stream));
+      ev_mc.push_back(a);
+      ev_mc.push_back(b);
+      ev_x.push_back(x0);
+      ev_x.push_back(x1);
+    } else if (conf.use_graph) {
+      if (!gexec) {
+        GD_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc2 = enqueue_cycle();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc2) {
+          if (g) (void) hipGraphDestroy(g);
+          return rc2;
+        }
+        GD_HIP_TRY(e);
+        graph = g;
+        GD_HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        ++n_captures;
+      }
+      GD_HIP_TRY(hipGraphLaunch(gexec, stream));
+    } else {
+      int rc2 = enqueue_cycle();
+      if (rc2)
+        return rc2;
+    }
+    ++n_cycles;
+    for (auto *s : slots) {
+      const int left = s->total_steps - s->steps_done;
+      const int k = left < s->spe ? left : s->spe;
+      if (k > 0) {
+        s->steps_done += k;
+        s->cycles_done += 1;
+      }
+    }
+  }
+  return GD_OK;
+}
+
+/* One MC launch at the current step of every complex with the current
+ * launch shape, then the queue re-armed; no step counter moves. */
+int gd_engine::trial_cycle()
+{
+  const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
+  const gd_task *tasks = (const gd_task *) (arena.base() + tasks_off);
+  int *queue = lc.persistent ? queue_ptr() : nullptr;
+  GD_HIP_TRY(gd_launch_mc(lc, packs, tasks, n_tasks, queue, grid(), 0, stream));
+  GD_HIP_TRY(gd_launch_advance(packs, 0, queue_ptr(), stream));
+  return GD_OK;
+}
+
+/* Times one MC launch for each launch shape (waves per group 4/8/16, grid
+ * or persistent form) on the resident complexes and keeps the fastest.
+ * The replica state is saved before and restored after, so tuning leaves
+ * no trace in the results. */
+int gd_engine::autotune(gd_tune_report *rep)
+{
+  GD_HIP_TRY(hipSetDevice(conf.device));
+  int rc = ensure_built();
+  if (rc)
+    return rc;
+  if (rep)
+    memset(rep, 0, sizeof *rep);
+  if (n_tasks == 0)
+    return GD_OK;
+  size_t snap_bytes = 0;
+  for (auto *s : slots)
+    snap_bytes += GdArena::round(s->state_bytes());
+  const size_t snap = arena_take(*this, snap_bytes, true);
+  if (snap == (size_t) -1)
+    return gd_fail(GD_ERR_NOMEM, "arena too small to save the replica state for tuning");
+  auto copy_state = [&](bool save) -> hipError_t {
+    size_t o = snap;
+    for (auto *s : slots) {
+      char *st = arena.base() + s->off + s->blk.mv;
+      char *sv = arena.base() + o;
+      hipError_t e = hipMemcpyAsync(save ? sv : st, save ? st : sv, s->state_bytes(),
+                                    hipMemcpyDeviceToDevice, stream);
+      if (e != hipSuccess)
+        return e;
+      o += GdArena::round(s->state_bytes());
+    }
+    return hipSuccess;
+  };
+  hipError_t he = copy_state(true);
+  const int wpgs[3] = {4, 8, 16};
+  float best = 3.0e38f;
+  gd_launch_cfg best_c = lc;
+  int n = 0;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (he == hipSuccess)
+    he = hipEventCreate(&a);
+  if (he == hipSuccess)
+    he = hipEventCreate(&b);
+  for (int wi = 0; wi < 3 && he == hipSuccess && !rc; ++wi) {
+    for (int pers = 0; pers < 2 && he == hipSuccess && !rc; ++pers) {
+      lc.wpg = wpgs[wi];
+      lc.persistent = pers;
+      rc = rebuild();
+      if (!rc)
+        rc = trial_cycle();   /* warm-up */
+      if (!rc)
+        he = hipEventRecord(a, stream);
+      if (!rc && he == hipSuccess)
+        rc = trial_cycle();
+      if (!rc && he == hipSuccess)
+        he = hipEventRecord(b, stream);
+      if (!rc && he == hipSuccess)
+        he = hipEventSynchronize(b);
+      float ms = 0.0f;
+      if (!rc && he == hipSuccess)
+        he = hipEventElapsedTime(&ms, a, b);
+      if (rc || he != hipSuccess)
+        break;
+      reap();
+      if (rep && n < 8) {
+        rep->wpg[n] = lc.wpg;
+        rep->persistent[n] = lc.persistent;
+        rep->ms[n] = ms;
+      }
+      ++n;
+      if (ms < best) {
+        best = ms;
+        best_c = lc;
+      }
+    }
+  }
+  if (a) (void) hipEventDestroy(a);
+  if (b) (void) hipEventDestroy(b);
+  lc = best_c;
+  if (he == hipSuccess)
+    he = copy_state(false);
+  defer(snap, snap_bytes);
+  if (rc)
+    return rc;
+  if (he != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+  if (rep) {
+    rep->n = n;
+    rep->chosen_wpg = lc.wpg;
+    rep->chosen_persistent = lc.persistent;
+  }
+  return rebuild();
+}
+
+int gd_engine::sync()
+{
+  GD_HIP_TRY(hipSetDevice(conf.device));
+  GD_HIP_TRY(hipStreamSynchronize(stream));
+  drain_timing();
+  reap();
+  return GD_OK;
+}
+
+void gd_engine::drain_timing()
+{
+  for (size_t i = 0; i + 1 < ev_mc.size(); i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev_mc[i], ev_mc[i + 1]) == hipSuccess) {
+      mc_ms += ms;
+      mc_launches++;
+    }
+    (void) hipEventDestroy(ev_mc[i]);
+    (void) hipEventDestroy(ev_mc[i + 1]);
+  }
+  ev_mc.clear();
+  for (size_t i = 0; i + 1 < ev_x.size(); i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev_x[i], ev_x[i + 1]) == hipSuccess) {
+      x_ms += ms;
+      x_launches++;
+    }
+    (void) hipEventDestroy(ev_x[i]);
+    (void) hipEventDestroy(ev_x[i + 1]);
+  }
+  ev_x.clear();
+}
+
+size_t gd_engine::collect_bytes(int handle)
+{
+  GdSlot *s = find(handle);
+  if (!s)
+    return 0;
+  return GdArena::round(s->state_bytes()) + (size_t) s->img->P.n_rep * 10 * sizeof(double);
+}
+
+int gd_engine::collect_async(int handle, GdCollect &c)
+{
+  GdSlot *s = find(handle);
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "no complex with handle %d", handle);
+  if (!c.host || c.host_bytes < collect_bytes(handle))
+    return gd_fail(GD_ERR_ARG, "read-back buffer too small");
+  GD_HIP_TRY(hipSetDevice(conf.device));
+  const gd_pack &P = s->img->P;
+  const int n = P.n_rep;
+  const size_t e64_bytes = (size_t) n * 10 * sizeof(double);
+  const size_t eb = GdArena::round(e64_bytes);
+  const size_t eoff = arena_take(*this, eb, true);
+  if (eoff == (size_t) -1)
+    return gd_fail(GD_ERR_NOMEM, "arena too small for the FP64 re-score");
+  double *e64 = (double *) (arena.base() + eoff);
+  const size_t sb = s->state_bytes();
+  /* FP64 re-score of every replica's best movematrix (stride 8 in the
+   * state), then the state region and the FP64 energies to the host */
+  hipError_t he = gd_launch_rescore64(s->pack_dev, 0, P.rep_lig, P.rep_prt, P.S.mv_best, 8, n,
+                                      e64, stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(c.host, arena.base() + s->off + s->blk.mv, sb, hipMemcpyDeviceToHost,
+                        stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(c.host + GdArena::round(sb), e64, e64_bytes, hipMemcpyDeviceToHost,
+                        stream);
+  if (he == hipSuccess && !c.done)
+    he = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
+  if (he == hipSuccess)
+    he = hipEventRecord(c.done, stream);
+  defer(eoff, eb);
+  if (he != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "read-back: %s", hipGetErrorString(he));
+  c.handle = handle;
+  return GD_OK;
+}
+
+int gd_engine::results(int handle, gd_mclog *mclog, gd_replica_result *results)
+{
+  GdSlot *s = find(handle);
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "no complex with handle %d", handle);
+  std::vector<char> host(collect_bytes(handle));
+  GdCollect c;
+  c.host = host.data();
+  c.host_bytes = host.size();
+  int rc = collect_async(handle, c);
+  hipError_t he = hipStreamSynchronize(stream);
+  if (c.done)
+    (void) hipEventDestroy(c.done);
+  if (rc)
+    return rc;
+  if (he != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "read-back: %s", hipGetErrorString(he));
+  drain_timing();
+  reap();
+  gd_parse_state(*s->img, s->blk, host.data(),
+                 (const double *) (host.data() + GdArena::round(s->state_bytes())), mclog,
+                 results);
+  return GD_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* C ABI: engine                                                             */
+/* ------------------------------------------------------------------------ */
+static gd_options default_opts(const gd_options *opt)
+{
+  gd_options o;
+  if (opt)
+    return *opt;
+  o.seed = 1;
+  o.complex_id = 0;
+  o.exchange = 1;
+  o.away = 0;
+  o.term_order = GD_TERM_ORDER_FILE;
+  o.device = 0;
+  return o;
+}
+
+extern "C" int gd_engine_create(const gd_engine_config *cfg, gd_engine **out)
+{
+  if (!out)
+    return gd_fail(GD_ERR_ARG, "null engine out pointer");
+  *out = nullptr;
+  gd_engine *e = new gd_engine();
+  int rc = e->init(cfg);
+  if (rc) {
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return GD_OK;
+}
+
+extern "C" void gd_engine_destroy(gd_engine *e) { delete e; }
+
+extern "C" int gd_engine_add(gd_engine *e, const gd_job *job, int *handle)
+{
+  if (!e || !job || !handle)
+    return gd_fail(GD_ERR_ARG, "null argument");
+  const gd_options o = default_opts(job->opt);
+  return e->add(job->lig, job->prt, job->psp, job->kde, job->mcs, job->enepara, job->temp,
+                job->replica, job->mcpara, job->size, o, handle);
+}
+
+extern "C" int gd_engine_remove(gd_engine *e, int handle)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  return e->remove(handle);
+}
+
+extern "C" int gd_engine_cycles(gd_engine *e, int n)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  return e->cycles(n);
+}
+
+extern "C" int gd_engine_sync(gd_engine *e)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  return e->sync();
+}
+
+extern "C" int gd_engine_cycles_left(gd_engine *e, int handle)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  GdSlot *s = e->find(handle);
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "no complex with handle %d", handle);
+  const int left = s->total_steps - s->steps_done;
+  return left <= 0 ? 0 : (left + s->spe - 1) / s->spe;
+}
+
+extern "C" int gd_engine_results(gd_engine *e, int handle, gd_mclog *mclog,
+                                 gd_replica_result *results)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  return e->results(handle, mclog, results);
+}
+
+extern "C" int gd_engine_autotune(gd_engine *e, gd_tune_report *rep)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  return e->autotune(rep);
+}
+
+extern "C" int gd_engine_set_timing(gd_engine *e, int on)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  e->timing = on ? 1 : 0;
+  return GD_OK;
+}
+
+extern "C" int gd_engine_timing(gd_engine *e, double *mc_ms, int *mc_launches, double *x_ms,
+                                int *x_launches)
+{
+  if (!e)
+    return gd_fail(GD_ERR_ARG, "null engine");
+  e->drain_timing();
+  if (mc_ms) *mc_ms = e->mc_ms;
+  if (mc_launches) *mc_launches = e->mc_launches;
+  if (x_ms) *x_ms = e->x_ms;
+  if (x_launches) *x_launches = e->x_launches;
+  return GD_OK;
+}
+
+extern "C" int gd_engine_layout(gd_engine *e, gd_engine_layout_info *info)
+{
+  if (!e || !info)
+    return gd_fail(GD_ERR_ARG, "null argument");
+  int rc = e->ensure_built();
+  if (rc)
+    return rc;
+  info->stage_bytes = e->lc.stage_bytes;
+  info->wpg = e->lc.wpg;
+  info->persistent = e->lc.persistent;
+  info->n_tasks = e->n_tasks;
+  info->n_chains = e->n_chain;
+  info->n_complexes = e->n_cx;
+  info->grid = e->grid();
+  info->resident_groups = e->resident;
+  info->arena_used = e->arena.used();
+  info->arena_size = e->arena.size();
+  int kde_lds = 1;
+  for (auto *s : e->slots)
+    kde_lds &= s->img->P.kde_in_lds;
+  info->kde_in_lds = kde_lds;
+  return GD_OK;
+}
+
+extern "C" int gd_job_footprint(const gd_job *job, int device, size_t *bytes)
+{
+  if (!job || !bytes)
+    return gd_fail(GD_ERR_ARG, "null argument");
+  int rc = gd_check_inputs(job->lig, job->prt, job->psp, job->kde, job->mcs, job->enepara,
+                           job->size);
+  if (rc)
+    return rc;
+  int max_lds = 65536;
+  if (gd_device_count() > 0)
+    (void) hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  GdImage I;
+  const gd_options o = default_opts(job->opt);
+  rc = gd_build_image(I, job->lig, job->prt, job->psp, job->kde, job->mcs, job->enepara,
+                      job->temp, job->replica, job->mcpara, job->size, o, max_lds, 0);
+  if (rc)
+    return rc;
+  *bytes = gd_image_footprint(I);
+  return GD_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* C ABI: gd_run (drop-in for Run(), run.cu:40-352) and the session          */
+/* ------------------------------------------------------------------------ */
+struct gd_session {
+  gd_engine eng;
+  int handle = -1;
+  int spe = 1, total = 0;
+};
+
+extern "C" int gd_session_create(const gd_ligand *lig, const gd_protein *prt,
+                                 const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                                 const gd_enepara *enepara, const gd_temp *temp,
+                                 const gd_replica *replica, const gd_mcpara *mcpara,
+                                 gd_complex_size size, const gd_options *opt,
+                                 gd_session **out)
+{
+  if (!out)
+    return gd_fail(GD_ERR_ARG, "null session out pointer");
+  *out = nullptr;
+  int rc = gd_check_inputs(lig, prt, psp, kde, mcs, enepara, size);
+  if (rc)
+    return rc;
+  if (!mcpara || !temp)
+    return gd_fail(GD_ERR_ARG, "null temp/mcpara");
+  const gd_options o = default_opts(opt);
+  gd_job job;
+  memset(&job, 0, sizeof job);
+  job.lig = lig; job.prt = prt; job.psp = psp; job.kde = kde; job.mcs = mcs;
+  job.enepara = enepara; job.temp = temp; job.replica = replica; job.mcpara = mcpara;
+  job.size = size; job.opt = &o;
+  size_t bytes = 0;
+  rc = gd_job_footprint(&job, o.device, &bytes);
+  if (rc)
+    return rc;
+  std::unique_ptr<gd_session> s(new gd_session());
+  gd_engine_config c;
+  memset(&c, 0, sizeof c);
+  c.device = o.device;
+  c.arena_bytes = bytes + ((size_t) 8 << 20) + (size_t) size.n_rep * 256;
+  c.wpg = 8;
+  c.persistent = 0;
+  c.use_graph = 0;
+  rc = s->eng.init(&c);
+  if (rc)
+    return rc;
+  rc = s->eng.add(lig, prt, psp, kde, mcs, enepara, temp, replica, mcpara, size, o,
+                  &s->handle);
+  if (rc)
+    return rc;
+  s->spe = mcpara->steps_per_exchange;
+  s->total = mcpara->steps_total;
+  *out = s.release();
+  return GD_OK;
+}
+
+extern "C" void gd_session_destroy(gd_session *s) { delete s; }
+
+/* initial energies happen at admission (gd_session_create); kept for the
+ * create / init / cycles call sequence */
+extern "C" int gd_session_init(gd_session *s)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  return GD_OK;
+}
+
+extern "C" int gd_session_cycles(gd_session *s, int n_cycles)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  return s->eng.cycles(n_cycles);
+}
+
+extern "C" int gd_session_sync(gd_session *s)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  return s->eng.sync();
+}
+
+extern "C" long long gd_session_steps_done(const gd_session *s)
+{
+  if (!s || s->eng.slots.empty())
+    return -1;
+  return s->eng.slots[0]->steps_done;
+}
+
+extern "C" int gd_session_results(gd_session *s, gd_mclog *mclog, gd_replica_result *results)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  return s->eng.results(s->handle, mclog, results);
+}
+
+extern "C" int gd_session_set_timing(gd_session *s, int on)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  s->eng.timing = on ? 1 : 0;
+  return GD_OK;
+}
+
+extern "C" int gd_session_timing(gd_session *s, double *mc_ms, int *mc_launches, double *x_ms,
+                                 int *x_launches)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  return gd_engine_timing(&s->eng, mc_ms, mc_launches, x_ms, x_launches);
+}
+
+extern "C" int gd_session_layout(const gd_session *s, int *lds_bytes, int *kde_in_lds,
+                                 int *n_tasks, int *waves_per_group)
+{
+  if (!s)
+    return gd_fail(GD_ERR_ARG, "null session");
+  gd_engine_layout_info info;
+  int rc = gd_engine_layout(const_cast<gd_engine *>(&s->eng), &info);
+  if (rc)
+    return rc;
+  if (lds_bytes) *lds_bytes = info.stage_bytes;
+  if (kde_in_lds) *kde_in_lds = info.kde_in_lds;
+  if (n_tasks) *n_tasks = info.n_tasks;
+  if (waves_per_group) *waves_per_group = info.wpg;
+  return GD_OK;
+}
+
+extern "C" int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
+                      const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *enepara,
+                      const gd_temp *temp, const gd_replica *replica,
+                      const gd_mcpara *mcpara, gd_mclog *mclog, gd_complex_size size,
+                      const gd_options *opt, gd_replica_result *results)
+{
+  auto t_begin = std::chrono::steady_clock::now();
+  gd_session *s = nullptr;
+  int rc = gd_session_create(lig, prt, psp, kde, mcs, enepara, temp, replica, mcpara, size,
+                             opt, &s);
+  if (rc)
+    return rc;
+  const int spe = mcpara->steps_per_exchange;
+  const int n_cycles = (mcpara->steps_total + spe - 1) / spe;
+  rc = s->eng.cycles(n_cycles);
+  if (!rc)
+    rc = s->eng.sync();
+  auto t_comp = std::chrono::steady_clock::now();
+  if (!rc)
+    rc = s->eng.results(s->handle, mclog, results);
+  auto t_end = std::chrono::steady_clock::now();
+  if (!rc && mclog) {
+    mclog->t0 += std::chrono::duration<double>(t_comp - t_begin).count();
+    mclog->t1 += std::chrono::duration<double>(t_end - t_begin).count();
+  }
+  gd_session_destroy(s);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* C ABI: batch (many complexes, one engine, autotuned, graph-run)           */
+/* ------------------------------------------------------------------------ */
+extern "C" int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_config *cfg,
+                            gd_replica_result **results, gd_mclog *mclogs,
+                            gd_batch_report *report)
+{
+  if (n_jobs < 0 || (n_jobs > 0 && !jobs))
+    return gd_fail(GD_ERR_ARG, "bad job array");
+  gd_batch_config bc;
+  memset(&bc, 0, sizeof bc);
+  if (cfg)
+    bc = *cfg;
+  size_t total = (size_t) 16 << 20;
+  int max_cycles = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    size_t b = 0;
+    int rc = gd_job_footprint(&jobs[i], bc.device, &b);
+    if (rc)
+      return rc;
+    total += b + (size_t) jobs[i].size.n_rep * 256;
+    const int spe = jobs[i].mcpara->steps_per_exchange;
+    max_cycles = std::max(max_cycles, (jobs[i].mcpara->steps_total + spe - 1) / spe);
+  }
+  gd_engine eng;
+  gd_engine_config c;
+  memset(&c, 0, sizeof c);
+  c.device = bc.device;
+  c.arena_bytes = total;
+  c.wpg = bc.wpg ? bc.wpg : 8;
+  c.persistent = bc.persistent;
+  c.use_graph = bc.no_graph ? 0 : 1;
+  c.kde_policy = bc.kde_policy;
+  int rc = eng.init(&c);
+  if (rc)
+    return rc;
+  std::vector<int> handles(n_jobs);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < n_jobs; ++i) {
+    const gd_options o = default_opts(jobs[i].opt);
+    rc = eng.add(jobs[i].lig, jobs[i].prt, jobs[i].psp, jobs[i].kde, jobs[i].mcs,
+                 jobs[i].enepara, jobs[i].temp, jobs[i].replica, jobs[i].mcpara, jobs[i].size,
+                 o, &handles[i]);
+    if (rc)
+      return rc;
+  }
+  rc = eng.sync();
+  if (rc)
+    return rc;
+  auto t1 = std::chrono::steady_clock::now();
+  gd_tune_report tr;
+  memset(&tr, 0, sizeof tr);
+  if (bc.autotune) {
+    rc = eng.autotune(&tr);
+    if (rc)
+      return rc;
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  rc = eng.cycles(max_cycles);
+  if (!rc)
+    rc = eng.sync();
+  if (rc)
+    return rc;
+  auto t3 = std::chrono::steady_clock::now();
+  for (int i = 0; i < n_jobs; ++i) {
+    rc = eng.results(handles[i], mclogs ? &mclogs[i] : nullptr, results ? results[i] : nullptr);
+    if (rc)
+      return rc;
+  }
+  auto t4 = std::chrono::steady_clock::now();
+  if (report) {
+    report->admit_s = std::chrono::duration<double>(t1 - t0).count();
+    report->tune_s = std::chrono::duration<double>(t2 - t1).count();
+    report->run_s = std::chrono::duration<double>(t3 - t2).count();
+    report->results_s = std::chrono::duration<double>(t4 - t3).count();
+    report->cycles = max_cycles;
+    report->tune = tr;
+    gd_engine_layout_info info;
+    if (gd_engine_layout(&eng, &info) == GD_OK)
+      report->layout = info;
+  }
+  return GD_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* gd_score_poses                                                            */
+/* ------------------------------------------------------------------------ */
+extern "C" int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
+                              const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                              const gd_enepara *enepara, gd_complex_size size,
+                              int n_pose, const int *lig_conf, const int *prt_conf,
+                              const float *move, int term_order, int device,
+                              float *energy, double *energy_fp64)
+{
+  int rc = gd_check_inputs(lig, prt, psp, kde, mcs, enepara, size);
+  if (rc)
+    return rc;
+  if (n_pose < 0 || (n_pose > 0 && (!lig_conf || !prt_conf || !move)))
+    return gd_fail(GD_ERR_ARG, "bad pose arrays");
+  if (!energy && !energy_fp64)
+    return gd_fail(GD_ERR_ARG, "no output requested");
+  for (int i = 0; i < n_pose; ++i)
+    if (lig_conf[i] < 0 || lig_conf[i] >= size.n_lig || prt_conf[i] < 0 ||
+        prt_conf[i] >= size.n_prt)
+      return gd_fail(GD_ERR_ARG, "pose %d: conformer index out of range", i);
+  const int ndev = gd_device_count();
+  if (ndev < 1)
+    return gd_fail(GD_ERR_NODEV, "no HIP device visible");
+  if (device < 0 || device >= ndev)
+    return gd_fail(GD_ERR_ARG, "device %d of %d", device, ndev);
+  if (n_pose == 0)
+    return GD_OK;
+  GD_HIP_TRY(hipSetDevice(device));
+  int max_lds = 0;
+  GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  gd_options o = default_opts(nullptr);
+  o.term_order = term_order;
+  o.device = device;
+  /* the pose arrays ride in the image as a 1-temperature replica set */
+  gd_complex_size sz1 = size;
+  sz1.n_tmp = 1;
+  sz1.n_rep = size.n_lig * size.n_prt;
+  GdImage I;
+  rc = gd_build_image(I, lig, prt, psp, kde, mcs, enepara, nullptr, nullptr, nullptr, sz1, o,
+                      max_lds, 0);
+  if (rc)
+    return rc;
+  std::vector<int> order(n_pose);
+  for (int i = 0; i < n_pose; ++i)
+    order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return prt_conf[a] < prt_conf[b]; });
+  std::vector<gd_task> tasks;
+  for (int i = 0; i < n_pose;) {
+    int j = i;
+    while (j < n_pose && prt_conf[order[j]] == prt_conf[order[i]] && j - i < 8)
+      ++j;
+    tasks.push_back(gd_task{0, prt_conf[order[i]], i, j - i});
+    i = j;
+  }
+  GdArena arena;
+  const size_t img = gd_image_footprint(I);
+  const size_t extra = GdArena::round(vbytes(tasks)) + GdArena::round(n_pose * 4) * 3 +
+                       GdArena::round((size_t) n_pose * 24) +
+                       GdArena::round((size_t) n_pose * 40) +
+                       GdArena::round((size_t) n_pose * 80) + 4096;
+  GD_HIP_TRY(arena.init(img + extra));
+  struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard() { if (s) (void) hipStreamDestroy(s); }
+  } sg;
+  GD_HIP_TRY(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+  hipStream_t st = sg.s;
+  char *B = arena.base();
+  /* every copy and launch is ordered on st; the host buffers outlive the
+   * final synchronisation */
+  const size_t o_img = arena.alloc(img);
+  std::vector<char> host(img);
+  GdBlock b;
+  gd_pack_image(I, host.data(), B + o_img, b);
+  const gd_pack *pack_dev = (const gd_pack *) (B + o_img + b.pack);
+  hipError_t he = hipMemcpyAsync(B + o_img, host.data(), img, hipMemcpyHostToDevice, st);
+  auto put = [&](const void *src, size_t bytes, size_t *off) {
+    *off = arena.alloc(bytes);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(B + *off, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  size_t o_tasks, o_order, o_lc, o_pc, o_mv;
+  put(tasks.data(), vbytes(tasks), &o_tasks);
+  put(order.data(), (size_t) n_pose * 4, &o_order);
+  put(lig_conf, (size_t) n_pose * 4, &o_lc);
+  put(prt_conf, (size_t) n_pose * 4, &o_pc);
+  put(move, (size_t) n_pose * 24, &o_mv);
+  const size_t o_e = arena.alloc((size_t) n_pose * 40);
+  const size_t o_e64 = arena.alloc((size_t) n_pose * 80);
+  if (o_e64 == (size_t) -1)
+    return gd_fail(GD_ERR_NOMEM, "score: workspace sizing");
+  if (he == hipSuccess && energy) {
+    he = gd_set_kernel_attrs(I.P.lds_bytes);
+    if (he == hipSuccess)
+      he = gd_launch_score(pack_dev, (const gd_task *) (B + o_tasks), (int) tasks.size(),
+                           (const int *) (B + o_order), (const int *) (B + o_lc),
+                           (const float *) (B + o_mv), I.P.lds_bytes, (float *) (B + o_e), st);
+  }
+  if (he == hipSuccess && energy_fp64)
+    he = gd_launch_rescore64(pack_dev, 0, (const int *) (B + o_lc), (const int *) (B + o_pc),
+                             (const float *) (B + o_mv), 6, n_pose, (double *) (B + o_e64), st);
+  if (he == hipSuccess && energy)
+    he = hipMemcpyAsync(energy, B + o_e, (size_t) n_pose * 40, hipMemcpyDeviceToHost, st);
+  if (he == hipSuccess && energy_fp64)
+    he = hipMemcpyAsync(energy_fp64, B + o_e64, (size_t) n_pose * 80, hipMemcpyDeviceToHost,
+                        st);
+  {
+    const hipError_t hs = hipStreamSynchronize(st);
+    if (he == hipSuccess)
+      he = hs;
+  }
+  if (he != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "score: %s", hipGetErrorString(he));
+  return GD_OK;
+}

@@ -1,32 +1,40 @@
 /*
  * gd_kernels.hip -- CDNA4 (gfx950) kernels of the REMC docking core.
  *
- *   gd_mc_kernel      fused Move + CalcEnergy + Accept for a run of MC
- *                     steps (montecarlo_d.cu:59-106 with move_d.cu:73-134,
- *                     calcenergy_d.cu:22-374, accept_d.cu:12-52), or the
- *                     initial energies (MonteCarlo_Init_d, :12-55).
+ *   gd_mc_kernel       fused Move + CalcEnergy + Accept for one cycle of MC
+ *                      steps of every replica of every resident complex
+ *                      (montecarlo_d.cu:59-106 with move_d.cu:73-134,
+ *                      calcenergy_d.cu:22-374, accept_d.cu:12-52), or the
+ *                      initial energies (MonteCarlo_Init_d, :12-55).
  *   gd_exchange_kernel temperature swaps, one lane per chain
- *                     (exchangereplicas_d.cu:7-136).
- *   gd_score_kernel   energies of caller-given poses (FP32).
+ *                      (exchangereplicas_d.cu:7-136).
+ *   gd_advance_kernel  advances each complex's device step counter after a
+ *                      cycle and re-arms the work queue.
+ *   gd_score_kernel    energies of caller-given poses (FP32).
  *   gd_rescore64_kernel FP64 energies of caller-given poses.
  *
- * Execution model: a workgroup is a task of up to GD_WPG replicas that share
- * one protein conformer.  The workgroup stages that conformer's pocket
- * points, the pair-parameter rows of the ligand's atom types and (when it
- * fits) the type-sorted KDE points in LDS once; then each wavefront owns one
- * replica for all steps of the launch.  Lanes run over pocket points (and
- * over KDE points / ligand atoms in the smaller loops); the ligand atom of
- * the outer loop is broadcast from the lane that holds it with v_readlane,
- * so it sits in scalar registers.  Per-term sums are per-lane FP32
- * accumulators reduced by an xor butterfly -- the order the CPU replay
- * reproduces bit for bit.
+ * Execution model: a workgroup takes a task -- up to WPG replicas of one
+ * protein conformer of one complex -- either by its block index or, in the
+ * persistent form, from a device work queue (an atomic counter over a task
+ * list sorted by cost).  It stages that conformer's pocket points, the
+ * pair-parameter rows of the ligand's atom types and (when it fits) the
+ * type-grouped KDE points in LDS, restaging only when the next task needs
+ * another conformer; each wavefront then owns one replica for the whole
+ * cycle.  Lanes run over pocket points (over KDE points / ligand atoms in
+ * the smaller loops); the ligand atom of the outer loop is broadcast from
+ * the lane that holds it with v_readlane, so it sits in scalar registers.
+ * The replica's uniform state (movematrices, energies) lives in a per-wave
+ * LDS slot, not in vector registers.  Per-term sums are per-lane FP32
+ * accumulators combined with the near-first pairing tree the CPU replay
+ * reproduces bit for bit.  The step index of each complex lives in device
+ * memory, so the three launches of a cycle (MC, exchange, advance) replay
+ * unchanged from a captured hipGraph.
  */
 #include <hip/hip_runtime.h>
 #include "gd_device.h"
+#include "gd_launch.h"
 
-#define GD_WPG 8  /* wavefronts per workgroup */
-
-__device__ __forceinline__ float gd_bcast_early(float v, int lane)
+__device__ __forceinline__ float gd_bcast(float v, int lane)
 {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
@@ -38,12 +46,12 @@ __device__ __forceinline__ float gd_dpp(float v)
                                                     false));
 }
 
-/* Sum over the 64 lanes with the pairing tree of an xor butterfly
- * (1, 2, 4, 8, 16, 32): DPP quad_perm [1,0,3,2] and [2,3,0,1] pair lanes
- * j^1 and j^2, row_half_mirror pairs the two quads of each 8-lane half-row,
- * row_mirror the two halves of each 16-lane row; the four row sums are
- * then added as (r0 + r1) + (r2 + r3).  Every partial sum is the same
- * float addition as in the butterfly (addition commutes), so the result is
+/* Sum over the 64 lanes with the pairing tree of an xor butterfly taken
+ * near-first (lane pairs j^1, then j^2, j^4, j^8, j^16, j^32): DPP
+ * quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror (pairs the two quads
+ * of a half-row), row_mirror (pairs the two halves of a row), then the four
+ * row sums as (r0 + r1) + (r2 + r3).  Every partial is the same float
+ * addition as in that butterfly (addition commutes), so the result is
  * bit-identical to it; it is returned uniform. */
 __device__ __forceinline__ float gd_wave_sum(float v)
 {
@@ -51,8 +59,8 @@ __device__ __forceinline__ float gd_wave_sum(float v)
   v += gd_dpp<0x4E>(v);
   v += gd_dpp<0x141>(v);
   v += gd_dpp<0x140>(v);
-  const float r0 = gd_bcast_early(v, 0), r1 = gd_bcast_early(v, 16);
-  const float r2 = gd_bcast_early(v, 32), r3 = gd_bcast_early(v, 48);
+  const float r0 = gd_bcast(v, 0), r1 = gd_bcast(v, 16);
+  const float r2 = gd_bcast(v, 32), r3 = gd_bcast(v, 48);
   return (r0 + r1) + (r2 + r3);
 }
 
@@ -64,29 +72,18 @@ __device__ __forceinline__ double gd_wave_sum64(double v)
   return v;
 }
 
-__device__ __forceinline__ float gd_bcast(float v, int lane)
-{
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-/* LDS carve-up, offsets in bytes (host computes them with the same rule). */
-struct gd_lds_layout {
-  int off_eh, off_tab, off_kde, bytes;
-};
-
 /* ------------------------------------------------------------------------ */
 /* Workgroup staging                                                         */
 /* ------------------------------------------------------------------------ */
-template <bool KDE_LDS>
-__device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem,
-                                         const gd_lds_layout &L)
+__device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
 {
   float4 *s_pos = (float4 *) smem;
-  float2 *s_eh = (float2 *) (smem + L.off_eh);
-  float4 *s_tab = (float4 *) (smem + L.off_tab);
-  const float4 *g_pos = P.pt_pos + (size_t) prt * P.pnp_pad;
-  const float2 *g_eh = P.pt_eh + (size_t) prt * P.pnp_pad;
-  for (int i = threadIdx.x; i < P.pnp_pad; i += blockDim.x) {
+  float2 *s_eh = (float2 *) (smem + P.off_eh);
+  float4 *s_tab = (float4 *) (smem + P.off_tab);
+  const int pnp_pad = P.pnp_pad;
+  const float4 *g_pos = P.pt_pos + (size_t) prt * pnp_pad;
+  const float2 *g_eh = P.pt_eh + (size_t) prt * pnp_pad;
+  for (int i = threadIdx.x; i < pnp_pad; i += blockDim.x) {
     s_pos[i] = g_pos[i];
     s_eh[i] = g_eh[i];
   }
@@ -94,9 +91,10 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem,
   const int ntab4 = P.n_rows * GD_ROW_STRIDE * 2;
   for (int i = threadIdx.x; i < ntab4; i += blockDim.x)
     s_tab[i] = g_tab[i];
-  if (KDE_LDS) {
-    float4 *s_kde = (float4 *) (smem + L.off_kde);
-    for (int i = threadIdx.x; i < P.pnk; i += blockDim.x)
+  if (P.kde_in_lds) {
+    float4 *s_kde = (float4 *) (smem + P.off_kde);
+    const int pnk = P.pnk;
+    for (int i = threadIdx.x; i < pnk; i += blockDim.x)
       s_kde[i] = P.kde[i];
   }
 }
@@ -107,14 +105,19 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem,
 /* On return out[] (slot order) is identical in every lane.                  */
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void gd_energy_wave(
-    const gd_pack &P, const float4 *__restrict__ s_pos,
-    const float2 *__restrict__ s_eh, const gd_pair_param *__restrict__ s_tab,
-    const float4 *__restrict__ kde, float ax0, float ay0, float az0, float ax1,
+    const gd_pack &P, const char *smem, float ax0, float ay0, float az0, float ax1,
     float ay1, float az1, const float cen[3], float out[GD_E_N])
 {
+  const float4 *__restrict__ s_pos = (const float4 *) smem;
+  const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
+  const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
+  const float4 *__restrict__ kde =
+      P.kde_in_lds ? (const float4 *) (smem + P.off_kde) : P.kde;
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
+  const int n_res = P.n_res;
+  const gd_pair_scalars sc = P.sc;
 
   gd_pair_acc acc;
   acc.vdw = 0.0f; acc.ele = 0.0f; acc.pmf = 0.0f; acc.psp = 0.0f; acc.hdb = 0.0f;
@@ -131,7 +134,7 @@ __device__ __forceinline__ void gd_energy_wave(
     const int lt = P.lig_type[l];
     const float lq = P.lig_q[l];
     const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
-    const float *psp_row = P.psp + (size_t) lt * P.n_res;
+    const float *psp_row = P.psp + (size_t) lt * n_res;
     float hacc = 0.0f;
     for (int k = 0; k < nchunk; ++k) {
       const int p = lane + k * GD_WAVE;
@@ -145,14 +148,15 @@ __device__ __forceinline__ void gd_energy_wave(
       if ((bits & GD_PT_HPC) && d2 <= 81.0f)
         hacc += gd_hpc_pair(d2, eh.y);
       const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
-      gd_pair_terms(d2, lq * eh.x, &pp, &P.sc, c2, pv, &acc);
+      gd_pair_terms(d2, lq * eh.x, &pp, &sc, c2, pv, &acc);
     }
     const float hs = gd_wave_sum(hacc);
     const float z = (hs - P.hpl0[lt]) * P.inv_hpl1[lt];
     ehpc += 0.5f * z * z - P.hpl2[lt];
   }
 
-  /* KDE (calcenergy_d.cu:175-213) over the type-sorted segment */
+  /* KDE (calcenergy_d.cu:175-213) over the type-grouped segment */
+  const float kde2 = P.kde2;
   float ekde = 0.0f;
   for (int l = 0; l < lna; ++l) {
     const int src = l & (GD_WAVE - 1);
@@ -167,7 +171,7 @@ __device__ __forceinline__ void gd_energy_wave(
     for (int i = s0 + lane; i < s1; i += GD_WAVE) {
       const float4 q = kde[i];
       const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      v += gd_exp_hot(P.kde2 * d2);
+      v += gd_exp_hot(kde2 * d2);
     }
     v = gd_wave_sum(v);
     if (s1 > s0)
@@ -177,7 +181,8 @@ __device__ __forceinline__ void gd_energy_wave(
 
   /* position restraints (calcenergy_d.cu:229-270) */
   float elhm = 0.0f;
-  for (int m = 0; m < P.pos; ++m) {
+  const int pos = P.pos;
+  for (int m = 0; m < pos; ++m) {
     float s = 0.0f, c = 0.0f;
     const float4 *tm = P.mcs + (size_t) m * lna;
     if (lane < lna) {
@@ -199,7 +204,7 @@ __device__ __forceinline__ void gd_energy_wave(
     if (c > 0.0f)
       elhm += P.mcs_tcc[m] * gd_sqrt(s / c);
   }
-  elhm = (P.pos > 0) ? gd_log(elhm / (float) P.pos) : 0.0f;
+  elhm = (pos > 0) ? gd_log(elhm / (float) pos) : 0.0f;
 
   /* distance to the pocket centre (calcenergy_d.cu:276-283) */
   const float edst = gd_sqrt(gd_dist2(cen[0] - P.pocket_center[0],
@@ -229,125 +234,135 @@ __device__ __forceinline__ void gd_place(const gd_pack &P, const float m[6],
 {
   float rot[9];
   gd_rotation(m, rot);
-  gd_transform(rot, m, P.center, bx0, by0, bz0, ax0, ay0, az0);
-  gd_transform(rot, m, P.center, bx1, by1, bz1, ax1, ay1, az1);
-  cen[0] = P.center[0] + m[0];
-  cen[1] = P.center[1] + m[1];
-  cen[2] = P.center[2] + m[2];
+  const float c[3] = {P.center[0], P.center[1], P.center[2]};
+  gd_transform(rot, m, c, bx0, by0, bz0, ax0, ay0, az0);
+  gd_transform(rot, m, c, bx1, by1, bz1, ax1, ay1, az1);
+  cen[0] = c[0] + m[0];
+  cen[1] = c[1] + m[1];
+  cen[2] = c[2] + m[2];
 }
 
-/* ------------------------------------------------------------------------ */
-/* MC kernel                                                                 */
-/* mode 0: n_steps MC steps from global step s_begin                         */
-/* mode 1: initial energy of the current movematrix                          */
-/* mode 2: initial displacement by 44.5 * move_scale, forced accept          */
-/* ------------------------------------------------------------------------ */
-template <bool KDE_LDS>
-__global__ void __launch_bounds__(GD_WAVE *GD_WPG)
-gd_mc_kernel(gd_pack P, gd_state S, const gd_task *__restrict__ tasks,
-             const int *__restrict__ rep_lig, gd_lds_layout L, int s_begin,
-             int n_steps, uint32_t seed, uint32_t cid, int mode)
+/* Centroid-relative conformer coordinates, two atoms per lane. */
+__device__ __forceinline__ void gd_load_conformer(const gd_pack &P, int conf, float *bx0,
+                                                  float *by0, float *bz0, float *bx1,
+                                                  float *by1, float *bz1)
 {
-  extern __shared__ __attribute__((aligned(16))) char smem[];
-  const gd_task task = tasks[blockIdx.x];
-  gd_stage<KDE_LDS>(P, task.prt, smem, L);
-  __syncthreads();
-
-  const int w = threadIdx.x / GD_WAVE;
-  if (w >= task.r_count)
-    return;  /* no barrier follows */
   const int lane = threadIdx.x & (GD_WAVE - 1);
-  const int r = task.r_begin + w;
-
-  const float4 *s_pos = (const float4 *) smem;
-  const float2 *s_eh = (const float2 *) (smem + L.off_eh);
-  const gd_pair_param *s_tab = (const gd_pair_param *) (smem + L.off_tab);
-  const float4 *kde = KDE_LDS ? (const float4 *) (smem + L.off_kde) : P.kde;
-
-  /* conformer coordinates (centroid-relative), two atoms per lane */
-  const float4 *cx = P.lig_xyz + (size_t) rep_lig[r] * P.lna;
-  float bx0 = 0.0f, by0 = 0.0f, bz0 = 0.0f, bx1 = 0.0f, by1 = 0.0f, bz1 = 0.0f;
+  const float4 *cx = P.lig_xyz + (size_t) conf * P.lna;
+  *bx0 = *by0 = *bz0 = *bx1 = *by1 = *bz1 = 0.0f;
   if (lane < P.lna) {
     const float4 v = cx[lane];
-    bx0 = v.x; by0 = v.y; bz0 = v.z;
+    *bx0 = v.x; *by0 = v.y; *bz0 = v.z;
   }
   if (lane + GD_WAVE < P.lna) {
     const float4 v = cx[lane + GD_WAVE];
-    bx1 = v.x; by1 = v.y; bz1 = v.z;
+    *bx1 = v.x; *by1 = v.y; *bz1 = v.z;
   }
+}
 
-  float m_old[6];
+/* Steps of this cycle for a complex: min(spe, steps_total - step). */
+__device__ __forceinline__ int gd_cycle_steps(const gd_pack &P, int s0)
+{
+  const int left = P.steps_total - s0;
+  return left < P.spe ? left : P.spe;
+}
+
+/* ------------------------------------------------------------------------ */
+/* One replica, one wavefront.                                               */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *smem, int r)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const gd_state S = P.S;
+  float bx0, by0, bz0, bx1, by1, bz1;
+  gd_load_conformer(P, P.rep_lig[r], &bx0, &by0, &bz0, &bx1, &by1, &bz1);
+  float m[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k)
-    m_old[k] = S.mv[r * 8 + k];
-
-  float ax0, ay0, az0, ax1, ay1, az1, cen[3];
-  float e_new[GD_E_N];
-
-  if (mode != 0) {
-    if (mode == 2) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-        m_old[k] = P.move_scale[k] * 44.5f + m_old[k];
-    }
-    gd_place(P, m_old, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1,
-             &ay1, &az1, cen);
-    gd_energy_wave(P, s_pos, s_eh, s_tab, kde, ax0, ay0, az0, ax1, ay1, az1, cen,
-                   e_new);
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        S.mv[r * 8 + k] = m_old[k];
-        S.mv_best[r * 8 + k] = m_old[k];
-      }
-#pragma unroll
-      for (int i = 0; i < GD_E_N; ++i) {
-        S.e[r * 10 + i] = e_new[i];
-        S.e_best[r * 10 + i] = e_new[i];
-      }
-      S.step_best[r] = -1;
-    }
-    return;
-  }
-
-  float e_old[GD_E_N];
-#pragma unroll
-  for (int i = 0; i < GD_E_N; ++i)
-    e_old[i] = S.e[r * 10 + i];
-  float best_total = S.e_best[r * 10 + GD_E_TOTAL];
-  const float mb = P.minus_beta[S.idx_tmp[r]];
-  int acs = 0;
-
-  for (int s = s_begin; s < s_begin + n_steps; ++s) {
-    const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
-    const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
-    const uint32_t wds[7] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3], rb.v[0], rb.v[1], rb.v[2]};
-    float m_new[6];
+    m[k] = S.mv[r * 8 + k];
+  if (P.away) {
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      m_new[k] = P.move_scale[k] * (2.0f * gd_u01(wds[k]) - 1.0f) + m_old[k];
+      m[k] = P.move_scale[k] * 44.5f + m[k];
+  }
+  float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N];
+  gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
+  gd_energy_wave(P, smem, ax0, ay0, az0, ax1, ay1, az1, cen, e);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      S.mv[r * 8 + k] = m[k];
+      S.mv_best[r * 8 + k] = m[k];
+    }
+#pragma unroll
+    for (int i = 0; i < GD_E_N; ++i) {
+      S.e[r * 10 + i] = e[i];
+      S.e_best[r * 10 + i] = e[i];
+    }
+    S.step_best[r] = -1;
+  }
+}
 
-    gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1,
-             &ay1, &az1, cen);
-    gd_energy_wave(P, s_pos, s_eh, s_tab, kde, ax0, ay0, az0, ax1, ay1, az1, cen,
-                   e_new);
+/* n_steps MC steps from step s0.  The movematrices and energies live in
+ * the wave's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in registers. */
+__device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *smem,
+                                                 float *ws, int r, int s0, int n_steps)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const gd_state S = P.S;
+  float bx0, by0, bz0, bx1, by1, bz1;
+  gd_load_conformer(P, P.rep_lig[r], &bx0, &by0, &bz0, &bx1, &by1, &bz1);
+  if (lane < 6)
+    ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
+  if (lane < GD_E_N)
+    ws[GD_WS_EOLD + lane] = S.e[r * 10 + lane];
+  __builtin_amdgcn_wave_barrier();
+  float e_old_total = S.e[r * 10 + GD_E_TOTAL];
+  float best_total = S.e_best[r * 10 + GD_E_TOTAL];
+  const float mb = P.minus_beta[S.idx_tmp[r]];
+  const uint32_t seed = P.seed, cid = P.cid;
+  int acs = 0;
 
-    const float de = e_new[GD_E_TOTAL] - e_old[GD_E_TOTAL];
-    const bool accept = gd_u01(wds[6]) < gd_exp(de * mb);
-    if (accept) {
-      ++acs;
+  for (int s = s0; s < s0 + n_steps; ++s) {
+    const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
+    const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
+    float m_new[6];
+    m_new[0] = P.move_scale[0] * (2.0f * gd_u01(ra.v[0]) - 1.0f) + ws[GD_WS_MOLD + 0];
+    m_new[1] = P.move_scale[1] * (2.0f * gd_u01(ra.v[1]) - 1.0f) + ws[GD_WS_MOLD + 1];
+    m_new[2] = P.move_scale[2] * (2.0f * gd_u01(ra.v[2]) - 1.0f) + ws[GD_WS_MOLD + 2];
+    m_new[3] = P.move_scale[3] * (2.0f * gd_u01(ra.v[3]) - 1.0f) + ws[GD_WS_MOLD + 3];
+    m_new[4] = P.move_scale[4] * (2.0f * gd_u01(rb.v[0]) - 1.0f) + ws[GD_WS_MOLD + 4];
+    m_new[5] = P.move_scale[5] * (2.0f * gd_u01(rb.v[1]) - 1.0f) + ws[GD_WS_MOLD + 5];
+    const float u_acc = gd_u01(rb.v[2]);
+
+    float ax0, ay0, az0, ax1, ay1, az1, cen[3], e_new[GD_E_N];
+    gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
+             &az1, cen);
+    if (lane == 0) {
 #pragma unroll
       for (int k = 0; k < 6; ++k)
-        m_old[k] = m_new[k];
+        ws[GD_WS_MNEW + k] = m_new[k];
+    }
+    gd_energy_wave(P, smem, ax0, ay0, az0, ax1, ay1, az1, cen, e_new);
+
+    const float de = e_new[GD_E_TOTAL] - e_old_total;
+    if (u_acc < gd_exp(de * mb)) {
+      ++acs;
+      e_old_total = e_new[GD_E_TOTAL];
+      if (lane == 0) {
 #pragma unroll
-      for (int i = 0; i < GD_E_N; ++i)
-        e_old[i] = e_new[i];
+        for (int i = 0; i < GD_E_N; ++i)
+          ws[GD_WS_EOLD + i] = e_new[i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          ws[GD_WS_MOLD + k] = ws[GD_WS_MNEW + k];
+      }
       if (e_new[GD_E_TOTAL] < best_total) {
         best_total = e_new[GD_E_TOTAL];
         if (lane == 0) {
 #pragma unroll
           for (int k = 0; k < 6; ++k)
-            S.mv_best[r * 8 + k] = m_new[k];
+            S.mv_best[r * 8 + k] = ws[GD_WS_MNEW + k];
 #pragma unroll
           for (int i = 0; i < GD_E_N; ++i)
             S.e_best[r * 10 + i] = e_new[i];
@@ -355,16 +370,73 @@ gd_mc_kernel(gd_pack P, gd_state S, const gd_task *__restrict__ tasks,
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
   }
 
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-      S.mv[r * 8 + k] = m_old[k];
-#pragma unroll
-    for (int i = 0; i < GD_E_N; ++i)
-      S.e[r * 10 + i] = e_old[i];
+  if (lane < 6)
+    S.mv[r * 8 + lane] = ws[GD_WS_MOLD + lane];
+  if (lane < GD_E_N)
+    S.e[r * 10 + lane] = ws[GD_WS_EOLD + lane];
+  if (lane == 0)
     S.acs_mc[r] += acs;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MC kernel                                                                 */
+/* mode 0: one cycle (steps from each complex's device counter)              */
+/* mode 1: initial energies (44.5 * move_scale displacement if P.away)       */
+/* queue != nullptr: persistent -- tasks are taken from *queue until it      */
+/* passes n_tasks (every workgroup exits once the list is drained).          */
+/* ------------------------------------------------------------------------ */
+template <int WPG>
+__global__ void __launch_bounds__(GD_WAVE *WPG)
+gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ tasks,
+             int n_tasks, int *queue, int mode, int wslot_off)
+{
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ int s_task;
+  float *ws = (float *) (smem + wslot_off) + (threadIdx.x / GD_WAVE) * GD_WSLOT_FLOATS;
+  const int w = threadIdx.x / GD_WAVE;
+  int staged_cx = -1, staged_prt = -1;
+  int t = queue ? -1 : (int) blockIdx.x;
+
+  for (;;) {
+    if (queue) {
+      if (threadIdx.x == 0)
+        s_task = atomicAdd(queue, 1);
+      __syncthreads();
+      t = s_task;
+      __syncthreads();
+    } else if (t != (int) blockIdx.x) {
+      break;  /* grid form: one task per workgroup */
+    }
+    if (t >= n_tasks)
+      break;
+    const gd_task task = tasks[t];
+    const gd_pack &P = packs[task.cx];
+    int s0 = 0, n_steps = 0;
+    if (mode == 0) {
+      s0 = P.step[0];
+      n_steps = gd_cycle_steps(P, s0);
+    }
+    if (mode != 0 || n_steps > 0) {
+      if (task.cx != staged_cx || task.prt != staged_prt) {
+        __syncthreads();
+        gd_stage(P, task.prt, smem);
+        __syncthreads();
+        staged_cx = task.cx;
+        staged_prt = task.prt;
+      }
+      if (w < task.r_count) {
+        const int r = task.r_begin + w;
+        if (mode == 0)
+          gd_replica_cycle(P, smem, ws, r, s0, n_steps);
+        else
+          gd_replica_init(P, smem, r);
+      }
+    }
+    if (!queue)
+      t = -2;
   }
 }
 
@@ -374,16 +446,23 @@ gd_mc_kernel(gd_pack P, gd_state S, const gd_task *__restrict__ tasks,
 #define GD_MAXTMP_DEV 64
 
 __global__ void __launch_bounds__(256)
-gd_exchange_kernel(gd_pack P, gd_state S, int x, uint32_t seed, uint32_t cid)
+gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict__ chains,
+                   int n_chain)
 {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n_chain = P.n_prt * P.n_lig;
-  if (c >= n_chain)
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_chain)
     return;
+  const gd_chain ch = chains[g];
+  const gd_pack &P = packs[ch.cx];
+  const int n_tmp = P.n_tmp;
+  if (!P.exchange || n_tmp < 2 || gd_cycle_steps(P, P.step[0]) <= 0)
+    return;
+  const int x = P.step[1];
+  const int c = ch.c;
   const int p = c / P.n_lig;
   const int l = c - p * P.n_lig;
-  const int n_tmp = P.n_tmp;
   const int base = n_tmp * P.n_lig * p + l;
+  const gd_state S = P.S;
   int temps[GD_MAXTMP_DEV];
   int orders[GD_MAXTMP_DEV];
   float ener[GD_MAXTMP_DEV];
@@ -398,7 +477,8 @@ gd_exchange_kernel(gd_pack P, gd_state S, int x, uint32_t seed, uint32_t cid)
   gd_u32x4 u4;
   for (int j = 0; j < maxpair; ++j) {
     if ((j & 3) == 0)
-      u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2), seed, cid);
+      u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2),
+                            P.seed, P.cid);
     const int i1 = 2 * j + mode;
     const int i2 = i1 + 1;
     const int o1 = orders[i1];
@@ -416,48 +496,52 @@ gd_exchange_kernel(gd_pack P, gd_state S, int x, uint32_t seed, uint32_t cid)
     S.idx_tmp[base + P.n_lig * t] = temps[t];
 }
 
+/* After MC and exchange: step[0] += steps of this cycle, step[1] += 1;
+ * thread 0 re-arms the persistent work queue. */
+__global__ void __launch_bounds__(256)
+gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && queue)
+    queue[0] = 0;
+  if (i >= n_cx)
+    return;
+  const gd_pack &P = packs[i];
+  const int s0 = P.step[0];
+  const int n = gd_cycle_steps(P, s0);
+  if (n > 0) {
+    P.step[0] = s0 + n;
+    P.step[1] = P.step[1] + 1;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
-/* Energies of given poses (FP32).  Task r range indexes poses.             */
+/* Energies of given poses (FP32).                                           */
 /* ------------------------------------------------------------------------ */
-template <bool KDE_LDS>
-__global__ void __launch_bounds__(GD_WAVE *GD_WPG)
-gd_score_kernel(gd_pack P, const gd_task *__restrict__ tasks,
+__global__ void __launch_bounds__(GD_WAVE * 8)
+gd_score_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ tasks,
                 const int *__restrict__ pose_ids, const int *__restrict__ lig_conf,
-                const float *__restrict__ move, gd_lds_layout L,
-                float *__restrict__ energy)
+                const float *__restrict__ move, float *__restrict__ energy)
 {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const gd_task task = tasks[blockIdx.x];
-  gd_stage<KDE_LDS>(P, task.prt, smem, L);
+  const gd_pack &P = packs[task.cx];
+  gd_stage(P, task.prt, smem);
   __syncthreads();
   const int w = threadIdx.x / GD_WAVE;
   if (w >= task.r_count)
     return;
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int i = pose_ids[task.r_begin + w];
-  const float4 *s_pos = (const float4 *) smem;
-  const float2 *s_eh = (const float2 *) (smem + L.off_eh);
-  const gd_pair_param *s_tab = (const gd_pair_param *) (smem + L.off_tab);
-  const float4 *kde = KDE_LDS ? (const float4 *) (smem + L.off_kde) : P.kde;
-
-  const float4 *cx = P.lig_xyz + (size_t) lig_conf[i] * P.lna;
-  float bx0 = 0.0f, by0 = 0.0f, bz0 = 0.0f, bx1 = 0.0f, by1 = 0.0f, bz1 = 0.0f;
-  if (lane < P.lna) {
-    const float4 v = cx[lane];
-    bx0 = v.x; by0 = v.y; bz0 = v.z;
-  }
-  if (lane + GD_WAVE < P.lna) {
-    const float4 v = cx[lane + GD_WAVE];
-    bx1 = v.x; by1 = v.y; bz1 = v.z;
-  }
+  float bx0, by0, bz0, bx1, by1, bz1;
+  gd_load_conformer(P, lig_conf[i], &bx0, &by0, &bz0, &bx1, &by1, &bz1);
   float m[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k)
     m[k] = move[i * 6 + k];
   float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N];
-  gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
-           &az1, cen);
-  gd_energy_wave(P, s_pos, s_eh, s_tab, kde, ax0, ay0, az0, ax1, ay1, az1, cen, e);
+  gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
+  gd_energy_wave(P, smem, ax0, ay0, az0, ax1, ay1, az1, cen, e);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < GD_E_N; ++k)
@@ -470,18 +554,20 @@ gd_score_kernel(gd_pack P, const gd_task *__restrict__ tasks,
 /* one wavefront per pose, reading the pack straight from HBM.              */
 /* ------------------------------------------------------------------------ */
 __global__ void __launch_bounds__(GD_WAVE)
-gd_rescore64_kernel(gd_pack P, const int *__restrict__ lig_conf,
-                    const int *__restrict__ prt_conf, const float *__restrict__ move,
-                    int n_pose, double *__restrict__ energy)
+gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
+                    const int *__restrict__ lig_conf, const int *__restrict__ prt_conf,
+                    const float *__restrict__ move, int move_stride, int n_pose,
+                    double *__restrict__ energy)
 {
   const int i = blockIdx.x;
   if (i >= n_pose)
     return;
+  const gd_pack &P = packs[cx_index];
   const int lane = threadIdx.x;
   const int lna = P.lna, pnp = P.pnp;
   double m[6];
   for (int k = 0; k < 6; ++k)
-    m[k] = (double) move[i * 6 + k];
+    m[k] = (double) move[(size_t) i * move_stride + k];
   const double s1 = sin(m[3]), c1 = cos(m[3]);
   const double s2 = sin(m[4]), c2 = cos(m[4]);
   const double s3 = sin(m[5]), c3 = cos(m[5]);
@@ -620,75 +706,100 @@ gd_rescore64_kernel(gd_pack P, const int *__restrict__ lig_conf,
 /* ------------------------------------------------------------------------ */
 /* host-callable launch wrappers                                             */
 /* ------------------------------------------------------------------------ */
-#include "gd_launch.h"
-
-hipError_t gd_launch_mc(const gd_pack &P, const gd_state &S, const gd_task *tasks,
-                        int n_tasks, const int *rep_lig, const gd_launch_cfg &C,
-                        int s_begin, int n_steps, uint32_t seed, uint32_t cid,
-                        int mode, hipStream_t st)
+template <int WPG>
+static hipError_t launch_mc_wpg(const gd_pack *packs, const gd_task *tasks, int n_tasks,
+                                int *queue, int grid, int mode, int stage_bytes,
+                                hipStream_t st)
 {
-  gd_lds_layout L = {C.off_eh, C.off_tab, C.off_kde, C.lds_bytes};
-  if (C.kde_in_lds)
-    hipLaunchKernelGGL(gd_mc_kernel<true>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
-                       C.lds_bytes, st, P, S, tasks, rep_lig, L, s_begin, n_steps,
-                       seed, cid, mode);
-  else
-    hipLaunchKernelGGL(gd_mc_kernel<false>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
-                       C.lds_bytes, st, P, S, tasks, rep_lig, L, s_begin, n_steps,
-                       seed, cid, mode);
+  const int lds = stage_bytes + WPG * GD_WSLOT_FLOATS * 4;
+  hipLaunchKernelGGL(gd_mc_kernel<WPG>, dim3(grid), dim3(GD_WAVE * WPG), lds, st, packs,
+                     tasks, n_tasks, queue, mode, stage_bytes);
   return hipGetLastError();
 }
 
-hipError_t gd_launch_exchange(const gd_pack &P, const gd_state &S, int x,
-                              uint32_t seed, uint32_t cid, hipStream_t st)
+hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
+                        int n_tasks, int *queue, int grid, int mode, hipStream_t st)
 {
-  const int n_chain = P.n_prt * P.n_lig;
-  const int blocks = (n_chain + 255) / 256;
-  hipLaunchKernelGGL(gd_exchange_kernel, dim3(blocks), dim3(256), 0, st, P, S, x,
-                     seed, cid);
+  if (n_tasks <= 0 || grid <= 0)
+    return hipSuccess;
+  switch (C.wpg) {
+  case 4: return launch_mc_wpg<4>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  case 16: return launch_mc_wpg<16>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  default: return launch_mc_wpg<8>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  }
+}
+
+hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int n_chain,
+                              hipStream_t st)
+{
+  if (n_chain <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(gd_exchange_kernel, dim3((n_chain + 255) / 256), dim3(256), 0, st,
+                     packs, chains, n_chain);
   return hipGetLastError();
 }
 
-hipError_t gd_launch_score(const gd_pack &P, const gd_task *tasks, int n_tasks,
-                           const int *pose_ids, const int *lig_conf,
-                           const float *move, const gd_launch_cfg &C,
-                           float *energy, hipStream_t st)
+hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, hipStream_t st)
 {
-  gd_lds_layout L = {C.off_eh, C.off_tab, C.off_kde, C.lds_bytes};
-  if (C.kde_in_lds)
-    hipLaunchKernelGGL(gd_score_kernel<true>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
-                       C.lds_bytes, st, P, tasks, pose_ids, lig_conf, move, L, energy);
-  else
-    hipLaunchKernelGGL(gd_score_kernel<false>, dim3(n_tasks), dim3(GD_WAVE * GD_WPG),
-                       C.lds_bytes, st, P, tasks, pose_ids, lig_conf, move, L, energy);
+  const int n = n_cx > 1 ? n_cx : 1;
+  hipLaunchKernelGGL(gd_advance_kernel, dim3((n + 255) / 256), dim3(256), 0, st, packs, n_cx,
+                     queue);
   return hipGetLastError();
 }
 
-hipError_t gd_launch_rescore64(const gd_pack &P, const int *lig_conf,
-                               const int *prt_conf, const float *move, int n_pose,
-                               double *energy, hipStream_t st)
+hipError_t gd_launch_score(const gd_pack *packs, const gd_task *tasks, int n_tasks,
+                           const int *pose_ids, const int *lig_conf, const float *move,
+                           int stage_bytes, float *energy, hipStream_t st)
+{
+  if (n_tasks <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(gd_score_kernel, dim3(n_tasks), dim3(GD_WAVE * 8), stage_bytes, st,
+                     packs, tasks, pose_ids, lig_conf, move, energy);
+  return hipGetLastError();
+}
+
+hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *lig_conf,
+                               const int *prt_conf, const float *move, int move_stride,
+                               int n_pose, double *energy, hipStream_t st)
 {
   if (n_pose <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL(gd_rescore64_kernel, dim3(n_pose), dim3(GD_WAVE), 0, st, P,
-                     lig_conf, prt_conf, move, n_pose, energy);
+  hipLaunchKernelGGL(gd_rescore64_kernel, dim3(n_pose), dim3(GD_WAVE), 0, st, packs,
+                     cx_index, lig_conf, prt_conf, move, move_stride, n_pose, energy);
   return hipGetLastError();
 }
 
-hipError_t gd_set_kernel_attrs(int lds_bytes)
+template <int WPG>
+static hipError_t set_attr_wpg(int bytes)
 {
-  hipError_t e;
-  e = hipFuncSetAttribute((const void *) gd_mc_kernel<true>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *) gd_mc_kernel<false>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *) gd_score_kernel<true>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *) gd_score_kernel<false>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  return hipFuncSetAttribute((const void *) gd_mc_kernel<WPG>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             bytes + WPG * GD_WSLOT_FLOATS * 4);
 }
 
-int gd_waves_per_group(void) { return GD_WPG; }
+hipError_t gd_set_kernel_attrs(int stage_bytes)
+{
+  hipError_t e = set_attr_wpg<4>(stage_bytes);
+  if (e == hipSuccess) e = set_attr_wpg<8>(stage_bytes);
+  if (e == hipSuccess) e = set_attr_wpg<16>(stage_bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *) gd_score_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, stage_bytes);
+  return e;
+}
+
+hipError_t gd_mc_occupancy(int wpg, int stage_bytes, int *blocks_per_cu)
+{
+  const int lds = stage_bytes + wpg * GD_WSLOT_FLOATS * 4;
+  switch (wpg) {
+  case 4:
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, gd_mc_kernel<4>,
+                                                        GD_WAVE * 4, lds);
+  case 16:
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, gd_mc_kernel<16>,
+                                                        GD_WAVE * 16, lds);
+  default:
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, gd_mc_kernel<8>,
+                                                        GD_WAVE * 8, lds);
+  }
+}

@@ -7,29 +7,26 @@
 #include <hip/hip_runtime.h>
 #include "gd_device.h"
 
-/* LDS carve-up chosen by the host for one pack. */
+/* MC launch shape. */
 typedef struct gd_launch_cfg {
-  int off_eh;      /* bytes: pocket float2 array */
-  int off_tab;     /* bytes: pair-parameter rows */
-  int off_kde;     /* bytes: KDE points (when kde_in_lds) */
-  int lds_bytes;   /* total dynamic LDS per workgroup */
-  int kde_in_lds;
+  int wpg;          /* wavefronts (replicas) per workgroup: 4, 8 or 16 */
+  int stage_bytes;  /* dynamic LDS for staging: max over resident complexes */
+  int persistent;   /* 1: grid of resident workgroups pulling from a queue */
 } gd_launch_cfg;
 
-hipError_t gd_launch_mc(const gd_pack &P, const gd_state &S, const gd_task *tasks,
-                        int n_tasks, const int *rep_lig, const gd_launch_cfg &C,
-                        int s_begin, int n_steps, uint32_t seed, uint32_t cid,
-                        int mode, hipStream_t st);
-hipError_t gd_launch_exchange(const gd_pack &P, const gd_state &S, int x,
-                              uint32_t seed, uint32_t cid, hipStream_t st);
-hipError_t gd_launch_score(const gd_pack &P, const gd_task *tasks, int n_tasks,
-                           const int *pose_ids, const int *lig_conf,
-                           const float *move, const gd_launch_cfg &C,
-                           float *energy, hipStream_t st);
-hipError_t gd_launch_rescore64(const gd_pack &P, const int *lig_conf,
-                               const int *prt_conf, const float *move, int n_pose,
-                               double *energy, hipStream_t st);
-hipError_t gd_set_kernel_attrs(int lds_bytes);
-int gd_waves_per_group(void);
+hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
+                        int n_tasks, int *queue, int grid, int mode, hipStream_t st);
+hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int n_chain,
+                              hipStream_t st);
+hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, hipStream_t st);
+hipError_t gd_launch_score(const gd_pack *packs, const gd_task *tasks, int n_tasks,
+                           const int *pose_ids, const int *lig_conf, const float *move,
+                           int stage_bytes, float *energy, hipStream_t st);
+/* pose i: movematrix at move[i * move_stride .. +6] */
+hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *lig_conf,
+                               const int *prt_conf, const float *move, int move_stride,
+                               int n_pose, double *energy, hipStream_t st);
+hipError_t gd_set_kernel_attrs(int stage_bytes);
+hipError_t gd_mc_occupancy(int wpg, int stage_bytes, int *blocks_per_cu);
 
 #endif /* GD_LAUNCH_H */

@@ -111,6 +111,74 @@ class ReplicaResult(C.Structure):
                 ("idx_tmp_final", C.c_int), ("acs_mc", C.c_int), ("acs_exchg", C.c_int)]
 
 
+class Job(C.Structure):
+    _fields_ = [("lig", C.POINTER(Ligand)), ("prt", C.POINTER(Protein)),
+                ("psp", C.POINTER(Psp)), ("kde", C.POINTER(Kde)), ("mcs", C.POINTER(Mcs)),
+                ("enepara", C.POINTER(EnePara)), ("temp", C.POINTER(Temp)),
+                ("replica", C.POINTER(Replica)), ("mcpara", C.POINTER(McPara)),
+                ("size", ComplexSize), ("opt", C.POINTER(Options))]
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("arena_bytes", C.c_size_t), ("wpg", C.c_int),
+                ("persistent", C.c_int), ("use_graph", C.c_int), ("kde_policy", C.c_int),
+                ("staging_bytes", C.c_size_t)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n", C.c_int), ("wpg", C.c_int * 8), ("persistent", C.c_int * 8),
+                ("ms", C.c_double * 8), ("chosen_wpg", C.c_int),
+                ("chosen_persistent", C.c_int)]
+
+
+class EngineLayout(C.Structure):
+    _fields_ = [("stage_bytes", C.c_int), ("wpg", C.c_int), ("persistent", C.c_int),
+                ("n_tasks", C.c_int), ("n_chains", C.c_int), ("n_complexes", C.c_int),
+                ("grid", C.c_int), ("resident_groups", C.c_int), ("kde_in_lds", C.c_int),
+                ("arena_used", C.c_size_t), ("arena_size", C.c_size_t)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("wpg", C.c_int), ("persistent", C.c_int),
+                ("no_graph", C.c_int), ("autotune", C.c_int), ("kde_policy", C.c_int)]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("tune_s", C.c_double), ("run_s", C.c_double),
+                ("results_s", C.c_double), ("cycles", C.c_int), ("tune", TuneReport),
+                ("layout", EngineLayout)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("arena_bytes", C.c_size_t), ("wpg", C.c_int),
+                ("persistent", C.c_int), ("use_graph", C.c_int), ("kde_policy", C.c_int),
+                ("max_resident_replicas", C.c_int), ("cycles_per_tick", C.c_int),
+                ("max_ticks_in_flight", C.c_int), ("keep_results", C.c_int),
+                ("readback_bytes", C.c_size_t)]
+
+
+class ServerJobInfo(C.Structure):
+    _fields_ = [("ticket", C.c_longlong), ("state", C.c_int), ("status", C.c_int),
+                ("n_rep", C.c_int), ("steps_total", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("latency", C.c_double),
+                ("footprint", C.c_size_t)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong),
+                ("failed", C.c_longlong), ("cancelled", C.c_longlong), ("queued", C.c_int),
+                ("running", C.c_int), ("p50_latency", C.c_double),
+                ("p99_latency", C.c_double), ("mean_latency", C.c_double),
+                ("max_latency", C.c_double), ("p50_wait", C.c_double),
+                ("p99_wait", C.c_double), ("pose_evals", C.c_double),
+                ("first_submit", C.c_double), ("last_done", C.c_double),
+                ("pose_evals_per_s", C.c_double), ("arena_size", C.c_size_t),
+                ("arena_peak", C.c_size_t), ("running_peak", C.c_int),
+                ("cycles", C.c_longlong), ("rebuilds", C.c_longlong)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
+
 _LIB = None
 
 SYMBOLS = {
@@ -140,6 +208,34 @@ SYMBOLS = {
                                     C.POINTER(C.c_int)]),
     "gd_session_layout": (C.c_int, [C.c_void_p, IP, IP, IP, IP]),
     "gd_session_destroy": (None, [C.c_void_p]),
+    "gd_engine_create": (C.c_int, [C.POINTER(EngineConfig), C.POINTER(C.c_void_p)]),
+    "gd_engine_destroy": (None, [C.c_void_p]),
+    "gd_job_footprint": (C.c_int, [C.POINTER(Job), C.c_int, C.POINTER(C.c_size_t)]),
+    "gd_engine_add": (C.c_int, [C.c_void_p, C.POINTER(Job), IP]),
+    "gd_engine_remove": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_engine_cycles": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_engine_sync": (C.c_int, [C.c_void_p]),
+    "gd_engine_cycles_left": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_engine_results": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(McLog),
+                                    C.POINTER(ReplicaResult)]),
+    "gd_engine_autotune": (C.c_int, [C.c_void_p, C.POINTER(TuneReport)]),
+    "gd_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_engine_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "gd_engine_layout": (C.c_int, [C.c_void_p, C.POINTER(EngineLayout)]),
+    "gd_batch_run": (C.c_int, [C.c_int, C.POINTER(Job), C.POINTER(BatchConfig),
+                               C.POINTER(C.POINTER(ReplicaResult)), C.POINTER(McLog),
+                               C.POINTER(BatchReport)]),
+    "gd_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "gd_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_longlong)]),
+    "gd_server_wait": (C.c_int, [C.c_void_p, C.c_longlong, C.c_double,
+                                 C.POINTER(ServerJobInfo)]),
+    "gd_server_fetch": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(McLog),
+                                  C.POINTER(ReplicaResult), C.POINTER(ServerJobInfo)]),
+    "gd_server_cancel": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "gd_server_drain": (C.c_int, [C.c_void_p]),
+    "gd_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "gd_server_destroy": (None, [C.c_void_p]),
     "gd_optimize_enepara": (C.c_int, [C.POINTER(EnePara0), C.POINTER(EnePara)]),
     "gd_set_temperature": (C.c_int, [C.POINTER(Temp), C.c_int, C.c_float, C.c_float]),
     "gd_set_replica": (C.c_int, [C.POINTER(Replica), ComplexSize]),
@@ -402,6 +498,182 @@ class Session:
     def close(self):
         if self.h:
             lib().gd_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _struct_dict(st):
+    out = {}
+    for name, typ in st._fields_:
+        v = getattr(st, name)
+        if isinstance(v, C.Structure):
+            v = _struct_dict(v)
+        elif isinstance(v, C.Array):
+            v = list(v)
+        out[name] = v
+    return out
+
+
+class JobSpec:
+    """One gd_job: a complex, its temperature ladder, McPara and options.
+
+    Holds every ctypes object the job's pointers refer to."""
+
+    def __init__(self, cx, temps, mc, opt=None):
+        self.cx, self.temps, self.mc = cx, temps, mc
+        self.opt = opt or default_options()
+        self.size = cx.sized(len(temps))
+        self.c = Job(C.pointer(cx.c_lig), C.pointer(cx.c_prt), C.pointer(cx.c_psp),
+                     C.pointer(cx.c_kde), C.pointer(cx.c_mcs), C.pointer(cx.enepara),
+                     C.cast(temps, C.POINTER(Temp)), None, C.pointer(mc), self.size,
+                     C.pointer(self.opt))
+
+    @property
+    def n_rep(self):
+        return self.size.n_rep
+
+    def footprint(self, device=0):
+        b = C.c_size_t()
+        check(lib().gd_job_footprint(C.byref(self.c), device, C.byref(b)))
+        return b.value
+
+
+class Engine:
+    """gd_engine_*: many complexes resident on one GPU, advanced together."""
+
+    def __init__(self, device=0, arena_bytes=1 << 30, wpg=8, persistent=0, use_graph=1,
+                 kde_policy=0):
+        cfg = EngineConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy, 0)
+        h = C.c_void_p()
+        check(lib().gd_engine_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.jobs = {}
+
+    def add(self, job):
+        hd = C.c_int()
+        check(lib().gd_engine_add(self.h, C.byref(job.c), C.byref(hd)))
+        self.jobs[hd.value] = job
+        return hd.value
+
+    def remove(self, handle):
+        check(lib().gd_engine_remove(self.h, handle))
+        self.jobs.pop(handle, None)
+
+    def cycles(self, n):
+        check(lib().gd_engine_cycles(self.h, n))
+
+    def sync(self):
+        check(lib().gd_engine_sync(self.h))
+
+    def cycles_left(self, handle):
+        rc = lib().gd_engine_cycles_left(self.h, handle)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def results(self, handle):
+        job = self.jobs[handle]
+        res = (ReplicaResult * job.n_rep)()
+        log = McLog()
+        check(lib().gd_engine_results(self.h, handle, C.byref(log), res))
+        return results_to_numpy(res), log
+
+    def autotune(self):
+        rep = TuneReport()
+        check(lib().gd_engine_autotune(self.h, C.byref(rep)))
+        return _struct_dict(rep)
+
+    def set_timing(self, on):
+        check(lib().gd_engine_set_timing(self.h, int(on)))
+
+    def timing(self):
+        a, b = C.c_double(), C.c_int()
+        c, d = C.c_double(), C.c_int()
+        check(lib().gd_engine_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return a.value, b.value, c.value, d.value
+
+    def layout(self):
+        info = EngineLayout()
+        check(lib().gd_engine_layout(self.h, C.byref(info)))
+        return _struct_dict(info)
+
+    def close(self):
+        if self.h:
+            lib().gd_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_run(jobs, device=0, wpg=0, persistent=0, no_graph=0, autotune=0, kde_policy=0):
+    """gd_batch_run over JobSpecs; returns ([results dict per job], report dict)."""
+    n = len(jobs)
+    arr = (Job * n)(*[j.c for j in jobs])
+    res = [(ReplicaResult * j.n_rep)() for j in jobs]
+    ptrs = (C.POINTER(ReplicaResult) * n)(*[C.cast(r, C.POINTER(ReplicaResult)) for r in res])
+    logs = (McLog * n)()
+    rep = BatchReport()
+    cfg = BatchConfig(device, wpg, persistent, no_graph, autotune, kde_policy)
+    check(lib().gd_batch_run(n, arr, C.byref(cfg), ptrs, logs, C.byref(rep)))
+    return [results_to_numpy(r) for r in res], _struct_dict(rep)
+
+
+class Server:
+    """gd_server_*: request loop with continuous batching on one GPU."""
+
+    def __init__(self, device=0, arena_bytes=0, wpg=8, persistent=0, use_graph=0,
+                 kde_policy=0, max_resident_replicas=0, cycles_per_tick=1,
+                 max_ticks_in_flight=2, keep_results=1, readback_bytes=0):
+        cfg = ServerConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy,
+                           max_resident_replicas, cycles_per_tick, max_ticks_in_flight,
+                           keep_results, readback_bytes)
+        h = C.c_void_p()
+        check(lib().gd_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.n_rep = {}
+
+    def submit(self, job):
+        t = C.c_longlong()
+        check(lib().gd_server_submit(self.h, C.byref(job.c), C.byref(t)))
+        self.n_rep[t.value] = job.n_rep
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        info = ServerJobInfo()
+        check(lib().gd_server_wait(self.h, ticket, timeout, C.byref(info)))
+        return _struct_dict(info)
+
+    def fetch(self, ticket):
+        n = self.n_rep.pop(ticket)
+        res = (ReplicaResult * n)()
+        log = McLog()
+        info = ServerJobInfo()
+        check(lib().gd_server_fetch(self.h, ticket, C.byref(log), res, C.byref(info)))
+        return results_to_numpy(res), log, _struct_dict(info)
+
+    def cancel(self, ticket):
+        check(lib().gd_server_cancel(self.h, ticket))
+
+    def drain(self):
+        check(lib().gd_server_drain(self.h))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().gd_server_get_stats(self.h, C.byref(st)))
+        return _struct_dict(st)
+
+    def close(self):
+        if self.h:
+            lib().gd_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -28,6 +28,7 @@ import math
 import os
 
 import numpy as np
+from scipy.spatial import cKDTree
 
 from . import remc
 
@@ -111,7 +112,8 @@ def _in_sphere(rng, n, radius):
 
 def make_complex(pnp, radius, lna, n_lig, n_prt, pnk, seed=1, pos=11, enepara=None,
                  center=(0.0, 0.0, 0.0), **_unused):
-    """Build one synthetic prepared complex (remc.Complex)."""
+    """Build one synthetic prepared complex (remc.Complex).  radius None:
+    pocket_radius(pnp), or more when the ligand ensemble needs it."""
     rng = np.random.default_rng(seed)
     if enepara is None:
         ep0, ep = enepara_from_params()
@@ -129,11 +131,22 @@ def make_complex(pnp, radius, lna, n_lig, n_prt, pnk, seed=1, pos=11, enepara=No
 
     # pocket: uniform in the sphere, >= 3 A from every start-pose ligand atom
     allatoms = confs.reshape(-1, 3)
+    if radius is None:
+        # pocket_radius(pnp), widened so that the ligand ensemble (plus the
+        # 3 A exclusion) leaves room for every pocket point
+        r_lig = float(np.max(np.linalg.norm(allatoms, axis=1)))
+        radius = max(pocket_radius(pnp), r_lig + 5.0)
+    tree = cKDTree(allatoms)
     pts = np.zeros((0, 3))
-    while len(pts) < pnp:
+    for _ in range(200):
+        if len(pts) >= pnp:
+            break
         cand = _in_sphere(rng, 4 * pnp, radius)
-        d = np.min(np.linalg.norm(cand[:, None, :] - allatoms[None, :, :], axis=2), axis=1)
+        d, _ = tree.query(cand, k=1)
         pts = np.concatenate([pts, cand[d >= 3.0]])
+    else:
+        raise RuntimeError(f"pocket of {pnp} points does not fit a {radius:.1f} A sphere "
+                           "around the ligand")
     pts = pts[:pnp]
     prt_t = rng.integers(0, 24, size=pnp)
     d_code = rng.integers(0, 20, size=pnp)
@@ -192,3 +205,57 @@ def subset(cx, n_lig, n_prt):
         prt_hpcflag=cx.prt_hpcflag[:n_prt], prt_hpp=cx.prt_hpp[:n_prt],
         pocket_center=cx.pocket_center, psp=cx.psp, kde_xyz=cx.kde_xyz, kde_t=cx.kde_t,
         mcs_xyz=cx.mcs_xyz, mcs_tcc=cx.mcs_tcc, enepara=cx.enepara)
+
+
+def pocket_radius(pnp, density=0.07):
+    """Sphere radius holding pnp points at the density of the single-complex
+    configurations (300 points in 10 A, 800 in 14 A: about 0.07 per A^3)."""
+    return (3.0 * pnp / (4.0 * math.pi * density)) ** (1.0 / 3.0)
+
+
+def make_batch_jobs(n=256, seed=1, steps=2000, spe=10):
+    """Batched uneven workload: n complexes with ligand atoms U[12, 96],
+    pocket points U[200, 1500], ligand conformers U[4, 64], 8 or 16
+    temperatures (0.3-3.0), one protein conformer, steps MC steps, exchange
+    every spe steps.  Left open by the definition and fixed here: KDE points
+    = pocket points; pocket radius from pocket_radius(), widened around
+    large ligands (make_complex with radius None); move half-widths
+    0.2 A / 3 deg as in the single-complex runs; Philox key (seed, index)."""
+    rng = np.random.default_rng(seed)
+    ep = enepara_from_params()
+    scale = [0.2] * 3 + [math.radians(3.0)] * 3
+    jobs = []
+    for i in range(n):
+        lna = int(rng.integers(12, 97))
+        pnp = int(rng.integers(200, 1501))
+        n_lig = int(rng.integers(4, 65))
+        n_tmp = int(rng.choice([8, 16]))
+        cx = make_complex(pnp=pnp, radius=None, lna=lna, n_lig=n_lig, n_prt=1,
+                          pnk=pnp, seed=seed * 100003 + i, enepara=ep)
+        temps = remc.temperatures(n_tmp, 0.3, 3.0)
+        mc = remc.mcpara(steps, spe, scale)
+        opt = remc.default_options(seed=seed, complex_id=i)
+        jobs.append(remc.JobSpec(cx, temps, mc, opt))
+    return jobs
+
+
+def make_serving_pool(n=256, seed=2, steps=1000, spe=10, sigma=0.5):
+    """Complexes of the serving workload: ligand atoms lognormal (median 32,
+    clipped to [10, 120]), pocket points lognormal (median 600, clipped to
+    [150, 2000]), 16 ligand conformers x 8 temperatures, one protein
+    conformer, steps MC steps.  Left open and fixed here: lognormal sigma
+    0.5, KDE points = pocket points, pocket radius as in make_batch_jobs.
+    Returns [(complex, temps, mcpara)]; each served job draws one and gets
+    its own Philox key."""
+    rng = np.random.default_rng(seed)
+    ep = enepara_from_params()
+    scale = [0.2] * 3 + [math.radians(3.0)] * 3
+    temps = remc.temperatures(8, 0.3, 3.0)
+    pool = []
+    for i in range(n):
+        lna = int(np.clip(round(rng.lognormal(math.log(32.0), sigma)), 10, 120))
+        pnp = int(np.clip(round(rng.lognormal(math.log(600.0), sigma)), 150, 2000))
+        cx = make_complex(pnp=pnp, radius=None, lna=lna, n_lig=16, n_prt=1,
+                          pnk=pnp, seed=seed * 100003 + i, enepara=ep)
+        pool.append((cx, temps, remc.mcpara(steps, spe, scale)))
+    return pool

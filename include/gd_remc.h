@@ -26,6 +26,7 @@
 #ifndef GD_REMC_H
 #define GD_REMC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -48,7 +49,8 @@ enum gd_status {
   GD_ERR_NOMEM = -3,     /* device allocation failed / admission refused */
   GD_ERR_NODEV = -4,     /* no usable GPU */
   GD_ERR_IO = -5,        /* file could not be read or parsed */
-  GD_ERR_CANCELLED = -6  /* job cancelled (serving) */
+  GD_ERR_CANCELLED = -6, /* job cancelled (serving) */
+  GD_ERR_TIMEOUT = -7    /* wait timed out; the job is still queued or running */
 };
 
 enum gd_energy_slot {
@@ -265,6 +267,186 @@ int gd_session_timing(gd_session *s, double *mc_ms, int *mc_launches,
 int gd_session_layout(const gd_session *s, int *lds_bytes, int *kde_in_lds,
                       int *n_tasks, int *waves_per_group);
 void gd_session_destroy(gd_session *s);
+
+/* ---------------------------------------------------------------------- */
+/* Engine: many complexes resident on one GPU, advanced together.  One      */
+/* cycle = one MC launch over every replica of every complex (each complex  */
+/* takes min(steps_per_exchange, steps left) steps), one exchange launch    */
+/* over every temperature chain, one launch that advances the per-complex   */
+/* step counters held in device memory; a complex whose steps are done is   */
+/* left untouched.  A complex's results are identical whether it runs alone */
+/* or with others, and identical to gd_run on the same inputs.  No          */
+/* counterpart in the reference, which runs one complex per process         */
+/* (main.C:11-119 -> Run, run.cu:40-352).                                   */
+/* ---------------------------------------------------------------------- */
+typedef struct gd_engine gd_engine;
+
+/* A job = the arguments of one gd_run call (pointers are read at admission
+ * only).  opt NULL = the gd_run defaults. */
+typedef struct gd_job {
+  const gd_ligand *lig;
+  const gd_protein *prt;
+  const gd_psp *psp;
+  const gd_kde *kde;
+  const gd_mcs *mcs;
+  const gd_enepara *enepara;
+  const gd_temp *temp;
+  const gd_replica *replica;   /* NULL = the SetReplica layout */
+  const gd_mcpara *mcpara;
+  gd_complex_size size;
+  const gd_options *opt;
+} gd_job;
+
+enum gd_kde_policy {
+  GD_KDE_STAGE_AUTO = 0,   /* KDE points in LDS when they fit */
+  GD_KDE_STAGE_NEVER = 1   /* KDE points read from HBM (L2) */
+};
+
+typedef struct gd_engine_config {
+  int device;
+  size_t arena_bytes;  /* device memory reserved up front; 0 = 1 GiB */
+  int wpg;             /* waves (replicas) per workgroup: 4, 8 (default), 16 */
+  int persistent;      /* 1: resident workgroups pull tasks from a queue */
+  int use_graph;       /* 1: cycles replay one captured hipGraph */
+  int kde_policy;      /* enum gd_kde_policy */
+  size_t staging_bytes;/* pinned host memory for uploads; 0 = 64 MiB */
+} gd_engine_config;
+
+typedef struct gd_tune_report {
+  int n;               /* launch shapes tried */
+  int wpg[8];
+  int persistent[8];
+  double ms[8];        /* one MC launch */
+  int chosen_wpg, chosen_persistent;
+} gd_tune_report;
+
+typedef struct gd_engine_layout_info {
+  int stage_bytes;     /* LDS staged per workgroup */
+  int wpg, persistent;
+  int n_tasks, n_chains, n_complexes;
+  int grid;            /* workgroups per MC launch */
+  int resident_groups; /* workgroups the device holds at once */
+  int kde_in_lds;      /* 1 if every complex stages its KDE points */
+  size_t arena_used, arena_size;
+} gd_engine_layout_info;
+
+int gd_engine_create(const gd_engine_config *cfg, gd_engine **out);
+void gd_engine_destroy(gd_engine *e);
+/* device bytes a job occupies once admitted (HBM admission control) */
+int gd_job_footprint(const gd_job *job, int device, size_t *bytes);
+/* admit: upload + initial energies; GD_ERR_NOMEM if the arena is full */
+int gd_engine_add(gd_engine *e, const gd_job *job, int *handle);
+/* retire (waits for queued cycles first) */
+int gd_engine_remove(gd_engine *e, int handle);
+/* queue n cycles; returns before the GPU finishes */
+int gd_engine_cycles(gd_engine *e, int n);
+int gd_engine_sync(gd_engine *e);
+/* cycles still needed by the job after the queued ones */
+int gd_engine_cycles_left(gd_engine *e, int handle);
+/* results of one job, as gd_run (waits for queued cycles) */
+int gd_engine_results(gd_engine *e, int handle, gd_mclog *mclog,
+                      gd_replica_result *results);
+/* time each launch shape on the resident jobs, keep the fastest */
+int gd_engine_autotune(gd_engine *e, gd_tune_report *rep);
+int gd_engine_set_timing(gd_engine *e, int on);
+int gd_engine_timing(gd_engine *e, double *mc_ms, int *mc_launches,
+                     double *x_ms, int *x_launches);
+int gd_engine_layout(gd_engine *e, gd_engine_layout_info *info);
+
+/* Batch: admit every job, autotune (optional), run until every job has
+ * done its steps, collect.  results[i]: [jobs[i].size.n_rep] or NULL;
+ * results and mclogs may be NULL. */
+typedef struct gd_batch_config {
+  int device;
+  int wpg;             /* 0 = 8 */
+  int persistent;
+  int no_graph;        /* 1: launch the cycle kernels one by one */
+  int autotune;
+  int kde_policy;
+} gd_batch_config;
+
+typedef struct gd_batch_report {
+  double admit_s, tune_s, run_s, results_s;
+  int cycles;
+  gd_tune_report tune;
+  gd_engine_layout_info layout;
+} gd_batch_report;
+
+int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_config *cfg,
+                 gd_replica_result **results, gd_mclog *mclogs,
+                 gd_batch_report *report);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: a request loop over one engine.  Jobs are submitted from any    */
+/* thread at any time (the inputs are copied at submission), wait in FIFO  */
+/* order until their HBM footprint fits, then run in the same cycles as     */
+/* every other resident job (continuous batching); a finished job is read   */
+/* back asynchronously and retired, freeing its memory for the next.  Each  */
+/* job's results are identical to gd_run of it alone.  No counterpart in    */
+/* the reference (one complex per process, main.C:11-119).                  */
+/* ---------------------------------------------------------------------- */
+typedef struct gd_server gd_server;
+
+typedef struct gd_server_config {
+  int device;
+  size_t arena_bytes;          /* HBM for resident jobs; 0 = 4 GiB */
+  int wpg;                     /* waves per workgroup: 4, 8 (default), 16 */
+  int persistent;              /* 1: persistent MC launches with a work queue */
+  int use_graph;               /* 1: replay captured cycles between changes */
+  int kde_policy;              /* enum gd_kde_policy */
+  int max_resident_replicas;   /* admission cap besides memory; 0 = none */
+  int cycles_per_tick;         /* cycles queued per admission round; 0 = 1 */
+  int max_ticks_in_flight;     /* ticks queued ahead of the GPU; 0 = 2 */
+  int keep_results;            /* 1: results kept until gd_server_fetch;
+                                  0: only latency statistics are kept */
+  size_t readback_bytes;       /* pinned host memory for read-back; 0 = 64 MiB */
+} gd_server_config;
+
+enum gd_job_state {
+  GD_JOB_QUEUED = 0, GD_JOB_RUNNING = 1, GD_JOB_DONE = 2, GD_JOB_FAILED = 3,
+  GD_JOB_CANCELLED = 4
+};
+
+typedef struct gd_server_job_info {
+  long long ticket;
+  int state;                   /* enum gd_job_state */
+  int status;                  /* gd_status of a failed job */
+  int n_rep, steps_total;
+  double t_submit, t_admit, t_done;   /* seconds since gd_server_create */
+  double latency;              /* t_done - t_submit */
+  size_t footprint;            /* device bytes while resident */
+} gd_server_job_info;
+
+typedef struct gd_server_stats {
+  long long submitted, completed, failed, cancelled;
+  int queued, running;
+  double p50_latency, p99_latency, mean_latency, max_latency;  /* s */
+  double p50_wait, p99_wait;   /* submission to admission, s */
+  double pose_evals;           /* MC pose evaluations of completed jobs */
+  double first_submit, last_done;
+  double pose_evals_per_s;     /* pose_evals / (last_done - first_submit) */
+  size_t arena_size, arena_peak;
+  int running_peak;
+  long long cycles, rebuilds;
+} gd_server_stats;
+
+int gd_server_create(const gd_server_config *cfg, gd_server **out);
+/* copies the job's inputs; GD_ERR_NOMEM if it can never fit the arena */
+int gd_server_submit(gd_server *s, const gd_job *job, long long *ticket);
+/* until the job is done, failed or cancelled (GD_OK) or timeout_s passes
+ * (GD_ERR_TIMEOUT); timeout_s < 0 waits without limit */
+int gd_server_wait(gd_server *s, long long ticket, double timeout_s,
+                   gd_server_job_info *info);
+/* results of a finished job (keep_results = 1), then forgets the job;
+ * results: [n_rep] or NULL */
+int gd_server_fetch(gd_server *s, long long ticket, gd_mclog *mclog,
+                    gd_replica_result *results, gd_server_job_info *info);
+int gd_server_cancel(gd_server *s, long long ticket);
+/* until nothing is queued or running */
+int gd_server_drain(gd_server *s);
+int gd_server_get_stats(gd_server *s, gd_server_stats *st);
+/* cancels what is queued or running and stops the loop */
+void gd_server_destroy(gd_server *s);
 
 /* ---------------------------------------------------------------------- */
 /* Host preparation (util.C:199-558)                                        */

@@ -1,0 +1,200 @@
+"""GPU tests of the engine: many complexes resident and advanced together.
+
+Every job's trajectory must equal the CPU replay of that job alone (and so
+gd_run of it alone), whatever else shares the launches, whatever the launch
+shape (waves per workgroup, grid or persistent queue, graph or not), and
+whenever it was admitted.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpudock_amd import remc, synth
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("move_final", "energy_final", "move_best", "energy_best", "step_best",
+        "idx_tmp_final", "acs_mc", "acs_exchg")
+
+
+def _same(a, b, what=""):
+    for k in KEYS:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what} {k}")
+
+
+def _uneven_jobs(n, seed=0, steps=(30, 90)):
+    """Jobs of uneven sizes, schedules and options (small enough for the replay)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        lna = int(rng.integers(5, 100))
+        cx, temps, mc, cfg = synth.make_config(
+            "c0", seed=100 + i, lna=lna, radius=max(10.0, 3.0 * lna ** (1 / 3) + 4),
+            pnp=int(rng.integers(40, 420)), pnk=int(rng.integers(0, 300)),
+            n_lig=int(rng.integers(1, 4)), n_prt=int(rng.integers(1, 3)),
+            n_tmp=int(rng.choice([1, 2, 3, 8])), steps=int(rng.integers(*steps)),
+            spe=int(rng.choice([1, 5, 10])))
+        opt = remc.default_options(seed=7 + i, complex_id=i, exchange=int(i % 4 != 3),
+                                   away=int(i % 5 == 4), term_order=int(i % 3 == 2))
+        jobs.append(remc.JobSpec(cx, temps, mc, opt))
+    return jobs
+
+
+@pytest.fixture(scope="module")
+def jobs():
+    js = _uneven_jobs(6)
+    replays = [O.replay_run(j.cx, j.temps, j.mc, j.opt) for j in js]
+    return js, replays
+
+
+def test_batch_equals_replay_of_each_job(jobs):
+    """gd_batch_run (graph-replayed cycles, autotuned shape) on uneven jobs:
+    each job's results are those of its replay alone."""
+    js, replays = jobs
+    res, rep = remc.batch_run(js, autotune=1)
+    assert rep["tune"]["n"] == 6
+    assert rep["layout"]["n_complexes"] == len(js)
+    for i, (g, r) in enumerate(zip(res, replays)):
+        _same(g, r, f"job {i}")
+
+
+@pytest.mark.parametrize("wpg,persistent,no_graph", [
+    (4, 0, 0), (8, 1, 0), (16, 1, 1), (16, 0, 0), (4, 1, 1)])
+def test_launch_shapes_identical(jobs, wpg, persistent, no_graph):
+    js, replays = jobs
+    res, _ = remc.batch_run(js, wpg=wpg, persistent=persistent, no_graph=no_graph)
+    for i, (g, r) in enumerate(zip(res, replays)):
+        _same(g, r, f"wpg={wpg} persistent={persistent} job {i}")
+
+
+def test_alone_equals_mix(jobs):
+    """gd_run of one job alone == the same job inside the mix."""
+    js, replays = jobs
+    for i in (0, 3):
+        j = js[i]
+        alone, _ = remc.run(j.cx, j.temps, j.mc, j.opt)
+        _same(alone, replays[i], f"alone {i}")
+
+
+def test_staggered_admission_and_retirement(jobs):
+    """Jobs admitted between cycles start at their own step 0; retiring a
+    finished job leaves the others untouched."""
+    js, replays = jobs
+    eng = remc.Engine(arena_bytes=256 << 20, wpg=8, persistent=1, use_graph=1)
+    h = {}
+    h[0] = eng.add(js[0])
+    h[1] = eng.add(js[1])
+    eng.cycles(2)
+    h[2] = eng.add(js[2])
+    eng.cycles(3)
+    h[3] = eng.add(js[3])
+    left = {k: eng.cycles_left(v) for k, v in h.items()}
+    done = {}
+    while any(eng.cycles_left(v) > 0 for k, v in h.items() if k not in done):
+        eng.cycles(1)
+        for k, v in list(h.items()):
+            if k not in done and eng.cycles_left(v) == 0:
+                done[k], _ = eng.results(v)
+                eng.remove(v)
+    for k in h:
+        if k not in done:
+            done[k], _ = eng.results(h[k])
+    eng.close()
+    assert max(left.values()) > 0
+    for k, g in done.items():
+        _same(g, replays[k], f"staggered job {k}")
+
+
+def test_admission_refused_when_arena_full(jobs):
+    js, replays = jobs
+    need = js[0].footprint()
+    slack = 32 << 10    # control block + task lists
+    assert min(j.footprint() for j in js[1:]) > slack
+    eng = remc.Engine(arena_bytes=need + slack, use_graph=0)
+    h0 = eng.add(js[0])
+    with pytest.raises(remc.GDError) as ei:
+        eng.add(js[1])
+    assert "arena" in str(ei.value)
+    while eng.cycles_left(h0) > 0:
+        eng.cycles(eng.cycles_left(h0))
+    g, _ = eng.results(h0)
+    eng.close()
+    _same(g, replays[0], "after refused admission")
+
+
+def test_engine_timing_and_layout(jobs):
+    js, _ = jobs
+    eng = remc.Engine(arena_bytes=256 << 20, use_graph=1)
+    for j in js[:3]:
+        eng.add(j)
+    lay = eng.layout()
+    assert lay["n_complexes"] == 3 and lay["n_tasks"] > 0 and lay["resident_groups"] > 0
+    eng.set_timing(True)
+    eng.cycles(3)
+    eng.sync()
+    ms, n, xms, xn = eng.timing()
+    eng.close()
+    assert n == 3 and xn == 3 and ms > 0.0
+
+
+def test_server_results_equal_alone_under_continuous_batching(jobs):
+    """Jobs submitted while others run (arrival spread over several ticks)
+    are admitted as memory frees up; each result equals the job run alone."""
+    import time
+    js, replays = jobs
+    srv = remc.Server(arena_bytes=64 << 20, wpg=8, persistent=1)
+    tickets = []
+    for i, j in enumerate(js):
+        tickets.append(srv.submit(j))
+        if i % 2 == 1:
+            time.sleep(0.005)
+    # the same complexes again, with other seeds, to keep the mix busy
+    extra = []
+    for i, j in enumerate(js[:3]):
+        o = remc.default_options(seed=1000 + i, complex_id=50 + i)
+        extra.append(remc.JobSpec(j.cx, j.temps, j.mc, o))
+    t_extra = [srv.submit(e) for e in extra]
+    srv.drain()
+    for i, t in enumerate(tickets):
+        info = srv.wait(t, 10.0)
+        assert info["state"] == remc.JOB_DONE
+        g, log, info = srv.fetch(t)
+        _same(g, replays[i], f"served job {i}")
+        assert info["latency"] >= info["t_admit"] - info["t_submit"] >= 0.0
+    for e, t in zip(extra, t_extra):
+        g, _, _ = srv.fetch(t)
+        _same(g, O.replay_run(e.cx, e.temps, e.mc, e.opt), "served extra")
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(js) + len(extra)
+    assert st["failed"] == 0 and st["p99_latency"] >= st["p50_latency"] > 0.0
+    assert st["pose_evals"] == sum(j.n_rep * j.mc.steps_total for j in js + extra)
+
+
+def test_server_admission_by_footprint_and_cancel(jobs):
+    """An arena that holds one job at a time serialises admission (FIFO);
+    a cancelled queued job never runs; a job larger than the arena is
+    refused at submission."""
+    js, replays = jobs
+    need = max(j.footprint() for j in js[:3])
+    fp = sorted(j.footprint() for j in js[:3])
+    assert fp[0] + fp[1] > need + need // 2     # no two fit together
+    srv = remc.Server(arena_bytes=need + need // 2, wpg=4)
+    t = [srv.submit(j) for j in js[:3]]
+    t_cancel = srv.submit(js[0])
+    srv.cancel(t_cancel)
+    srv.drain()
+    infos = [srv.wait(x, 10.0) for x in t]
+    for i, x in enumerate(t):
+        g, _, _ = srv.fetch(x)
+        _same(g, replays[i], f"footprint-admitted job {i}")
+    # one resident at a time: each admission follows the previous completion
+    for a, b in zip(infos, infos[1:]):
+        assert b["t_admit"] >= a["t_done"] - 0.05
+    assert srv.wait(t_cancel, 10.0)["state"] == remc.JOB_CANCELLED
+    big = synth.make_config("c0", pnp=2000, n_lig=4, n_tmp=64, radius=16.0)
+    with pytest.raises(remc.GDError):
+        srv.submit(remc.JobSpec(big[0], big[1], big[2]))
+    st = srv.stats()
+    srv.close()
+    assert st["running_peak"] == 1 and st["cancelled"] == 1
