@@ -4,15 +4,17 @@ A "step" of this benchmark is one Monte Carlo cycle of the whole replica
 ensemble: steps_per_exchange (10) MC steps (Move + CalcEnergy + Accept) of
 every replica in one fused kernel launch, followed by one temperature-
 exchange launch -- the unit the reference's driver loop repeats
-(gpusingle.cu:14-20).  Default workload: configs[1] of the benchmark
-definition, one synthetic complex with pocket 800 points (14 A sphere),
-ligand 48 atoms, 64 ligand x 16 protein conformers x 16 temperatures =
-16,384 replicas, 2,000 KDE points, exchange every 10 steps; 500 cycles are
-the full 5,000-step run.
+(gpusingle.cu:14-20).  Default workload: one synthetic complex with pocket
+800 points (14 A sphere), ligand 48 atoms, 64 ligand x 16 protein
+conformers x 16 temperatures = 16,384 replicas, 2,000 KDE points, exchange
+every 10 steps; 500 cycles are the full 5,000-step run.
 
 Prints one JSON line: pose energy evals/s (= MC replica-steps/s), the
 roofline of the MC kernel (FP32 VALU), and the reference-order CPU oracle
 timed on this host on a bounded sample of the same complex.
+--workload batch (256 uneven complexes in one engine) and --workload serve
+(Poisson job arrivals into the serving loop, p50/p99 latency) print their
+own lines.
 """
 import argparse
 import json
@@ -55,7 +57,9 @@ def cpu_baseline(cx, temps_all, scale, cfg, seed):
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
     out = {}
-    for threads, n_lig, steps in ((1, 8, 40), (cores, 64, 40)):
+    # about 10 s of CPU work each: 1 thread ~2.6e3 and 16 threads ~3.6e4
+    # pose evals/s on this complex
+    for threads, n_lig, steps in ((1, 8, 800), (cores, 64, 1500 * cores // 16)):
         sub = synth.subset(cx, n_lig, 1)
         t_sub = remc.temperatures(4, cfg["t_lo"], cfg["t_hi"])
         n_rep = n_lig * 4
@@ -180,7 +184,8 @@ def bench_serve(args):
     specs = [job(i) for i in range(len(arrivals))]
     fp = max(s.footprint() for s in specs[:64])
     srv = remc.Server(arena_bytes=args.arena_gib << 30, wpg=args.wpg,
-                      persistent=args.persistent, keep_results=0)
+                      persistent=args.persistent, keep_results=0, occ=args.occ,
+                      kde_policy=args.kde_policy)
     print(f"serve: {len(specs)} jobs at {args.rate}/s over {args.duration} s", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
@@ -207,7 +212,8 @@ def bench_serve(args):
     if args.burst:
         bs = [job(len(arrivals) + k) for k in range(args.burst)]
         srv = remc.Server(arena_bytes=args.arena_gib << 30, wpg=args.wpg,
-                          persistent=args.persistent, keep_results=0)
+                          persistent=args.persistent, keep_results=0, occ=args.occ,
+                          kde_policy=args.kde_policy)
         tb = time.perf_counter()
         for sp in bs:
             srv.submit(sp)
@@ -250,7 +256,8 @@ def bench_serve(args):
         "cycles": st["cycles"], "rebuilds": st["rebuilds"],
         "submit_lag_max_s": behind, "wall_s": wall, "job_footprint_max_bytes": fp,
         "burst": burst,
-        "server": {"wpg": args.wpg, "persistent": args.persistent},
+        "server": {"wpg": args.wpg, "persistent": args.persistent, "occ": args.occ,
+                   "kde_policy": args.kde_policy},
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -278,6 +285,10 @@ def main():
     ap.add_argument("--wpg", type=int, default=8)
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--arena-gib", type=int, default=16)
+    ap.add_argument("--occ", type=int, default=6,
+                    help="serving: MC kernel register budget (0, 6 or 8 waves per SIMD)")
+    ap.add_argument("--kde-policy", type=int, default=1,
+                    help="serving: 1 = KDE points read from HBM, 0 = staged in LDS when they fit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -327,6 +338,13 @@ def main():
 
     poses = n_rep * spe * args.steps
     value = poses / elapsed
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "gd_mc_kernel_16384rep_traffic.json")
+    if args.config == "c1" and os.path.exists(tfile):
+        # HBM bytes per cycle launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # passes (calibrated against a device copy), kept with the profile
+        with open(tfile) as f:
+            traffic = json.load(f)["bytes_per_launch"]
     avg_mc_s = (mc_ms / mc_n) / 1e3 if mc_n else float("nan")
     achieved = f_pose * n_rep * spe / avg_mc_s / 1e12
     out = {
@@ -343,7 +361,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, reference parameter files)",
         "config": {
-            "workload": f"{args.config}: one complex, pocket {cfg['pnp']} points "
+            "workload": f"one complex, pocket {cfg['pnp']} points "
                         f"({cfg['radius']} A sphere), ligand {cfg['lna']} atoms, "
                         f"{cfg['n_lig']} lig x {cfg['n_prt']} prt conformers x "
                         f"{cfg['n_tmp']} temperatures = {n_rep} replicas, "
@@ -361,7 +379,10 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, "
+                              "profiles/gd_mc_kernel_16384rep.md" if traffic else None,
+            "hbm_gbps_at_traffic": traffic / (mc_ms / mc_n) / 1e6 if traffic and mc_n else None,
             "flops_per_pose": f_pose,
             "poses_per_launch": n_rep * spe,
             "avg_launch_ms": mc_ms / mc_n if mc_n else None,

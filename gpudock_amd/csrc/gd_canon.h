@@ -121,14 +121,19 @@ GD_HD float gd_exp(float x)
 }
 
 /* exp for the energy loops, where the argument is never NaN and never
- * above 88 (every PMF, H-bond and KDE exponent is <= 18): branch-free, a
- * one-constant Cody-Waite step (|r| error <= 2.4e-7 at |x| = 86) and a
- * degree-6 polynomial fitted here for minimax relative error on
- * |r| <= ln2/2 (6.5e-8 in FP32); total error below 4 ulp. */
+ * above 88 (every PMF, H-bond and KDE exponent is <= 18): branch-free.
+ * The argument is clamped at -86 (below it the result is e^-86 ~ 4e-38,
+ * which no energy sum can tell from 0).  n = round(x / ln2) comes from
+ * adding 1.5 * 2^23 in one fma (the integer lands in the low mantissa bits,
+ * so 2^n is one shift-and-add of those bits); then a one-constant
+ * Cody-Waite step (|r| error <= 2.4e-7 at |x| = 86) and a degree-6
+ * polynomial fitted here for minimax relative error on |r| <= ln2/2
+ * (6.5e-8 in FP32); total error below 4 ulp. */
 GD_HD float gd_exp_hot(float x)
 {
   const float xc = __builtin_fmaxf(x, -86.0f);
-  const float n = gd_rint(xc * 0x1.715476p+0f);
+  const float t = gd_fma(xc, 0x1.715476p+0f, 0x1.8p+23f);
+  const float n = t - 0x1.8p+23f;
   const float r = gd_fma(-n, 0x1.62e430p-1f, xc);
   float p = 0x1.6ab926p-10f;
   p = gd_fma(p, r, 0x1.126d64p-7f);
@@ -137,8 +142,8 @@ GD_HD float gd_exp_hot(float x)
   p = gd_fma(p, r, 0x1.fffffap-2f);
   p = gd_fma(p, r, 1.0f);
   p = gd_fma(p, r, 1.0f);
-  const float v = p * gd_as_float((uint32_t) ((int) n + 127) << 23);
-  return (x < -86.0f) ? 0.0f : v;
+  /* bits(t) = 0x4b400000 + n: (bits << 23) + (127 << 23) = bits of 2^n */
+  return p * gd_as_float((gd_as_uint(t) << 23) + 0x3f800000u);
 }
 
 /* ------------------------------------------------------------------ */

@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -55,6 +56,7 @@ int gd_engine::init(const gd_engine_config *cfg)
   conf = c;
   lc.wpg = (c.wpg == 4 || c.wpg == 16) ? c.wpg : 8;
   lc.persistent = c.persistent ? 1 : 0;
+  lc.occ = (c.occ == 6 || c.occ == 8) ? c.occ : 0;
   lc.stage_bytes = 16;
   GD_HIP_TRY(hipSetDevice(c.device));
   GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
@@ -206,32 +208,79 @@ int gd_engine::rebuild()
 {
   drop_graph();
   std::vector<gd_pack> packs;
-  std::vector<gd_task> tasks;
   std::vector<gd_chain> chains;
-  std::vector<double> cost;
-  int stage = 16;
+  /* occupancy class of every complex: workgroups per CU at its LDS need */
+  std::map<int, int> occ_cache;
+  auto per_cu_of = [&](int stage) -> int {
+    auto it = occ_cache.find(stage);
+    if (it != occ_cache.end())
+      return it->second;
+    int k = 0;
+    if (gd_mc_occupancy(lc, stage, &k) != hipSuccess || k < 1)
+      k = 1;
+    occ_cache[stage] = k;
+    return k;
+  };
+  struct Pending {
+    std::vector<gd_task> tasks;
+    std::vector<double> cost;
+    int stage = 16;
+  };
+  std::map<int, Pending> by_class;   /* key: workgroups per CU */
+  std::vector<gd_task> t1;
   for (auto *s : slots) {
     const int cx = (int) packs.size();
     s->cx_index = cx;
-    const gd_pack &P = s->img->P;
-    packs.push_back(P);
-    const size_t t0 = tasks.size();
-    gd_complex_tasks(*s->img, cx, lc.wpg, tasks);
-    for (size_t t = t0; t < tasks.size(); ++t)
-      cost.push_back((double) tasks[t].r_count * P.lna * P.pnp_pad);
-    const int nc = P.n_prt * P.n_lig;
+    gd_pack p = s->img->P;
+    if (kde_global && p.kde_in_lds) {
+      p.kde_in_lds = 0;
+      p.lds_bytes = p.off_kde;
+    }
+    packs.push_back(p);
+    const int stage = std::max(16, p.lds_bytes);
+    Pending &cl = by_class[per_cu_of(stage)];
+    cl.stage = std::max(cl.stage, stage);
+    t1.clear();
+    gd_complex_tasks(*s->img, cx, lc.wpg, t1);
+    for (const gd_task &t : t1) {
+      cl.tasks.push_back(t);
+      cl.cost.push_back((double) t.r_count * p.lna * p.pnp_pad);
+    }
+    const int nc = p.n_prt * p.n_lig;
     for (int c = 0; c < nc; ++c)
       chains.push_back(gd_chain{cx, c});
-    stage = std::max(stage, P.lds_bytes);
   }
-  /* longest tasks first: the queue hands out big tasks before small ones */
-  std::vector<int> order(tasks.size());
-  for (size_t i = 0; i < order.size(); ++i)
-    order[i] = (int) i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  std::vector<gd_task> sorted(tasks.size());
-  for (size_t i = 0; i < order.size(); ++i)
-    sorted[i] = tasks[order[i]];
+  /* per class, longest tasks first: the queue hands out big tasks before
+   * small ones; classes with the most work launch first */
+  std::vector<gd_task> sorted;
+  classes.clear();
+  std::vector<std::pair<double, int>> order_cls;
+  for (auto &kv : by_class) {
+    double w = 0.0;
+    for (double c : kv.second.cost)
+      w += c;
+    order_cls.push_back({-w, kv.first});
+  }
+  std::sort(order_cls.begin(), order_cls.end());
+  int max_stage = 16;
+  for (auto &oc : order_cls) {
+    Pending &cl = by_class[oc.second];
+    std::vector<int> order(cl.tasks.size());
+    for (size_t i = 0; i < order.size(); ++i)
+      order[i] = (int) i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return cl.cost[a] > cl.cost[b]; });
+    Class c;
+    c.stage = cl.stage;
+    c.off = (int) sorted.size();
+    c.n = (int) order.size();
+    c.per_cu = oc.second;
+    c.grid = lc.persistent ? std::min(c.n, c.per_cu * n_cu) : c.n;
+    for (int i : order)
+      sorted.push_back(cl.tasks[i]);
+    classes.push_back(c);
+    max_stage = std::max(max_stage, c.stage);
+  }
 
   if (packs_bytes) defer(packs_off, packs_bytes);
   if (tasks_bytes) defer(tasks_off, tasks_bytes);
@@ -256,11 +305,8 @@ int gd_engine::rebuild()
   n_cx = (int) packs.size();
   n_tasks = (int) sorted.size();
   n_chain = (int) chains.size();
-  lc.stage_bytes = stage;
-  GD_HIP_TRY(gd_set_kernel_attrs(stage));
-  int per_cu = 0;
-  GD_HIP_TRY(gd_mc_occupancy(lc.wpg, stage, &per_cu));
-  resident = std::max(1, per_cu) * n_cu;
+  lc.stage_bytes = max_stage;
+  GD_HIP_TRY(gd_set_kernel_attrs(max_stage));
   dirty = false;
   ++n_rebuilds;
   return GD_OK;
@@ -276,9 +322,36 @@ int gd_engine::ensure_built()
 
 int gd_engine::grid() const
 {
-  if (!lc.persistent)
-    return n_tasks;
-  return std::min(n_tasks, resident);
+  int g = 0;
+  for (const Class &c : classes)
+    g += c.grid;
+  return g;
+}
+
+int gd_engine::resident() const
+{
+  int r = 0;
+  for (const Class &c : classes)
+    r += c.per_cu * n_cu;
+  return r;
+}
+
+/* MC launches of one cycle, one per occupancy class */
+static hipError_t launch_classes(gd_engine &E, int *queue_base)
+{
+  const gd_pack *packs = (const gd_pack *) (E.arena.base() + E.packs_off);
+  const gd_task *tasks = (const gd_task *) (E.arena.base() + E.tasks_off);
+  for (size_t i = 0; i < E.classes.size(); ++i) {
+    const gd_engine::Class &c = E.classes[i];
+    gd_launch_cfg C = E.lc;
+    C.stage_bytes = c.stage;
+    hipError_t e = gd_launch_mc(C, packs, tasks + c.off, c.n,
+                                E.lc.persistent ? queue_base + i : nullptr, c.grid, 0,
+                                E.stream);
+    if (e != hipSuccess)
+      return e;
+  }
+  return hipSuccess;
 }
 
 int gd_engine::build_image(GdImage &I, const gd_job &job) const
@@ -419,12 +492,10 @@ int gd_engine::remove(int handle)
 int gd_engine::enqueue_cycle()
 {
   const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
-  const gd_task *tasks = (const gd_task *) (arena.base() + tasks_off);
   const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
-  int *queue = lc.persistent ? queue_ptr() : nullptr;
-  GD_HIP_TRY(gd_launch_mc(lc, packs, tasks, n_tasks, queue, grid(), 0, stream));
+  GD_HIP_TRY(launch_classes(*this, queue_ptr()));
   GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, stream));
-  GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), stream));
+  GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), (int) classes.size(), stream));
   return GD_OK;
 }
 
@@ -441,21 +512,19 @@ int gd_engine::cycles(int n)
   for (int c = 0; c < n; ++c) {
     if (timing) {
       const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
-      const gd_task *tasks = (const gd_task *) (arena.base() + tasks_off);
       const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
-      int *queue = lc.persistent ? queue_ptr() : nullptr;
       hipEvent_t a, b, x0, x1;
       GD_HIP_TRY(hipEventCreate(&a));
       GD_HIP_TRY(hipEventCreate(&b));
       GD_HIP_TRY(hipEventCreate(&x0));
       GD_HIP_TRY(hipEventCreate(&x1));
       GD_HIP_TRY(hipEventRecord(a, stream));
-      GD_HIP_TRY(gd_launch_mc(lc, packs, tasks, n_tasks, queue, grid(), 0, stream));
+      GD_HIP_TRY(launch_classes(*this, queue_ptr()));
       GD_HIP_TRY(hipEventRecord(b, stream));
       GD_HIP_TRY(hipEventRecord(x0, stream));
       GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, stream));
       GD_HIP_TRY(hipEventRecord(x1, stream));
-      GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), stream));
+      GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), (int) classes.size(), stream));
       ev_mc.push_back(a);
       ev_mc.push_back(b);
       ev_x.push_back(x0);
@@ -499,10 +568,8 @@ int gd_engine::cycles(int n)
 int gd_engine::trial_cycle()
 {
   const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
-  const gd_task *tasks = (const gd_task *) (arena.base() + tasks_off);
-  int *queue = lc.persistent ? queue_ptr() : nullptr;
-  GD_HIP_TRY(gd_launch_mc(lc, packs, tasks, n_tasks, queue, grid(), 0, stream));
-  GD_HIP_TRY(gd_launch_advance(packs, 0, queue_ptr(), stream));
+  GD_HIP_TRY(launch_classes(*this, queue_ptr()));
+  GD_HIP_TRY(gd_launch_advance(packs, 0, queue_ptr(), (int) classes.size(), stream));
   return GD_OK;
 }
 
@@ -540,51 +607,84 @@ int gd_engine::autotune(gd_tune_report *rep)
     return hipSuccess;
   };
   hipError_t he = copy_state(true);
-  const int wpgs[3] = {4, 8, 16};
+  bool any_kde_lds = false;
+  for (auto *s : slots)
+    any_kde_lds |= s->img->P.kde_in_lds != 0;
   float best = 3.0e38f;
   gd_launch_cfg best_c = lc;
+  int best_kg = kde_global;
   int n = 0;
   hipEvent_t a = nullptr, b = nullptr;
   if (he == hipSuccess)
     he = hipEventCreate(&a);
   if (he == hipSuccess)
     he = hipEventCreate(&b);
-  for (int wi = 0; wi < 3 && he == hipSuccess && !rc; ++wi) {
-    for (int pers = 0; pers < 2 && he == hipSuccess && !rc; ++pers) {
-      lc.wpg = wpgs[wi];
-      lc.persistent = pers;
-      rc = rebuild();
-      if (!rc)
-        rc = trial_cycle();   /* warm-up */
-      if (!rc)
-        he = hipEventRecord(a, stream);
-      if (!rc && he == hipSuccess)
-        rc = trial_cycle();
-      if (!rc && he == hipSuccess)
-        he = hipEventRecord(b, stream);
-      if (!rc && he == hipSuccess)
-        he = hipEventSynchronize(b);
-      float ms = 0.0f;
-      if (!rc && he == hipSuccess)
-        he = hipEventElapsedTime(&ms, a, b);
-      if (rc || he != hipSuccess)
-        break;
-      reap();
-      if (rep && n < 8) {
-        rep->wpg[n] = lc.wpg;
-        rep->persistent[n] = lc.persistent;
-        rep->ms[n] = ms;
-      }
-      ++n;
-      if (ms < best) {
-        best = ms;
-        best_c = lc;
-      }
+  /* one timed trial of a configuration (after a warm-up launch) */
+  auto trial = [&](int wpg, int pers, int kg, int occ) {
+    if (rc || he != hipSuccess)
+      return;
+    kde_global = kg;
+    lc.wpg = wpg;
+    lc.persistent = pers;
+    lc.occ = occ;
+    rc = rebuild();
+    if (!rc)
+      rc = trial_cycle();
+    if (!rc)
+      he = hipEventRecord(a, stream);
+    if (!rc && he == hipSuccess)
+      rc = trial_cycle();
+    if (!rc && he == hipSuccess)
+      he = hipEventRecord(b, stream);
+    if (!rc && he == hipSuccess)
+      he = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (!rc && he == hipSuccess)
+      he = hipEventElapsedTime(&ms, a, b);
+    if (rc || he != hipSuccess)
+      return;
+    reap();
+    if (rep && n < GD_TUNE_MAX) {
+      rep->wpg[n] = wpg;
+      rep->persistent[n] = pers;
+      rep->kde_global[n] = kg;
+      rep->occ[n] = occ;
+      rep->ms[n] = ms;
     }
+    ++n;
+    if (ms < best) {
+      best = ms;
+      best_c = lc;
+      best_kg = kg;
+    }
+  };
+  /* staged search: launch shape; then KDE placement x register budget;
+   * then the launch shape again at the chosen placement and budget */
+  const int wpgs[3] = {4, 8, 16};
+  const int kg0 = kde_global, occ0 = lc.occ;
+  for (int wi = 0; wi < 3; ++wi)
+    for (int pers = 0; pers < 2; ++pers)
+      trial(wpgs[wi], pers, kg0, occ0);
+  {
+    /* KDE placement x register budget at the best launch shape */
+    const gd_launch_cfg c = best_c;
+    for (int kg = 0; kg < (any_kde_lds ? 2 : 1); ++kg)
+      for (int occ : {0, 6, 8})
+        if (kg != kg0 || occ != occ0)
+          trial(c.wpg, c.persistent, any_kde_lds ? kg : kg0, occ);
+  }
+  {
+    const gd_launch_cfg c = best_c;
+    const int kg = best_kg;
+    for (int wi = 0; wi < 3; ++wi)
+      for (int pers = 0; pers < 2; ++pers)
+        if ((wpgs[wi] != c.wpg || pers != c.persistent) && (c.occ != occ0 || kg != kg0))
+          trial(wpgs[wi], pers, kg, c.occ);
   }
   if (a) (void) hipEventDestroy(a);
   if (b) (void) hipEventDestroy(b);
   lc = best_c;
+  kde_global = best_kg;
   if (he == hipSuccess)
     he = copy_state(false);
   defer(snap, snap_bytes);
@@ -594,8 +694,11 @@ int gd_engine::autotune(gd_tune_report *rep)
     return gd_fail(GD_ERR_HIP, "autotune: %s", hipGetErrorString(he));
   if (rep) {
     rep->n = n;
+    rep->n = std::min(n, GD_TUNE_MAX);
     rep->chosen_wpg = lc.wpg;
     rep->chosen_persistent = lc.persistent;
+    rep->chosen_kde_global = kde_global;
+    rep->chosen_occ = lc.occ;
   }
   return rebuild();
 }
@@ -829,7 +932,10 @@ extern "C" int gd_engine_layout(gd_engine *e, gd_engine_layout_info *info)
   info->n_chains = e->n_chain;
   info->n_complexes = e->n_cx;
   info->grid = e->grid();
-  info->resident_groups = e->resident;
+  info->resident_groups = e->resident();
+  info->n_classes = (int) e->classes.size();
+  info->kde_global = e->kde_global;
+  info->occ = e->lc.occ;
   info->arena_used = e->arena.used();
   info->arena_size = e->arena.size();
   int kde_lds = 1;
@@ -1048,6 +1154,7 @@ extern "C" int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_confi
   c.persistent = bc.persistent;
   c.use_graph = bc.no_graph ? 0 : 1;
   c.kde_policy = bc.kde_policy;
+  c.occ = bc.occ;
   int rc = eng.init(&c);
   if (rc)
     return rc;

@@ -86,7 +86,7 @@ struct GdCollect {
 
 struct gd_engine {
   gd_engine_config conf{};
-  gd_launch_cfg lc{8, 16, 0};
+  gd_launch_cfg lc{8, 16, 0, 0};
   int max_lds = 0, n_cu = 0;
   GdArena arena;
   GdArena pinned;                   /* pinned host staging for uploads */
@@ -100,7 +100,15 @@ struct gd_engine {
   size_t packs_off = 0, packs_bytes = 0, tasks_off = 0, tasks_bytes = 0, chains_off = 0,
          chains_bytes = 0;
   int n_cx = 0, n_tasks = 0, n_chain = 0;
-  int resident = 0;                 /* workgroups resident on the device at once */
+  /* MC launches of a cycle: complexes whose LDS need gives the same
+   * workgroups per CU share one launch (tasks [off, off + n) of the task
+   * list, queue counter = class index), so a large pocket does not lower
+   * the occupancy of the small ones */
+  struct Class {
+    int stage, off, n, grid, per_cu;
+  };
+  std::vector<Class> classes;
+  int kde_global = 0;               /* 1: every complex reads KDE points from HBM */
   int timing = 0;
   std::vector<hipEvent_t> ev_mc, ev_x;
   double mc_ms = 0.0, x_ms = 0.0;
@@ -150,7 +158,8 @@ struct gd_engine {
   void drain_timing();
   /* release deferred memory whose work has completed */
   void reap();
-  int grid() const;
+  int grid() const;                 /* workgroups per cycle, all classes */
+  int resident() const;             /* workgroups resident at once, all classes */
   int *queue_ptr();
   size_t free_bytes() const { return arena.size() - arena.used(); }
 

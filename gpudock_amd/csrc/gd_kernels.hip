@@ -388,8 +388,12 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
 /* queue != nullptr: persistent -- tasks are taken from *queue until it      */
 /* passes n_tasks (every workgroup exits once the list is drained).          */
 /* ------------------------------------------------------------------------ */
-template <int WPG>
-__global__ void __launch_bounds__(GD_WAVE *WPG)
+/* WPG: waves (replicas) per workgroup.  MINW: waves per SIMD the register
+ * allocation must allow (1 = the compiler's choice, 92 VGPRs, 5 waves;
+ * 6 = 80 VGPRs; 8 = 64 VGPRs with a few spill slots) -- the launch shape
+ * and this are picked by timing. */
+template <int WPG, int MINW>
+__global__ void __launch_bounds__(GD_WAVE *WPG) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ tasks,
              int n_tasks, int *queue, int mode, int wslot_off)
 {
@@ -497,13 +501,13 @@ gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict
 }
 
 /* After MC and exchange: step[0] += steps of this cycle, step[1] += 1;
- * thread 0 re-arms the persistent work queue. */
+ * threads 0 .. n_queue-1 re-arm the persistent work queues. */
 __global__ void __launch_bounds__(256)
-gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue)
+gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue, int n_queue)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && queue)
-    queue[0] = 0;
+  if (i < n_queue)
+    queue[i] = 0;
   if (i >= n_cx)
     return;
   const gd_pack &P = packs[i];
@@ -706,15 +710,27 @@ gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
 /* ------------------------------------------------------------------------ */
 /* host-callable launch wrappers                                             */
 /* ------------------------------------------------------------------------ */
-template <int WPG>
-static hipError_t launch_mc_wpg(const gd_pack *packs, const gd_task *tasks, int n_tasks,
+template <int WPG, int MINW>
+static hipError_t launch_mc_var(const gd_pack *packs, const gd_task *tasks, int n_tasks,
                                 int *queue, int grid, int mode, int stage_bytes,
                                 hipStream_t st)
 {
   const int lds = stage_bytes + WPG * GD_WSLOT_FLOATS * 4;
-  hipLaunchKernelGGL(gd_mc_kernel<WPG>, dim3(grid), dim3(GD_WAVE * WPG), lds, st, packs,
-                     tasks, n_tasks, queue, mode, stage_bytes);
+  hipLaunchKernelGGL((gd_mc_kernel<WPG, MINW>), dim3(grid), dim3(GD_WAVE * WPG), lds, st,
+                     packs, tasks, n_tasks, queue, mode, stage_bytes);
   return hipGetLastError();
+}
+
+template <int WPG>
+static hipError_t launch_mc_wpg(const gd_launch_cfg &C, const gd_pack *packs,
+                                const gd_task *tasks, int n_tasks, int *queue, int grid,
+                                int mode, hipStream_t st)
+{
+  switch (C.occ) {
+  case 6: return launch_mc_var<WPG, 6>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  case 8: return launch_mc_var<WPG, 8>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  default: return launch_mc_var<WPG, 1>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  }
 }
 
 hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
@@ -723,9 +739,9 @@ hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_t
   if (n_tasks <= 0 || grid <= 0)
     return hipSuccess;
   switch (C.wpg) {
-  case 4: return launch_mc_wpg<4>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
-  case 16: return launch_mc_wpg<16>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
-  default: return launch_mc_wpg<8>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  case 4: return launch_mc_wpg<4>(C, packs, tasks, n_tasks, queue, grid, mode, st);
+  case 16: return launch_mc_wpg<16>(C, packs, tasks, n_tasks, queue, grid, mode, st);
+  default: return launch_mc_wpg<8>(C, packs, tasks, n_tasks, queue, grid, mode, st);
   }
 }
 
@@ -739,11 +755,12 @@ hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int 
   return hipGetLastError();
 }
 
-hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, hipStream_t st)
+hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, int n_queue,
+                             hipStream_t st)
 {
-  const int n = n_cx > 1 ? n_cx : 1;
+  const int n = n_cx > n_queue ? n_cx : (n_queue > 1 ? n_queue : 1);
   hipLaunchKernelGGL(gd_advance_kernel, dim3((n + 255) / 256), dim3(256), 0, st, packs, n_cx,
-                     queue);
+                     queue, n_queue);
   return hipGetLastError();
 }
 
@@ -769,12 +786,21 @@ hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *li
   return hipGetLastError();
 }
 
+template <int WPG, int MINW>
+static hipError_t set_attr_var(int bytes)
+{
+  return hipFuncSetAttribute((const void *) gd_mc_kernel<WPG, MINW>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             bytes + WPG * GD_WSLOT_FLOATS * 4);
+}
+
 template <int WPG>
 static hipError_t set_attr_wpg(int bytes)
 {
-  return hipFuncSetAttribute((const void *) gd_mc_kernel<WPG>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             bytes + WPG * GD_WSLOT_FLOATS * 4);
+  hipError_t e = set_attr_var<WPG, 1>(bytes);
+  if (e == hipSuccess) e = set_attr_var<WPG, 6>(bytes);
+  if (e == hipSuccess) e = set_attr_var<WPG, 8>(bytes);
+  return e;
 }
 
 hipError_t gd_set_kernel_attrs(int stage_bytes)
@@ -788,18 +814,28 @@ hipError_t gd_set_kernel_attrs(int stage_bytes)
   return e;
 }
 
-hipError_t gd_mc_occupancy(int wpg, int stage_bytes, int *blocks_per_cu)
+template <int WPG, int MINW>
+static hipError_t occ_var(int stage_bytes, int *blocks)
 {
-  const int lds = stage_bytes + wpg * GD_WSLOT_FLOATS * 4;
-  switch (wpg) {
-  case 4:
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, gd_mc_kernel<4>,
-                                                        GD_WAVE * 4, lds);
-  case 16:
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, gd_mc_kernel<16>,
-                                                        GD_WAVE * 16, lds);
-  default:
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, gd_mc_kernel<8>,
-                                                        GD_WAVE * 8, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks, gd_mc_kernel<WPG, MINW>, GD_WAVE * WPG, stage_bytes + WPG * GD_WSLOT_FLOATS * 4);
+}
+
+template <int WPG>
+static hipError_t occ_wpg(int occ, int stage_bytes, int *blocks)
+{
+  switch (occ) {
+  case 6: return occ_var<WPG, 6>(stage_bytes, blocks);
+  case 8: return occ_var<WPG, 8>(stage_bytes, blocks);
+  default: return occ_var<WPG, 1>(stage_bytes, blocks);
+  }
+}
+
+hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_per_cu)
+{
+  switch (C.wpg) {
+  case 4: return occ_wpg<4>(C.occ, stage_bytes, blocks_per_cu);
+  case 16: return occ_wpg<16>(C.occ, stage_bytes, blocks_per_cu);
+  default: return occ_wpg<8>(C.occ, stage_bytes, blocks_per_cu);
   }
 }

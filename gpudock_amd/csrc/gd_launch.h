@@ -12,13 +12,16 @@ typedef struct gd_launch_cfg {
   int wpg;          /* wavefronts (replicas) per workgroup: 4, 8 or 16 */
   int stage_bytes;  /* dynamic LDS for staging: max over resident complexes */
   int persistent;   /* 1: grid of resident workgroups pulling from a queue */
+  int occ;          /* register budget for waves per SIMD: 0 (compiler), 6 or 8 */
 } gd_launch_cfg;
 
 hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
                         int n_tasks, int *queue, int grid, int mode, hipStream_t st);
 hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int n_chain,
                               hipStream_t st);
-hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, hipStream_t st);
+/* advances the step counters of packs[0 .. n_cx) and zeroes queue[0 .. n_queue) */
+hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, int n_queue,
+                             hipStream_t st);
 hipError_t gd_launch_score(const gd_pack *packs, const gd_task *tasks, int n_tasks,
                            const int *pose_ids, const int *lig_conf, const float *move,
                            int stage_bytes, float *energy, hipStream_t st);
@@ -27,6 +30,6 @@ hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *li
                                const int *prt_conf, const float *move, int move_stride,
                                int n_pose, double *energy, hipStream_t st);
 hipError_t gd_set_kernel_attrs(int stage_bytes);
-hipError_t gd_mc_occupancy(int wpg, int stage_bytes, int *blocks_per_cu);
+hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_per_cu);
 
 #endif /* GD_LAUNCH_H */

@@ -35,13 +35,13 @@ from . import remc
 PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "params")
 
 CONFIGS = {
-    # configs[0] of the benchmark definition
+    # plain REMC: one small complex, 64 replicas, 1,000 steps
     "c0": dict(pnp=300, radius=10.0, lna=24, n_lig=4, n_prt=2, pnk=300, n_tmp=8,
                t_lo=0.3, t_hi=3.0, steps=1000, spe=10, trans=0.2, rot_deg=3.0),
-    # configs[1]
+    # single large complex: 16,384 replicas, 5,000 steps
     "c1": dict(pnp=800, radius=14.0, lna=48, n_lig=64, n_prt=16, pnk=2000, n_tmp=16,
                t_lo=0.3, t_hi=3.0, steps=5000, spe=10, trans=0.2, rot_deg=3.0),
-    # configs[4]
+    # large-pocket stress: 4,000-point pocket, 128-atom ligand, 4,096 replicas
     "c4": dict(pnp=4000, radius=25.0, lna=128, n_lig=128, n_prt=1, pnk=5000, n_tmp=32,
                t_lo=0.3, t_hi=3.0, steps=2000, spe=10, trans=0.2, rot_deg=3.0),
 }

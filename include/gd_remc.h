@@ -310,24 +310,32 @@ typedef struct gd_engine_config {
   int use_graph;       /* 1: cycles replay one captured hipGraph */
   int kde_policy;      /* enum gd_kde_policy */
   size_t staging_bytes;/* pinned host memory for uploads; 0 = 64 MiB */
+  int occ;             /* MC kernel register budget: 0 (compiler's choice),
+                          6 or 8 waves per SIMD */
 } gd_engine_config;
 
+#define GD_TUNE_MAX 16
 typedef struct gd_tune_report {
   int n;               /* launch shapes tried */
-  int wpg[8];
-  int persistent[8];
-  double ms[8];        /* one MC launch */
-  int chosen_wpg, chosen_persistent;
+  int wpg[GD_TUNE_MAX];
+  int persistent[GD_TUNE_MAX];
+  int kde_global[GD_TUNE_MAX];
+  int occ[GD_TUNE_MAX];
+  double ms[GD_TUNE_MAX];   /* the MC launches of one cycle */
+  int chosen_wpg, chosen_persistent, chosen_kde_global, chosen_occ;
 } gd_tune_report;
 
 typedef struct gd_engine_layout_info {
   int stage_bytes;     /* LDS staged per workgroup */
   int wpg, persistent;
   int n_tasks, n_chains, n_complexes;
-  int grid;            /* workgroups per MC launch */
+  int grid;            /* workgroups per cycle (all MC launches) */
   int resident_groups; /* workgroups the device holds at once */
   int kde_in_lds;      /* 1 if every complex stages its KDE points */
   size_t arena_used, arena_size;
+  int n_classes;       /* MC launches per cycle (occupancy classes) */
+  int kde_global;      /* 1: KDE points read from HBM for every complex */
+  int occ;             /* MC kernel register budget (waves per SIMD, 0 = default) */
 } gd_engine_layout_info;
 
 int gd_engine_create(const gd_engine_config *cfg, gd_engine **out);
@@ -363,6 +371,7 @@ typedef struct gd_batch_config {
   int no_graph;        /* 1: launch the cycle kernels one by one */
   int autotune;
   int kde_policy;
+  int occ;             /* MC kernel register budget: 0, 6 or 8 waves per SIMD */
 } gd_batch_config;
 
 typedef struct gd_batch_report {
@@ -400,6 +409,7 @@ typedef struct gd_server_config {
   int keep_results;            /* 1: results kept until gd_server_fetch;
                                   0: only latency statistics are kept */
   size_t readback_bytes;       /* pinned host memory for read-back; 0 = 64 MiB */
+  int occ;                     /* MC kernel register budget: 0, 6 or 8 */
 } gd_server_config;
 
 enum gd_job_state {

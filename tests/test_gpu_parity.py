@@ -98,7 +98,7 @@ def test_pose_parity_edge_sizes(lna, pnp, pnk, pos):
 
 
 def test_pose_parity_c1_shape():
-    """configs[1] complex (pnp 800, lna 48, 2000 KDE points), a sample of
+    """The 16,384-replica complex (pnp 800, lna 48, 2000 KDE points), a sample of
     poses across all 64 ligand x 16 protein conformers."""
     cx, temps, mc, cfg = synth.make_config("c1")
     rng = np.random.default_rng(12)
@@ -107,7 +107,7 @@ def test_pose_parity_c1_shape():
 
 
 def test_pose_parity_c4_shape():
-    """configs[4] complex (pnp 4000, lna 128, 5000 KDE points): KDE does not
+    """The large-pocket complex (pnp 4000, lna 128, 5000 KDE points): KDE does not
     fit LDS beside the pocket and is read from HBM."""
     cx, temps, mc, cfg = synth.make_config("c4", n_lig=8)
     rng = np.random.default_rng(13)
@@ -122,7 +122,7 @@ def _compare_runs(g, r):
 
 
 def test_run_matches_replay_c0_full():
-    """configs[0] in full: 8 T x 4 ligand x 2 protein conformers, 1000 steps,
+    """The small plain-REMC workload in full: 8 T x 4 ligand x 2 protein conformers, 1000 steps,
     exchange every 10: every decision equals the replay's."""
     cx, temps, mc, cfg = synth.make_config("c0")
     opt = remc.default_options(seed=1)
