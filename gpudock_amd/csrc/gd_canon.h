@@ -307,10 +307,30 @@ typedef struct __attribute__((aligned(16))) {
 #define GD_PT_HPC 0x40u         /* (class 0 and residue GLY) or class 2 */
 #define GD_PT_SEQ_SHIFT 8
 
-/* Scalars shared by every pair. */
+/* Scalars shared by every pair.  The short-range electrostatics
+ * el0 + a1 s^2 + b1 s^3, s = el1 d (calcenergy_d.cu:99-105) is evaluated
+ * as el0 + d^2 (ea + eb d) with ea = a1 el1^2, eb = b1 el1^3, for
+ * d^2 < dcut2 = 1 / el1^2 (s < 1). */
 typedef struct {
-  float lj0, lj1, el0, el1, a1, b1, inv_el1, pad;
+  float lj0, lj1, el0, el1, a1, b1, inv_el1, ea, eb, dcut2, pad0, pad1;
 } gd_pair_scalars;
+
+GD_HD void gd_pair_scalars_init(gd_pair_scalars *sc, float lj0, float lj1, float el0,
+                                float el1, float a1, float b1)
+{
+  sc->lj0 = lj0;
+  sc->lj1 = lj1;
+  sc->el0 = el0;
+  sc->el1 = el1;
+  sc->a1 = a1;
+  sc->b1 = b1;
+  sc->inv_el1 = 1.0f / el1;
+  sc->ea = (a1 * el1) * el1;
+  sc->eb = ((b1 * el1) * el1) * el1;
+  sc->dcut2 = 1.0f / (el1 * el1);
+  sc->pad0 = 0.0f;
+  sc->pad1 = 0.0f;
+}
 
 /* Accumulators of one lane. */
 typedef struct {
@@ -345,9 +365,9 @@ GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
   const float r1 = gd_rsqrt(d2);
   const float d = d2 * r1;
   const float r2 = r1 * r1;
-  const float r4 = r2 * r2;
-  const float r6 = r4 * r2;
-  const float r9 = (r4 * r4) * r1;
+  const float r3 = r2 * r1;
+  const float r6 = r3 * r3;
+  const float r9 = r6 * r3;
 
   /* soft L-J numerator and denominator */
   const float p1 = pp->p1a * r9;
@@ -355,9 +375,8 @@ GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
   const float p4 = gd_fma(p1 * sc->lj0, gd_fma(sc->lj1, d2, 1.0f), 1.0f);
 
   /* distance-dependent electrostatics */
-  const float s = sc->el1 * d;
-  const float g = (s < 1.0f) ? gd_fma(s * s, gd_fma(sc->b1, s, sc->a1), sc->el0)
-                             : r1 * sc->inv_el1;
+  const float g = (d2 < sc->dcut2) ? gd_fma(d2, gd_fma(d, sc->eb, sc->ea), sc->el0)
+                                   : r1 * sc->inv_el1;
   acc->ele = gd_fma(lq_ele, g, acc->ele);
 
   /* PMF contact denominator */

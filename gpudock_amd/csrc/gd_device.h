@@ -36,6 +36,7 @@ typedef struct gd_pack {
   int lna, pnp, pnk, pos;
   int n_rows;            /* distinct ligand atom types = table rows */
   int n_res;             /* PSP residues */
+  int psp_ld;            /* PSP row length: n_res + 1 (last column all zero) */
   int term_order;
   int pnp_pad;           /* pnp rounded up to a multiple of 64 */
 
@@ -55,7 +56,7 @@ typedef struct gd_pack {
   /* pocket, per protein conformer: [n_prt][pnp_pad]; points pnp..pnp_pad-1
    * are padding that contributes exactly zero to every term */
   const float4 *pt_pos;  /* x, y, z, bits(type | flags | seq << 8) */
-  const float2 *pt_eh;   /* ele, hpp */
+  const float2 *pt_eh;   /* ele, hpp (0 outside the hydrophobic class) */
   /* pair table, [n_rows][GD_ROW_STRIDE] */
   const gd_pair_param *tab;
   /* ligand: conformer coordinates [n_lig][lna] (x, y, z, 0) */
@@ -70,7 +71,8 @@ typedef struct gd_pack {
   /* position restraints: targets per (m, atom) [pos][lna] (x, y, z, valid) */
   const float4 *mcs;
   const float *mcs_tcc;  /* [pos] */
-  /* PSP table [GD_MAXTP2][n_res] */
+  /* PSP table [GD_MAXTP2][psp_ld]; point bits hold the column (the zero
+   * column n_res for points outside class 2) */
   const float *psp;
   /* per ligand type (24 entries each) */
   const float *hpl0, *hpl1, *hpl2;

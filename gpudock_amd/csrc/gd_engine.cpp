@@ -1005,6 +1005,8 @@ extern "C" int gd_session_create(const gd_ligand *lig, const gd_protein *prt,
   memset(&c, 0, sizeof c);
   c.device = o.device;
   c.arena_bytes = bytes + ((size_t) 8 << 20) + (size_t) size.n_rep * 256;
+  /* one complex: staging for its block and task lists only */
+  c.staging_bytes = bytes + ((size_t) 1 << 20);
   c.wpg = 8;
   c.persistent = 0;
   c.use_graph = 0;

@@ -116,11 +116,16 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
         bits |= GD_PT_C2;
       if (prt->c0_and_d12_or_c2[i])
         bits |= GD_PT_HPC;
-      bits |= (uint32_t) (prt->c[i] == 2 ? prt->seq3r[i] : 0) << GD_PT_SEQ_SHIFT;
+      /* PSP column: the residue for class-2 points, else the all-zero
+       * column n_res, so the energy loop gathers without a class test */
+      bits |= (uint32_t) (prt->c[i] == 2 ? prt->seq3r[i] : psp->n_res) << GD_PT_SEQ_SHIFT;
       float w;
       memcpy(&w, &bits, 4);
       I.pt_pos[obase + j] = make_float4(prt->x[i], prt->y[i], prt->z[i], w);
-      I.pt_eh[obase + j] = make_float2(prt->ele[i], prt->hpp[i]);
+      /* hpp of points outside the hydrophobic class is stored as 0: their
+       * contribution is then an exact +0 and the loop needs no flag test */
+      I.pt_eh[obase + j] =
+          make_float2(prt->ele[i], prt->c0_and_d12_or_c2[i] ? prt->hpp[i] : 0.0f);
     }
   }
 
@@ -158,7 +163,12 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     }
   }
 
-  I.psp.assign(psp->psp, psp->psp + (size_t) GD_MAXTP2 * psp->n_res);
+  /* PSP [24][n_res + 1]: one extra all-zero column (see the point bits) */
+  P.psp_ld = psp->n_res + 1;
+  I.psp.assign((size_t) GD_MAXTP2 * P.psp_ld, 0.0f);
+  for (int t = 0; t < GD_MAXTP2; ++t)
+    memcpy(&I.psp[(size_t) t * P.psp_ld], psp->psp + (size_t) t * psp->n_res,
+           (size_t) psp->n_res * sizeof(float));
   I.hpl.resize(4 * GD_MAXTP2);
   for (int t = 0; t < GD_MAXTP2; ++t) {
     I.hpl[t] = ep->hpl0[t];
@@ -192,14 +202,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     I.idx_tmp[r] = t;
   }
 
-  P.sc.lj0 = ep->lj0;
-  P.sc.lj1 = ep->lj1;
-  P.sc.el0 = ep->el0;
-  P.sc.el1 = ep->el1;
-  P.sc.a1 = ep->a1;
-  P.sc.b1 = ep->b1;
-  P.sc.inv_el1 = 1.0f / ep->el1;
-  P.sc.pad = 0.0f;
+  gd_pair_scalars_init(&P.sc, ep->lj0, ep->lj1, ep->el0, ep->el1, ep->a1, ep->b1);
   P.kde2 = ep->kde2;
   P.kde3 = ep->kde3;
   for (int i = 0; i < 3; ++i) {

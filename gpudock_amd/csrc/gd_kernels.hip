@@ -116,7 +116,7 @@ __device__ __forceinline__ void gd_energy_wave(
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
-  const int n_res = P.n_res;
+  const int psp_ld = P.psp_ld;
   const gd_pair_scalars sc = P.sc;
 
   gd_pair_acc acc;
@@ -134,21 +134,24 @@ __device__ __forceinline__ void gd_energy_wave(
     const int lt = P.lig_type[l];
     const float lq = P.lig_q[l];
     const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
-    const float *psp_row = P.psp + (size_t) lt * n_res;
+    const __attribute__((address_space(1))) float *psp_row =
+        (const __attribute__((address_space(1))) float *) (P.psp + (size_t) lt * psp_ld);
     float hacc = 0.0f;
     for (int k = 0; k < nchunk; ++k) {
       const int p = lane + k * GD_WAVE;
       const float4 q = s_pos[p];
       const uint32_t bits = __float_as_uint(q.w);
-      const bool c2 = (bits & GD_PT_C2) != 0;
-      /* issue the PSP gather early; it is consumed at the end of the pair */
-      const float pv = c2 ? psp_row[bits >> GD_PT_SEQ_SHIFT] : 0.0f;
+      /* issue the PSP gather early; it is consumed at the end of the pair.
+       * Points outside class 2 read the zero PSP column and hydrophobic
+       * weights outside the class are 0: both add exact zeros, bit for
+       * bit what skipping them gives */
+      const float pv = psp_row[bits >> GD_PT_SEQ_SHIFT];
       const float2 eh = s_eh[p];
       const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      if ((bits & GD_PT_HPC) && d2 <= 81.0f)
+      if (d2 <= 81.0f)
         hacc += gd_hpc_pair(d2, eh.y);
       const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
-      gd_pair_terms(d2, lq * eh.x, &pp, &sc, c2, pv, &acc);
+      gd_pair_terms(d2, lq * eh.x, &pp, &sc, 1, pv, &acc);
     }
     const float hs = gd_wave_sum(hacc);
     const float z = (hs - P.hpl0[lt]) * P.inv_hpl1[lt];
@@ -619,7 +622,7 @@ gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
       const double dm = d - (double) tp.pmf0;
       epmf += (double) tp.pmf1 / (1.0 + exp((-0.5 * d + 6.0) * dm));
       if ((bits & GD_PT_C2) && dm <= 0.0)
-        epsp += (double) P.psp[(size_t) lt * P.n_res + (bits >> GD_PT_SEQ_SHIFT)];
+        epsp += (double) P.psp[(size_t) lt * P.psp_ld + (bits >> GD_PT_SEQ_SHIFT)];
       if (tp.hdb0 > 0.1f) {
         const double h = (d - (double) tp.hdb0) * (double) tp.hdb1;
         ehdb += (double) tp.hdb1 * exp(-0.5 * h * h);

@@ -547,8 +547,7 @@ static void rp_init(rp_ctx *c, const gd_ligand *lig, const gd_protein *prt,
 {
   c->lig = lig; c->prt = prt; c->psp = psp; c->kde = kde; c->mcs = mcs; c->ep = ep;
   c->sz = sz; c->term_order = term_order;
-  c->sc.lj0 = ep->lj0; c->sc.lj1 = ep->lj1; c->sc.el0 = ep->el0; c->sc.el1 = ep->el1;
-  c->sc.a1 = ep->a1; c->sc.b1 = ep->b1; c->sc.inv_el1 = 1.0f / ep->el1; c->sc.pad = 0.0f;
+  gd_pair_scalars_init(&c->sc, ep->lj0, ep->lj1, ep->el0, ep->el1, ep->a1, ep->b1);
   const int n = sz.pnk > 0 ? sz.pnk : 1;
   c->kx = (float *) malloc(sizeof(float) * 3 * n);
   c->ky = c->kx + n;
