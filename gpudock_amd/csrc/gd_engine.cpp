@@ -305,8 +305,40 @@ int gd_engine::rebuild()
   n_cx = (int) packs.size();
   n_tasks = (int) sorted.size();
   n_chain = (int) chains.size();
+  /* initial energies of newly admitted complexes (MonteCarlo_Init_d,
+   * montecarlo_d.cu:12-55): one grid launch over their tasks, queued before
+   * the first cycle that includes them */
+  std::vector<gd_task> init_tasks;
+  int init_stage = 16;
+  for (int h : pending_init) {
+    GdSlot *s = find(h);
+    if (!s)
+      continue;
+    gd_complex_tasks(*s->img, s->cx_index, lc.wpg, init_tasks);
+    init_stage = std::max(init_stage, packs[s->cx_index].lds_bytes);
+  }
+  pending_init.clear();
   lc.stage_bytes = max_stage;
-  GD_HIP_TRY(gd_set_kernel_attrs(max_stage));
+  GD_HIP_TRY(gd_set_kernel_attrs(std::max(max_stage, init_stage)));
+  if (!init_tasks.empty()) {
+    const size_t tb = GdArena::round(vbytes(init_tasks));
+    const size_t toff = arena_take(*this, tb, true);
+    if (toff == (size_t) -1)
+      return gd_fail(GD_ERR_NOMEM, "arena too small for the initial-energy tasks");
+    rc = upload(arena.base() + toff, init_tasks.data(), vbytes(init_tasks));
+    gd_launch_cfg ic = lc;
+    ic.persistent = 0;
+    ic.stage_bytes = init_stage;
+    hipError_t he = rc ? hipSuccess
+                       : gd_launch_mc(ic, (const gd_pack *) (arena.base() + packs_off),
+                                      (const gd_task *) (arena.base() + toff),
+                                      (int) init_tasks.size(), nullptr,
+                                      (int) init_tasks.size(), 1, stream);
+    defer(toff, tb);
+    if (rc)
+      return rc;
+    GD_HIP_TRY(he);
+  }
   dirty = false;
   ++n_rebuilds;
   return GD_OK;
@@ -430,36 +462,9 @@ int gd_engine::add_image(std::unique_ptr<GdImage> &img, int *handle)
   s->handle = next_handle++;
   s->total_steps = I.P.steps_total;
   s->spe = I.P.spe;
-  /* initial energies of this complex alone (grid form over its own pack),
-   * queued behind the cycles already on the stream */
-  std::vector<gd_task> t0;
-  gd_complex_tasks(I, 0, lc.wpg, t0);
-  const size_t tb = GdArena::round(vbytes(t0));
-  const size_t toff = arena_take(*this, tb, true);
-  if (toff == (size_t) -1) {
-    arena.free(off, bytes);
-    return gd_fail(GD_ERR_NOMEM, "arena too small for the initial-energy tasks");
-  }
-  rc = upload(arena.base() + toff, t0.data(), vbytes(t0));
-  if (rc) {
-    arena.free(toff, tb);
-    arena.free(off, bytes);
-    return rc;
-  }
-  hipError_t he = hipSuccess;
-  gd_launch_cfg ic = lc;
-  ic.persistent = 0;
-  ic.stage_bytes = std::max(lc.stage_bytes, I.P.lds_bytes);
-  if (he == hipSuccess)
-    he = gd_set_kernel_attrs(ic.stage_bytes);
-  if (he == hipSuccess)
-    he = gd_launch_mc(ic, s->pack_dev, (const gd_task *) (arena.base() + toff),
-                      (int) t0.size(), nullptr, (int) t0.size(), 1, stream);
-  defer(toff, tb);
-  if (he != hipSuccess) {
-    defer(off, bytes);
-    return gd_fail(GD_ERR_HIP, "admission: %s", hipGetErrorString(he));
-  }
+  /* initial energies: queued with the next task-list build, one launch for
+   * every complex admitted since the last one */
+  pending_init.push_back(s->handle);
   *handle = s->handle;
   slots.push_back(s.release());
   dirty = true;
@@ -479,6 +484,8 @@ int gd_engine::remove(int handle)
   for (size_t i = 0; i < slots.size(); ++i) {
     if (slots[i]->handle == handle) {
       GD_HIP_TRY(hipSetDevice(conf.device));
+      pending_init.erase(std::remove(pending_init.begin(), pending_init.end(), handle),
+                         pending_init.end());
       defer(slots[i]->off, slots[i]->bytes);
       delete slots[i];
       slots.erase(slots.begin() + i);
@@ -744,41 +751,63 @@ size_t gd_engine::collect_bytes(int handle)
   return GdArena::round(s->state_bytes()) + (size_t) s->img->P.n_rep * 10 * sizeof(double);
 }
 
-int gd_engine::collect_async(int handle, GdCollect &c)
+int gd_engine::collect_async(GdCollect *cs, int n, hipEvent_t *done)
 {
-  GdSlot *s = find(handle);
-  if (!s)
-    return gd_fail(GD_ERR_ARG, "no complex with handle %d", handle);
-  if (!c.host || c.host_bytes < collect_bytes(handle))
-    return gd_fail(GD_ERR_ARG, "read-back buffer too small");
+  *done = nullptr;
   GD_HIP_TRY(hipSetDevice(conf.device));
-  const gd_pack &P = s->img->P;
-  const int n = P.n_rep;
-  const size_t e64_bytes = (size_t) n * 10 * sizeof(double);
-  const size_t eb = GdArena::round(e64_bytes);
-  const size_t eoff = arena_take(*this, eb, true);
-  if (eoff == (size_t) -1)
+  int rc = ensure_built();   /* current cx indices; initial energies queued */
+  if (rc)
+    return rc;
+  std::vector<int2> items;
+  std::vector<GdSlot *> ss(n);
+  for (int i = 0; i < n; ++i) {
+    GdSlot *s = find(cs[i].handle);
+    if (!s)
+      return gd_fail(GD_ERR_ARG, "no complex with handle %d", cs[i].handle);
+    if (!cs[i].host || cs[i].host_bytes < collect_bytes(cs[i].handle))
+      return gd_fail(GD_ERR_ARG, "read-back buffer too small");
+    ss[i] = s;
+    for (int r = 0; r < s->img->P.n_rep; ++r)
+      items.push_back(make_int2(s->cx_index, r));
+  }
+  const size_t nitem = items.size();
+  const size_t ib = GdArena::round(nitem * sizeof(int2));
+  const size_t eb = GdArena::round(nitem * 10 * sizeof(double));
+  const size_t ioff = arena_take(*this, ib, true);
+  const size_t eoff = ioff == (size_t) -1 ? (size_t) -1 : arena_take(*this, eb, true);
+  if (eoff == (size_t) -1) {
+    if (ioff != (size_t) -1)
+      arena.free(ioff, ib);
     return gd_fail(GD_ERR_NOMEM, "arena too small for the FP64 re-score");
-  double *e64 = (double *) (arena.base() + eoff);
-  const size_t sb = s->state_bytes();
-  /* FP64 re-score of every replica's best movematrix (stride 8 in the
-   * state), then the state region and the FP64 energies to the host */
-  hipError_t he = gd_launch_rescore64(s->pack_dev, 0, P.rep_lig, P.rep_prt, P.S.mv_best, 8, n,
-                                      e64, stream);
-  if (he == hipSuccess)
-    he = hipMemcpyAsync(c.host, arena.base() + s->off + s->blk.mv, sb, hipMemcpyDeviceToHost,
-                        stream);
-  if (he == hipSuccess)
-    he = hipMemcpyAsync(c.host + GdArena::round(sb), e64, e64_bytes, hipMemcpyDeviceToHost,
-                        stream);
-  if (he == hipSuccess && !c.done)
-    he = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
-  if (he == hipSuccess)
-    he = hipEventRecord(c.done, stream);
+  }
+  const double *e64 = (const double *) (arena.base() + eoff);
+  rc = upload(arena.base() + ioff, items.data(), nitem * sizeof(int2));
+  hipError_t he = rc ? hipSuccess
+                     : gd_launch_rescore64_best((const gd_pack *) (arena.base() + packs_off),
+                                                (const int2 *) (arena.base() + ioff),
+                                                (int) nitem, (double *) e64, stream);
+  size_t first = 0;
+  for (int i = 0; i < n && !rc && he == hipSuccess; ++i) {
+    GdSlot *s = ss[i];
+    const size_t sb = s->state_bytes();
+    const int nr = s->img->P.n_rep;
+    he = hipMemcpyAsync(cs[i].host, arena.base() + s->off + s->blk.mv, sb,
+                        hipMemcpyDeviceToHost, stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(cs[i].host + GdArena::round(sb), e64 + first * 10,
+                          (size_t) nr * 10 * sizeof(double), hipMemcpyDeviceToHost, stream);
+    first += nr;
+  }
+  if (!rc && he == hipSuccess)
+    he = hipEventCreateWithFlags(done, hipEventDisableTiming);
+  if (!rc && he == hipSuccess)
+    he = hipEventRecord(*done, stream);
+  defer(ioff, ib);
   defer(eoff, eb);
+  if (rc)
+    return rc;
   if (he != hipSuccess)
     return gd_fail(GD_ERR_HIP, "read-back: %s", hipGetErrorString(he));
-  c.handle = handle;
   return GD_OK;
 }
 
@@ -789,12 +818,14 @@ int gd_engine::results(int handle, gd_mclog *mclog, gd_replica_result *results)
     return gd_fail(GD_ERR_ARG, "no complex with handle %d", handle);
   std::vector<char> host(collect_bytes(handle));
   GdCollect c;
+  c.handle = handle;
   c.host = host.data();
   c.host_bytes = host.size();
-  int rc = collect_async(handle, c);
+  hipEvent_t done = nullptr;
+  int rc = collect_async(&c, 1, &done);
   hipError_t he = hipStreamSynchronize(stream);
-  if (c.done)
-    (void) hipEventDestroy(c.done);
+  if (done)
+    (void) hipEventDestroy(done);
   if (rc)
     return rc;
   if (he != hipSuccess)

@@ -75,12 +75,11 @@ struct GdSlot {
   size_t state_bytes() const { return blk.step - blk.mv; }
 };
 
-/* A pending read-back of one complex's results (rescore + copies queued on
- * the engine stream into pinned host memory). */
+/* The read-back of one complex's results: its state region, then the FP64
+ * energies of its best poses, copied into host memory at `host`. */
 struct GdCollect {
   int handle = -1;
-  hipEvent_t done = nullptr;
-  char *host = nullptr;             /* state region, then FP64 energies */
+  char *host = nullptr;
   size_t host_off = 0, host_bytes = 0;
 };
 
@@ -95,6 +94,7 @@ struct gd_engine {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<GdSlot *> slots;
+  std::vector<int> pending_init;    /* admitted, initial energies not yet queued */
   int next_handle = 1;
   bool dirty = true;
   size_t packs_off = 0, packs_bytes = 0, tasks_off = 0, tasks_bytes = 0, chains_off = 0,
@@ -149,11 +149,12 @@ struct gd_engine {
   int sync();
   /* synchronous results of one complex */
   int results(int handle, gd_mclog *mclog, gd_replica_result *results);
-  /* queue the FP64 re-score and the read-back of one complex into host
-   * memory at c.host (c.host_bytes >= collect_bytes()); c.done is recorded
+  /* queue, for complexes cs[0..n), one FP64 re-score launch of all their
+   * replicas' best poses and the read-back of each into cs[i].host
+   * (cs[i].host_bytes >= collect_bytes()); *done (created here) is recorded
    * after the copies */
   size_t collect_bytes(int handle);
-  int collect_async(int handle, GdCollect &c);
+  int collect_async(GdCollect *cs, int n, hipEvent_t *done);
   int autotune(gd_tune_report *rep);
   void drain_timing();
   /* release deferred memory whose work has completed */

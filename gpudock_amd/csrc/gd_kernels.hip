@@ -12,6 +12,8 @@
  *                      cycle and re-arms the work queue.
  *   gd_score_kernel    energies of caller-given poses (FP32).
  *   gd_rescore64_kernel FP64 energies of caller-given poses.
+ *   gd_rescore64_best_kernel FP64 energies of the best poses of listed
+ *                      replicas of many complexes (result read-back).
  *
  * Execution model: a workgroup takes a task -- up to WPG replicas of one
  * protein conformer of one complex -- either by its block index or, in the
@@ -560,21 +562,14 @@ gd_score_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ t
 /* FP64 re-score: the reference formulas (calcenergy_d.cu:69-309) in double, */
 /* one wavefront per pose, reading the pack straight from HBM.              */
 /* ------------------------------------------------------------------------ */
-__global__ void __launch_bounds__(GD_WAVE)
-gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
-                    const int *__restrict__ lig_conf, const int *__restrict__ prt_conf,
-                    const float *__restrict__ move, int move_stride, int n_pose,
-                    double *__restrict__ energy)
+__device__ void gd_energy64_wave(const gd_pack &P, int lconf, int pconf, const float *move,
+                                 double *__restrict__ out)
 {
-  const int i = blockIdx.x;
-  if (i >= n_pose)
-    return;
-  const gd_pack &P = packs[cx_index];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna, pnp = P.pnp;
   double m[6];
   for (int k = 0; k < 6; ++k)
-    m[k] = (double) move[(size_t) i * move_stride + k];
+    m[k] = (double) move[k];
   const double s1 = sin(m[3]), c1 = cos(m[3]);
   const double s2 = sin(m[4]), c2 = cos(m[4]);
   const double s3 = sin(m[5]), c3 = cos(m[5]);
@@ -583,9 +578,9 @@ gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
                        -s2,     c2 * s3,                c2 * c3};
   const double cen[3] = {(double) P.center[0] + m[0], (double) P.center[1] + m[1],
                          (double) P.center[2] + m[2]};
-  const float4 *cx = P.lig_xyz + (size_t) lig_conf[i] * lna;
-  const float4 *pp = P.pt_pos + (size_t) prt_conf[i] * P.pnp_pad;
-  const float2 *pe = P.pt_eh + (size_t) prt_conf[i] * P.pnp_pad;
+  const float4 *cx = P.lig_xyz + (size_t) lconf * lna;
+  const float4 *pp = P.pt_pos + (size_t) pconf * P.pnp_pad;
+  const float2 *pe = P.pt_eh + (size_t) pconf * P.pnp_pad;
   const double lj0 = P.sc.lj0, lj1 = P.sc.lj1, el0 = P.sc.el0, el1 = P.sc.el1;
   const double a1 = 4.0 - 3.0 * el0, b1 = 2.0 * el0 - 3.0;
 
@@ -706,8 +701,38 @@ gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
     }
     slot[9] = total;
     for (int k = 0; k < 10; ++k)
-      energy[(size_t) i * 10 + k] = slot[k];
+      out[k] = slot[k];
   }
+}
+
+
+/* FP64 energies of caller-given poses of one complex. */
+__global__ void __launch_bounds__(GD_WAVE)
+gd_rescore64_kernel(const gd_pack *__restrict__ packs, int cx_index,
+                    const int *__restrict__ lig_conf, const int *__restrict__ prt_conf,
+                    const float *__restrict__ move, int move_stride, int n_pose,
+                    double *__restrict__ energy)
+{
+  const int i = blockIdx.x;
+  if (i >= n_pose)
+    return;
+  gd_energy64_wave(packs[cx_index], lig_conf[i], prt_conf[i], move + (size_t) i * move_stride,
+                   energy + (size_t) i * 10);
+}
+
+/* FP64 energies of the best poses of listed replicas (complex, replica) of
+ * many complexes, one launch: the read-back of finished jobs. */
+__global__ void __launch_bounds__(GD_WAVE)
+gd_rescore64_best_kernel(const gd_pack *__restrict__ packs, const int2 *__restrict__ items,
+                         int n, double *__restrict__ energy)
+{
+  const int i = blockIdx.x;
+  if (i >= n)
+    return;
+  const int2 it = items[i];
+  const gd_pack &P = packs[it.x];
+  gd_energy64_wave(P, P.rep_lig[it.y], P.rep_prt[it.y], P.S.mv_best + (size_t) it.y * 8,
+                   energy + (size_t) i * 10);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -775,6 +800,16 @@ hipError_t gd_launch_score(const gd_pack *packs, const gd_task *tasks, int n_tas
     return hipSuccess;
   hipLaunchKernelGGL(gd_score_kernel, dim3(n_tasks), dim3(GD_WAVE * 8), stage_bytes, st,
                      packs, tasks, pose_ids, lig_conf, move, energy);
+  return hipGetLastError();
+}
+
+hipError_t gd_launch_rescore64_best(const gd_pack *packs, const int2 *items, int n,
+                                    double *energy, hipStream_t st)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(gd_rescore64_best_kernel, dim3(n), dim3(GD_WAVE), 0, st, packs, items, n,
+                     energy);
   return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ hipError_t gd_launch_score(const gd_pack *packs, const gd_task *tasks, int n_tas
 hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *lig_conf,
                                const int *prt_conf, const float *move, int move_stride,
                                int n_pose, double *energy, hipStream_t st);
+/* items[i] = (complex index in packs, replica): FP64 energy of its best pose */
+hipError_t gd_launch_rescore64_best(const gd_pack *packs, const int2 *items, int n,
+                                    double *energy, hipStream_t st);
 hipError_t gd_set_kernel_attrs(int stage_bytes);
 hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_per_cu);
 

@@ -98,8 +98,14 @@ struct gd_server {
   void loop();
   bool admit_some();
   void finish(SrvJob *j, int state, int status);
-  void start_collect(SrvJob *j);
+  struct Batch {
+    hipEvent_t done;
+    std::vector<SrvJob *> jobs;
+  };
+  std::vector<Batch> batches;       /* read-backs in flight (worker only) */
+  void start_collects(std::vector<SrvJob *> &ready);
   bool reap_collects();
+  void drop_running(SrvJob *j);
 };
 
 /* Mark a job finished (lock held by the caller). */
@@ -178,34 +184,60 @@ bool gd_server::admit_some()
   }
 }
 
-void gd_server::start_collect(SrvJob *j)
+/* Remove a running job from the engine and the running list (lock held). */
+void gd_server::drop_running(SrvJob *j)
 {
-  const size_t need = eng.collect_bytes(j->handle);
-  size_t off = rb.alloc(need);
-  if (off == (size_t) -1 && need > rb.size()) {
-    j->rb_pageable.resize(need);
-    j->col.host = j->rb_pageable.data();
-  } else if (off == (size_t) -1) {
-    return;   /* read-back area full: retry after others land */
-  } else {
-    j->col.host = rb.base() + off;
-    j->col.host_off = off;
+  (void) eng.remove(j->handle);
+  running.erase(std::find(running.begin(), running.end(), j));
+  resident_reps -= j->info.n_rep;
+  --n_running;
+}
+
+/* Queue one read-back (one FP64 re-score launch) for every job in `ready`
+ * that finds room in the pinned read-back area; the rest wait for a later
+ * round. */
+void gd_server::start_collects(std::vector<SrvJob *> &ready)
+{
+  std::vector<GdCollect> cs;
+  std::vector<SrvJob *> js;
+  for (SrvJob *j : ready) {
+    const size_t need = eng.collect_bytes(j->handle);
+    size_t off = rb.alloc(need);
+    if (off == (size_t) -1 && need > rb.size()) {
+      j->rb_pageable.resize(need);
+      j->col.host = j->rb_pageable.data();
+      j->rb_bytes = 0;
+    } else if (off == (size_t) -1) {
+      continue;   /* read-back area full: retry after others land */
+    } else {
+      j->col.host = rb.base() + off;
+      j->col.host_off = off;
+      j->rb_bytes = need;
+    }
+    j->col.handle = j->handle;
+    j->col.host_bytes = need;
+    cs.push_back(j->col);
+    js.push_back(j);
   }
-  j->col.host_bytes = need;
-  j->rb_bytes = j->rb_pageable.empty() ? need : 0;
-  int rc = eng.collect_async(j->handle, j->col);
+  if (js.empty())
+    return;
+  hipEvent_t done = nullptr;
+  int rc = eng.collect_async(cs.data(), (int) cs.size(), &done);
   if (rc) {
-    if (j->rb_bytes)
-      rb.free(j->col.host_off, j->rb_bytes);
+    if (done)
+      (void) hipEventDestroy(done);
     std::lock_guard<std::mutex> lk(mu);
-    (void) eng.remove(j->handle);
-    running.erase(std::find(running.begin(), running.end(), j));
-    resident_reps -= j->info.n_rep;
-    --n_running;
-    finish(j, GD_JOB_FAILED, rc);
+    for (SrvJob *j : js) {
+      if (j->rb_bytes)
+        rb.free(j->col.host_off, j->rb_bytes);
+      drop_running(j);
+      finish(j, GD_JOB_FAILED, rc);
+    }
     return;
   }
-  j->collecting = true;
+  for (SrvJob *j : js)
+    j->collecting = true;
+  batches.push_back(Batch{done, js});
 }
 
 /* Parse read-backs that have landed; retire their jobs.  Returns true if
@@ -213,39 +245,37 @@ void gd_server::start_collect(SrvJob *j)
 bool gd_server::reap_collects()
 {
   bool any = false;
-  for (size_t i = 0; i < running.size();) {
-    SrvJob *j = running[i];
-    if (!j->collecting || hipEventQuery(j->col.done) != hipSuccess) {
-      ++i;
+  for (size_t b = 0; b < batches.size();) {
+    if (hipEventQuery(batches[b].done) != hipSuccess) {
+      ++b;
       continue;
     }
-    GdSlot *s = eng.find(j->handle);
-    if (s && cfg.keep_results && !j->cancel_req) {
-      j->res.resize(s->img->P.n_rep);
-      gd_mclog lg;
-      memset(&lg, 0, sizeof lg);
-      gd_parse_state(*s->img, s->blk, j->col.host,
-                     (const double *) (j->col.host + GdArena::round(s->state_bytes())), &lg,
-                     j->res.data());
-      j->ac_mc = lg.ac_mc;
-      j->ac_x = lg.ac_temp_exchg;
+    for (SrvJob *j : batches[b].jobs) {
+      GdSlot *s = eng.find(j->handle);
+      if (s && cfg.keep_results && !j->cancel_req) {
+        j->res.resize(s->img->P.n_rep);
+        gd_mclog lg;
+        memset(&lg, 0, sizeof lg);
+        gd_parse_state(*s->img, s->blk, j->col.host,
+                       (const double *) (j->col.host + GdArena::round(s->state_bytes())), &lg,
+                       j->res.data());
+        j->ac_mc = lg.ac_mc;
+        j->ac_x = lg.ac_temp_exchg;
+      }
+      if (j->rb_bytes)
+        rb.free(j->col.host_off, j->rb_bytes);
+      j->rb_pageable.clear();
+      j->rb_pageable.shrink_to_fit();
+      std::lock_guard<std::mutex> lk(mu);
+      drop_running(j);
+      if (j->cancel_req)
+        finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
+      else
+        finish(j, GD_JOB_DONE, GD_OK);
+      any = true;
     }
-    (void) hipEventDestroy(j->col.done);
-    j->col.done = nullptr;
-    if (j->rb_bytes)
-      rb.free(j->col.host_off, j->rb_bytes);
-    j->rb_pageable.clear();
-    j->rb_pageable.shrink_to_fit();
-    (void) eng.remove(j->handle);
-    running.erase(running.begin() + i);
-    std::lock_guard<std::mutex> lk(mu);
-    resident_reps -= j->info.n_rep;
-    --n_running;
-    if (j->cancel_req)
-      finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
-    else
-      finish(j, GD_JOB_DONE, GD_OK);
-    any = true;
+    (void) hipEventDestroy(batches[b].done);
+    batches.erase(batches.begin() + b);
   }
   return any;
 }
@@ -281,11 +311,8 @@ void gd_server::loop()
         ++i;
         continue;
       }
-      (void) eng.remove(j->handle);
-      running.erase(running.begin() + i);
       std::lock_guard<std::mutex> lk(mu);
-      resident_reps -= j->info.n_rep;
-      --n_running;
+      drop_running(j);
       finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
     }
 
@@ -317,12 +344,16 @@ void gd_server::loop()
         break;
       }
     }
-    /* read-backs for jobs whose steps are all queued */
-    for (size_t i = 0; i < running.size(); ++i) {
-      SrvJob *j = running[i];
-      GdSlot *s = eng.find(j->handle);
-      if (s && !j->collecting && s->steps_done >= s->total_steps)
-        start_collect(j);
+    /* read-backs for jobs whose steps are all queued: one batch */
+    {
+      std::vector<SrvJob *> ready;
+      for (SrvJob *j : running) {
+        GdSlot *s = eng.find(j->handle);
+        if (s && !j->collecting && s->steps_done >= s->total_steps)
+          ready.push_back(j);
+      }
+      if (!ready.empty())
+        start_collects(ready);
     }
     reap_collects();
     eng.reap();
@@ -337,12 +368,8 @@ void gd_server::loop()
       collecting |= j->collecting;
     if ((int) ticks.size() >= max_flight) {
       (void) hipEventSynchronize(ticks.front());
-    } else if (!work && collecting) {
-      for (SrvJob *j : running)
-        if (j->collecting) {
-          (void) hipEventSynchronize(j->col.done);
-          break;
-        }
+    } else if (!work && !batches.empty()) {
+      (void) hipEventSynchronize(batches.front().done);
     } else if (!work && !collecting) {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait_for(lk, std::chrono::milliseconds(20),
@@ -352,10 +379,11 @@ void gd_server::loop()
   (void) hipStreamSynchronize(eng.stream);
   for (auto e : ticks)
     (void) hipEventDestroy(e);
+  for (auto &b : batches)
+    (void) hipEventDestroy(b.done);
+  batches.clear();
   std::lock_guard<std::mutex> lk(mu);
   for (SrvJob *j : running) {
-    if (j->col.done)
-      (void) hipEventDestroy(j->col.done);
     --n_running;
     finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
   }
