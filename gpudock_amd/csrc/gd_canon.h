@@ -398,13 +398,26 @@ GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
   }
 }
 
-/* Order of the pocket points inside the canonical sums: stable by
- * (point type, class-2 flag, hydrophobic flag), so that the lanes of one
- * wavefront mostly see one point type (coherent branches, broadcast table
- * reads). */
-GD_HD int gd_point_sort_key(int type, int is_c2, int hpc)
+/* Point types that form an H-bond with at least one ligand type
+ * (hdb0 > 0.1 somewhere in their column of the [n_lt][n_pt] table). */
+GD_HD uint32_t gd_hdb_point_mask(const float *hdb0, int n_lt, int n_pt)
 {
-  return type * 4 + (is_c2 ? 2 : 0) + (hpc ? 1 : 0);
+  uint32_t m = 0u;
+  for (int lt = 0; lt < n_lt; ++lt)
+    for (int pt = 0; pt < n_pt; ++pt)
+      if (hdb0[lt * n_pt + pt] > 0.1f)
+        m |= 1u << pt;
+  return m;
+}
+
+/* Order of the pocket points inside the canonical sums: stable by
+ * (H-bond-capable type first, point type, class-2 flag, hydrophobic flag),
+ * so that the lanes of one wavefront mostly see one point type (coherent
+ * branches, broadcast table reads) and the H-bond branch is skipped by every
+ * wavefront past the H-bond-capable points. */
+GD_HD int gd_point_sort_key(int type, int is_c2, int hpc, uint32_t hdb_mask)
+{
+  return (((hdb_mask >> type) & 1u) ? 0 : 128) + type * 4 + (is_c2 ? 2 : 0) + (hpc ? 1 : 0);
 }
 
 /* Raw (averaged, not normalised) terms in the parameter-file order used by

@@ -97,6 +97,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     I.pt_pos.assign(np, make_float4(1.0e3f, 1.0e3f, 1.0e3f, pw));
     I.pt_eh.assign(np, make_float2(0.0f, 0.0f));
   }
+  const uint32_t hdb_mask = gd_hdb_point_mask(&ep->hdb0[0][0], GD_MAXTP2, GD_MAXTP1);
   std::vector<int> perm(sz.pnp);
   for (int cf = 0; cf < sz.n_prt; ++cf) {
     const size_t base = (size_t) cf * sz.pnp;
@@ -105,9 +106,9 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
       perm[j] = j;
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
       return gd_point_sort_key(prt->t[base + a], prt->c[base + a] == 2,
-                               prt->c0_and_d12_or_c2[base + a] != 0) <
+                               prt->c0_and_d12_or_c2[base + a] != 0, hdb_mask) <
              gd_point_sort_key(prt->t[base + b], prt->c[base + b] == 2,
-                               prt->c0_and_d12_or_c2[base + b] != 0);
+                               prt->c0_and_d12_or_c2[base + b] != 0, hdb_mask);
     });
     for (int j = 0; j < sz.pnp; ++j) {
       const size_t i = base + perm[j];

@@ -515,12 +515,14 @@ typedef struct {
 
 static const gd_protein *g_sort_prt;
 static size_t g_sort_base;
+static uint32_t g_sort_hdb_mask;
 
 static int point_key(int i)
 {
   const gd_protein *p = g_sort_prt;
   const size_t q = g_sort_base + i;
-  return gd_point_sort_key(p->t[q], p->c[q] == 2, p->c0_and_d12_or_c2[q] != 0);
+  return gd_point_sort_key(p->t[q], p->c[q] == 2, p->c0_and_d12_or_c2[q] != 0,
+                           g_sort_hdb_mask);
 }
 
 /* stable insertion-merge sort of indices by point_key (n is small) */
@@ -571,6 +573,7 @@ static void rp_init(rp_ctx *c, const gd_ligand *lig, const gd_protein *prt,
       pp[j] = j;
     g_sort_prt = prt;
     g_sort_base = (size_t) cf * sz.pnp;
+    g_sort_hdb_mask = gd_hdb_point_mask(&ep->hdb0[0][0], GD_MAXTP2, GD_MAXTP1);
     stable_sort_points(pp, sz.pnp);
   }
 }
