@@ -83,6 +83,11 @@ typedef struct gd_pack {
   const int *rep_lig;    /* [n_rep] ligand conformer */
   const int *rep_prt;    /* [n_rep] protein conformer */
   gd_state S;
+  /* per-step records (RecordLigand_d): NULL, or [n_rep][rec_per_window]
+   * records of GD_REC_WORDS words for the steps of the current
+   * steps_per_dump window with step % rec_stride == 0 */
+  uint32_t *rec;
+  int rec_stride, rec_window, rec_per_window, rec_pad;
 
   gd_pair_scalars sc;
   float kde2, kde3;
@@ -91,6 +96,10 @@ typedef struct gd_pack {
   float move_scale[6];
   float a_para[10], b_para[10], w[10];
 } gd_pack;
+
+/* words of one step record: replica (4), energy (10 + MCC), movematrix (6),
+ * step -- the layout of gd_step_record */
+#define GD_REC_WORDS 22
 
 /* One workgroup task: up to waves-per-group replicas of one protein
  * conformer of one complex. */

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -844,6 +845,7 @@ int gd_engine::results(int handle, gd_mclog *mclog, gd_replica_result *results)
 static gd_options default_opts(const gd_options *opt)
 {
   gd_options o;
+  memset(&o, 0, sizeof o);
   if (opt)
     return *opt;
   o.seed = 1;
@@ -1125,6 +1127,82 @@ extern "C" int gd_session_layout(const gd_session *s, int *lds_bytes, int *kde_i
   return GD_OK;
 }
 
+/* Records of the current steps_per_dump window of a one-complex engine:
+ * first step and records per replica, then [n_rep][n_step] records. */
+static int window_records(gd_engine &E, GdSlot *sl, int *first_step, int *n_step,
+                          gd_step_record *dst)
+{
+  const gd_pack &P = sl->img->P;
+  if (P.rec_stride <= 0)
+    return gd_fail(GD_ERR_ARG, "records were not requested (gd_options.record_stride = 0)");
+  int rc = E.sync();
+  if (rc)
+    return rc;
+  const int done = sl->steps_done;
+  if (done == 0) {
+    *first_step = 0;
+    *n_step = 0;
+    return GD_OK;
+  }
+  const int first = (done - 1) / P.rec_window * P.rec_window;
+  const int n = (done - first + P.rec_stride - 1) / P.rec_stride;
+  *first_step = first;
+  *n_step = n;
+  if (dst) {
+    const size_t rb = GD_REC_WORDS * 4;
+    GD_HIP_TRY(hipMemcpy2D(dst, (size_t) n * rb, E.arena.base() + sl->off + sl->blk.rec,
+                           (size_t) P.rec_per_window * rb, (size_t) n * rb, P.n_rep,
+                           hipMemcpyDeviceToHost));
+  }
+  return GD_OK;
+}
+
+extern "C" int gd_session_records(gd_session *s, int *first_step, int *n_step,
+                                  gd_step_record *dst)
+{
+  static_assert(sizeof(gd_step_record) == GD_REC_WORDS * 4, "record layout");
+  if (!s || !first_step || !n_step)
+    return gd_fail(GD_ERR_ARG, "null argument");
+  if (s->eng.slots.empty())
+    return gd_fail(GD_ERR_ARG, "empty session");
+  return window_records(s->eng, s->eng.slots[0], first_step, n_step, dst);
+}
+
+/* <prefix>_%04d.gdtraj for one window (the reference's DumpLigRecord,
+ * hdf5io.C:10-30, called per steps_per_dump window, gpusingle.cu:26-34) */
+static int dump_window(gd_session *s, const char *prefix, int seq)
+{
+  GdSlot *sl = s->eng.slots[0];
+  int first = 0, n = 0;
+  int rc = window_records(s->eng, sl, &first, &n, nullptr);
+  if (rc)
+    return rc;
+  const int n_rep = sl->img->P.n_rep;
+  std::vector<gd_step_record> recs((size_t) n_rep * n);
+  rc = window_records(s->eng, sl, &first, &n, recs.data());
+  if (rc)
+    return rc;
+  char path[4096];
+  snprintf(path, sizeof path, "%s_%04d.gdtraj", prefix, seq);
+  FILE *f = fopen(path, "wb");
+  if (!f)
+    return gd_fail(GD_ERR_IO, "cannot write %s", path);
+  gd_traj_header h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, "GDTRAJ01", 8);
+  h.n_rep = n_rep;
+  h.n_step = n;
+  h.record_bytes = (int32_t) sizeof(gd_step_record);
+  h.first_step = first;
+  h.stride = sl->img->P.rec_stride;
+  const bool ok = fwrite(&h, sizeof h, 1, f) == 1 &&
+                  (recs.empty() || fwrite(recs.data(), sizeof(gd_step_record), recs.size(), f) ==
+                                       recs.size());
+  if (fclose(f) != 0 || !ok)
+    return gd_fail(GD_ERR_IO, "short write to %s", path);
+  return GD_OK;
+}
+
 extern "C" int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
                       const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *enepara,
                       const gd_temp *temp, const gd_replica *replica,
@@ -1139,7 +1217,21 @@ extern "C" int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp 
     return rc;
   const int spe = mcpara->steps_per_exchange;
   const int n_cycles = (mcpara->steps_total + spe - 1) / spe;
-  rc = s->eng.cycles(n_cycles);
+  double t_io = 0.0;
+  if (opt && opt->record_stride > 0 && opt->record_prefix) {
+    /* one window of steps_per_dump steps at a time, dumped after it ends */
+    const int per_window = mcpara->steps_per_dump / spe;
+    int seq = 0;
+    for (int c = 0; c < n_cycles && !rc; c += per_window) {
+      rc = s->eng.cycles(std::min(per_window, n_cycles - c));
+      auto t_w = std::chrono::steady_clock::now();
+      if (!rc)
+        rc = dump_window(s, opt->record_prefix, seq++);
+      t_io += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_w).count();
+    }
+  } else {
+    rc = s->eng.cycles(n_cycles);
+  }
   if (!rc)
     rc = s->eng.sync();
   auto t_comp = std::chrono::steady_clock::now();
@@ -1147,7 +1239,7 @@ extern "C" int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp 
     rc = s->eng.results(s->handle, mclog, results);
   auto t_end = std::chrono::steady_clock::now();
   if (!rc && mclog) {
-    mclog->t0 += std::chrono::duration<double>(t_comp - t_begin).count();
+    mclog->t0 += std::chrono::duration<double>(t_comp - t_begin).count() - t_io;
     mclog->t1 += std::chrono::duration<double>(t_end - t_begin).count();
   }
   gd_session_destroy(s);

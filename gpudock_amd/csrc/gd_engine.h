@@ -40,7 +40,7 @@ struct GdImage {
 struct GdBlock {
   size_t pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_seg, kde_inv, mcs,
       tcc, psp, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
-      acs_x, step, pack, total;
+      acs_x, step, pack, rec, rec_bytes, total;
 };
 
 /* temp, replica and mc may be NULL (pose scoring) */

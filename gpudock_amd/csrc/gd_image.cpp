@@ -49,6 +49,18 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   P.away = o.away;
   P.seed = o.seed;
   P.cid = o.complex_id;
+  if (o.record_stride > 0 && mc) {
+    /* records of one steps_per_dump window stay on the device until read */
+    const int w = mc->steps_per_dump;
+    if (w < 1 || w % mc->steps_per_exchange != 0 || w % o.record_stride != 0)
+      return gd_fail(GD_ERR_ARG,
+                     "records need steps_per_dump (%d) to be a multiple of "
+                     "steps_per_exchange (%d) and of record_stride (%d)",
+                     w, mc->steps_per_exchange, o.record_stride);
+    P.rec_stride = o.record_stride;
+    P.rec_window = w;
+    P.rec_per_window = w / o.record_stride;
+  }
 
   /* table rows: distinct ligand atom types in order of first appearance */
   int row_of_type[GD_MAXTP2];
@@ -275,6 +287,8 @@ GdBlock gd_layout_block(const GdImage &I)
   b.acs_x = take(n * 4);
   b.step = take(2 * 4);
   b.pack = take(sizeof(gd_pack));
+  b.rec_bytes = (size_t) n * I.P.rec_per_window * GD_REC_WORDS * 4;
+  b.rec = I.P.rec_stride > 0 ? take(b.rec_bytes) : 0;
   b.total = off;
   return b;
 }
@@ -343,6 +357,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.S.acs_mc = (int *) (dev + b.acs_mc);
   P.S.acs_x = (int *) (dev + b.acs_x);
   P.step = (int *) (dev + b.step);
+  P.rec = P.rec_stride > 0 ? (uint32_t *) (dev + b.rec) : nullptr;
   memcpy(host + b.pack, &P, sizeof P);
 }
 

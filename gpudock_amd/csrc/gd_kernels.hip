@@ -324,11 +324,31 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
   __builtin_amdgcn_wave_barrier();
   float e_old_total = S.e[r * 10 + GD_E_TOTAL];
   float best_total = S.e_best[r * 10 + GD_E_TOTAL];
-  const float mb = P.minus_beta[S.idx_tmp[r]];
+  const int itmp = S.idx_tmp[r];
+  const float mb = P.minus_beta[itmp];
   const uint32_t seed = P.seed, cid = P.cid;
   int acs = 0;
 
   for (int s = s0; s < s0 + n_steps; ++s) {
+    if (P.rec != nullptr && s % P.rec_stride == 0) {
+      /* RecordLigand_d (util_d.cu:72-94): the state before this step, one
+       * word per lane */
+      uint32_t *rec = P.rec + ((size_t) r * P.rec_per_window +
+                               (s % P.rec_window) / P.rec_stride) * GD_REC_WORDS;
+      float fv = 0.0f;
+      if (lane >= 4 && lane < 14)
+        fv = ws[GD_WS_EOLD + lane - 4];
+      else if (lane >= 15 && lane < 21)
+        fv = ws[GD_WS_MOLD + lane - 15];
+      uint32_t v = __float_as_uint(fv);
+      if (lane == 0) v = (uint32_t) r;
+      if (lane == 1) v = (uint32_t) P.rep_prt[r];
+      if (lane == 2) v = (uint32_t) itmp;
+      if (lane == 3) v = (uint32_t) P.rep_lig[r];
+      if (lane == 21) v = (uint32_t) s;
+      if (lane < GD_REC_WORDS)
+        rec[lane] = v;
+    }
     const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
     const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
     float m_new[6];

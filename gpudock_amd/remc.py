@@ -101,7 +101,19 @@ class McLog(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("complex_id", C.c_uint32), ("exchange", C.c_int),
-                ("away", C.c_int), ("term_order", C.c_int), ("device", C.c_int)]
+                ("away", C.c_int), ("term_order", C.c_int), ("device", C.c_int),
+                ("record_stride", C.c_int), ("record_prefix", C.c_char_p)]
+
+
+class StepRecord(C.Structure):
+    """gd_step_record (LigRecordSingleStep, dock.h:487-494)."""
+    _fields_ = [("replica", Replica), ("energy", F * (MAXWEI + 1)), ("movematrix", F * 6),
+                ("step", C.c_int)]
+
+
+STEP_RECORD_DTYPE = np.dtype([("idx_rep", "<i4"), ("idx_prt", "<i4"), ("idx_tmp", "<i4"),
+                              ("idx_lig", "<i4"), ("energy", "<f4", (MAXWEI + 1,)),
+                              ("movematrix", "<f4", (6,)), ("step", "<i4")])
 
 
 class ReplicaResult(C.Structure):
@@ -214,6 +226,7 @@ SYMBOLS = {
                                     C.POINTER(C.c_int), C.POINTER(C.c_double),
                                     C.POINTER(C.c_int)]),
     "gd_session_layout": (C.c_int, [C.c_void_p, IP, IP, IP, IP]),
+    "gd_session_records": (C.c_int, [C.c_void_p, IP, IP, C.c_void_p]),
     "gd_session_destroy": (None, [C.c_void_p]),
     "gd_engine_create": (C.c_int, [C.POINTER(EngineConfig), C.POINTER(C.c_void_p)]),
     "gd_engine_destroy": (None, [C.c_void_p]),
@@ -408,8 +421,10 @@ def optimize_enepara(ep0):
     return ep
 
 
-def default_options(seed=1, complex_id=0, exchange=1, away=0, term_order=0, device=0):
-    return Options(seed, complex_id, exchange, away, term_order, device)
+def default_options(seed=1, complex_id=0, exchange=1, away=0, term_order=0, device=0,
+                    record_stride=0, record_prefix=None):
+    return Options(seed, complex_id, exchange, away, term_order, device, record_stride,
+                   record_prefix.encode() if isinstance(record_prefix, str) else record_prefix)
 
 
 def mcpara(steps_total, steps_per_exchange, move_scale, steps_per_dump=1000):
@@ -501,6 +516,16 @@ class Session:
         log = McLog()
         check(lib().gd_session_results(self.h, C.byref(log), res))
         return results_to_numpy(res), log
+
+    def records(self):
+        """(first step, [n_rep, n_step] records of STEP_RECORD_DTYPE) of the
+        current steps_per_dump window."""
+        first, n = C.c_int(), C.c_int()
+        check(lib().gd_session_records(self.h, C.byref(first), C.byref(n), None))
+        out = np.zeros((self.size.n_rep, n.value), STEP_RECORD_DTYPE)
+        check(lib().gd_session_records(self.h, C.byref(first), C.byref(n),
+                                       out.ctypes.data_as(C.c_void_p)))
+        return first.value, out
 
     def close(self):
         if self.h:

@@ -195,7 +195,36 @@ typedef struct gd_options {
   int away;             /* IS_AWAY (toggle.h:15): start 44.5*move_scale away */
   int term_order;       /* enum gd_term_order */
   int device;           /* HIP device ordinal */
+  int record_stride;    /* IS_OUTPUT (toggle.h): record the state before every
+                           record_stride-th MC step; 0 = no records */
+  const char *record_prefix; /* gd_run writes <prefix>_%04d.gdtraj per
+                                steps_per_dump window (gpusingle.cu:26-34);
+                                NULL = keep records on the device only */
 } gd_options;
+
+/* LigRecordSingleStep (dock.h:487-494): the state of one replica slot
+ * before MC step `step` (RecordLigand_d, util_d.cu:72-94).  88 bytes. */
+typedef struct gd_step_record {
+  gd_replica replica;          /* idx_rep, idx_prt, idx_tmp held then, idx_lig */
+  float energy[GD_MAXWEI + 1]; /* energy_old e[10] (slot order) + cmcc (0: MCC
+                                  is not computed) */
+  float movematrix[6];
+  int step;
+} gd_step_record;
+
+/* .gdtraj file: this header, then n_rep x n_step records, replica-major
+ * (the reference's LigRecord[n_rep].step[] order); the reference writes
+ * the same records as HDF5 (hdf5io.C:10-30), which this build does not
+ * link. */
+typedef struct gd_traj_header {
+  char magic[8];               /* "GDTRAJ01" */
+  int32_t n_rep;
+  int32_t n_step;              /* records per replica in this file */
+  int32_t record_bytes;        /* sizeof(gd_step_record) = 88 */
+  int32_t first_step;          /* MC step of each replica's first record */
+  int32_t stride;              /* MC steps between records */
+  int32_t reserved[3];
+} gd_traj_header;
 
 /* Per-replica outcome (no counterpart: the reference leaves the final
  * state on the device, montecarlo_d.cu:98-102, and never extracts a best
@@ -263,6 +292,12 @@ int gd_session_results(gd_session *s, gd_mclog *mclog,
 int gd_session_set_timing(gd_session *s, int on);
 int gd_session_timing(gd_session *s, double *mc_ms, int *mc_launches,
                       double *x_ms, int *x_launches);
+/* per-step records (gd_options.record_stride > 0) of the current
+ * steps_per_dump window, the device buffer the reference copies out per dump
+ * (gpusingle.cu:26-34): *first_step and *n_step records per replica, then,
+ * if dst is not NULL, [n_rep][n_step] records into dst */
+int gd_session_records(gd_session *s, int *first_step, int *n_step,
+                       gd_step_record *dst);
 /* LDS bytes per workgroup and whether KDE points are staged in LDS */
 int gd_session_layout(const gd_session *s, int *lds_bytes, int *kde_in_lds,
                       int *n_tasks, int *waves_per_group);

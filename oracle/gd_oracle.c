@@ -722,12 +722,16 @@ ORC_EXPORT void orc_replay_energy(const gd_ligand *lig, const gd_protein *prt,
  * cadence and counter keys.  Fills results[n_rep] (energy_best_fp64 is
  * left zero).  steps_total need not be a multiple of steps_per_exchange;
  * the last, shorter cycle is followed by an exchange like the others. */
-ORC_EXPORT int orc_replay_run(const gd_ligand *lig, const gd_protein *prt,
-                              const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
-                              const gd_enepara *ep, const gd_temp *temp,
-                              const gd_replica *replica, const gd_mcpara *mc,
-                              gd_complex_size sz, const gd_options *opt,
-                              gd_replica_result *res)
+/* The replay with RecordLigand_d (util_d.cu:72-94): before every MC step s
+ * with s % record_stride == 0 the old state goes to
+ * rec[r * n_rec + s / record_stride] (record_stride 0: no records). */
+ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
+                                  const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                                  const gd_enepara *ep, const gd_temp *temp,
+                                  const gd_replica *replica, const gd_mcpara *mc,
+                                  gd_complex_size sz, const gd_options *opt,
+                                  gd_replica_result *res, int record_stride,
+                                  gd_step_record *rec, int n_rec)
 {
   rp_ctx c;
   rp_init(&c, lig, prt, psp, kde, mcs, ep, sz, opt->term_order);
@@ -770,6 +774,17 @@ ORC_EXPORT int orc_replay_run(const gd_ligand *lig, const gd_protein *prt,
       const float mb = temp[it[r]].minus_beta;
       float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3], en[10], mn[6];
       for (int s = s0; s < s1; ++s) {
+        if (record_stride > 0 && s % record_stride == 0 && s / record_stride < n_rec) {
+          gd_step_record *q = &rec[(size_t) r * n_rec + s / record_stride];
+          q->replica.idx_rep = r;
+          q->replica.idx_prt = pc[r];
+          q->replica.idx_tmp = it[r];
+          q->replica.idx_lig = lc[r];
+          memcpy(q->energy, R->energy_final, sizeof(float) * 10);
+          q->energy[10] = 0.0f;
+          memcpy(q->movematrix, R->move_final, sizeof(float) * 6);
+          q->step = s;
+        }
         const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u,
                                              opt->seed, opt->complex_id);
         const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u,
@@ -834,6 +849,17 @@ ORC_EXPORT int orc_replay_run(const gd_ligand *lig, const gd_protein *prt,
   free(it);
   rp_free(&c);
   return 0;
+}
+
+ORC_EXPORT int orc_replay_run(const gd_ligand *lig, const gd_protein *prt,
+                              const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                              const gd_enepara *ep, const gd_temp *temp,
+                              const gd_replica *replica, const gd_mcpara *mc,
+                              gd_complex_size sz, const gd_options *opt,
+                              gd_replica_result *res)
+{
+  return orc_replay_run_rec(lig, prt, psp, kde, mcs, ep, temp, replica, mc, sz, opt, res, 0,
+                            NULL, 0);
 }
 
 /* ====================================================================== */

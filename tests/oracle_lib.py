@@ -35,6 +35,8 @@ def lib():
                                            C.POINTER(remc.McPara), remc.ComplexSize,
                                            C.POINTER(remc.Options),
                                            C.POINTER(remc.ReplicaResult)]
+        L.orc_replay_run_rec.argtypes = L.orc_replay_run.argtypes + [C.c_int, C.c_void_p,
+                                                                     C.c_int]
         L.orc_ref_run.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int,
                                         C.c_int, IP, IP, IP, FP, FP]
@@ -83,6 +85,18 @@ def replay_energy(cx, lig_conf, prt_conf, moves, term_order=0):
     lib().orc_replay_energy(*cx.args(), cx.size, len(moves), _ip(lig_conf), _ip(prt_conf),
                             _fp(moves), term_order, _fp(out))
     return out
+
+
+def replay_run_records(cx, temps, mc, opt, stride):
+    """Replay plus the pre-move state of every stride-th step:
+    (results, records [n_rep, ceil(steps / stride)])."""
+    size = cx.sized(len(temps))
+    res = (remc.ReplicaResult * size.n_rep)()
+    n_rec = (mc.steps_total + stride - 1) // stride
+    rec = np.zeros((size.n_rep, n_rec), remc.STEP_RECORD_DTYPE)
+    lib().orc_replay_run_rec(*cx.args(), temps, None, C.byref(mc), size, C.byref(opt), res,
+                             stride, rec.ctypes.data_as(C.c_void_p), n_rec)
+    return remc.results_to_numpy(res), rec
 
 
 def replay_run(cx, temps, mc, opt):
