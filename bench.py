@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mcc", action="store_true",
+                    help="compute the per-pose MCC contact score (IS_CALCU_MCC)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape (8 waves per workgroup, grid form)")
     ap.add_argument("--workload", choices=("single", "batch", "serve"), default="single",
@@ -316,7 +318,7 @@ def main():
     # every warm-up and timed cycle must do a full spe steps
     mc.steps_total = max(mc.steps_total, (args.warmup + args.steps) * spe)
     f_pose, m_kde = flops_per_pose(cx)
-    opt = remc.default_options(seed=args.seed)
+    opt = remc.default_options(seed=args.seed, mcc=int(args.mcc))
     job = remc.JobSpec(cx, temps, mc, opt)
     eng = remc.Engine(arena_bytes=job.footprint() + (64 << 20), use_graph=1)
     h = eng.add(job)
@@ -368,6 +370,7 @@ def main():
                         f"{cfg['pnk']} KDE points, exchange every {spe} steps",
             "replicas": n_rep, "lna": cfg["lna"], "pnp": cfg["pnp"], "pnk": cfg["pnk"],
             "steps_per_exchange": spe, "mc_steps_timed": spe * args.steps,
+            "mcc": bool(args.mcc),
             "parallelism": "single GPU",
         },
         "pair_interactions_per_s": value * cfg["lna"] * cfg["pnp"],

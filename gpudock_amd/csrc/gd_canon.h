@@ -357,9 +357,9 @@ GD_HD float gd_hpc_pair(float d2, float hpp)
  * psp[ligand type][residue] of the point (any value for points that are
  * not class 2); it is added for class-2 points inside the PMF radius
  * (calcenergy_d.cu:123-126). */
-GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
-                         const gd_pair_scalars *sc, int is_c2, float psp_val,
-                         gd_pair_acc *acc)
+GD_HD int gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
+                        const gd_pair_scalars *sc, int is_c2, float psp_val,
+                        gd_pair_acc *acc)
 {
   /* all inverse powers from one reciprocal square root */
   const float r1 = gd_rsqrt(d2);
@@ -396,6 +396,18 @@ GD_HD void gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
     const float h = (d - pp->hdb0) * pp->hdb1;
     acc->hdb = gd_fma(pp->hdb1, gd_exp_hot((-0.5f * h) * h), acc->hdb);
   }
+  /* contact (d <= pmf0): the PSP gate above and the MCC contact test
+   * (calcmcc_d.cu:72-73, 182-189) */
+  return dm <= 0.0f;
+}
+
+/* Matthews correlation of the contact confusion counts, in the reference's
+ * FP32 expression order (calcmcc_d.cu:196-207); 10000 when undefined. */
+#define GD_CMCC_INVALID 10000.0f
+GD_HD float gd_mcc(float tp, float fn, float fp, float tn)
+{
+  const float dividend = gd_sqrt((tp + fp) * (tp + fn) * (tn + fp) * (tn + fn));
+  return dividend != 0.0f ? (tp * tn - fp * fn) / dividend : GD_CMCC_INVALID;
 }
 
 /* Point types that form an H-bond with at least one ligand type

@@ -27,6 +27,8 @@ typedef struct gd_state {
   int *idx_tmp;     /* [n_rep] temperature index held by the slot */
   int *acs_mc;      /* [n_rep] */
   int *acs_x;       /* [n_rep] */
+  float *cmcc;      /* [n_rep] energy_old.cmcc (0 without MCC) */
+  float *cmcc_best; /* [n_rep] cmcc of the best pose */
 } gd_state;
 
 /* One complex. */
@@ -88,6 +90,11 @@ typedef struct gd_pack {
    * steps_per_dump window with step % rec_stride == 0 */
   uint32_t *rec;
   int rec_stride, rec_window, rec_per_window, rec_pad;
+  /* MCC (IS_CALCU_MCC): reference contacts of replica 0's ligand conformer
+   * at its start pose, per protein conformer in its canonical point order:
+   * bit j of mcc_ref[(c * lna + l) * nchunk + k] = contact of atom l with
+   * point k * 64 + j; NULL = no MCC */
+  const unsigned long long *mcc_ref;
 
   gd_pair_scalars sc;
   float kde2, kde3;
@@ -117,11 +124,12 @@ typedef struct gd_chain {
 } gd_chain;
 
 /* Per-wave LDS slot holding the replica's uniform state between steps:
- * m_old[6] e_old[10] m_new[6] e_new[10] */
-#define GD_WSLOT_FLOATS 32
+ * m_old[6] e_old[10] m_new[6] e_new[10] cmcc_old */
+#define GD_WSLOT_FLOATS 36
 #define GD_WS_MOLD 0
 #define GD_WS_EOLD 6
 #define GD_WS_MNEW 16
 #define GD_WS_ENEW 22
+#define GD_WS_CMCC 32     /* energy_old.cmcc */
 
 #endif /* GD_DEVICE_H */

@@ -32,6 +32,7 @@ struct GdImage {
   std::vector<float> hpl;
   std::vector<float> mb;
   std::vector<int> rep_lig, rep_prt, idx_tmp;
+  std::vector<unsigned long long> mcc_ref;   /* [n_prt][lna][nchunk] */
 };
 
 /* Byte offsets of a complex's arrays inside its arena block.  The replica
@@ -40,7 +41,7 @@ struct GdImage {
 struct GdBlock {
   size_t pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_seg, kde_inv, mcs,
       tcc, psp, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
-      acs_x, step, pack, rec, rec_bytes, total;
+      acs_x, cmcc, cmcc_best, step, pack, rec, rec_bytes, mcc_ref, total;
 };
 
 /* temp, replica and mc may be NULL (pose scoring) */

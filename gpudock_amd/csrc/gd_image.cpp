@@ -111,6 +111,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   }
   const uint32_t hdb_mask = gd_hdb_point_mask(&ep->hdb0[0][0], GD_MAXTP2, GD_MAXTP1);
   std::vector<int> perm(sz.pnp);
+  std::vector<int> perms((size_t) sz.n_prt * sz.pnp);   /* sorted slot -> point */
   for (int cf = 0; cf < sz.n_prt; ++cf) {
     const size_t base = (size_t) cf * sz.pnp;
     const size_t obase = (size_t) cf * pnp_pad;
@@ -122,6 +123,8 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
              gd_point_sort_key(prt->t[base + b], prt->c[base + b] == 2,
                                prt->c0_and_d12_or_c2[base + b] != 0, hdb_mask);
     });
+    for (int j = 0; j < sz.pnp; ++j)
+      perms[(size_t) cf * sz.pnp + j] = perm[j];
     for (int j = 0; j < sz.pnp; ++j) {
       const size_t i = base + perm[j];
       uint32_t bits = (uint32_t) prt->t[i] & GD_PT_TYPE_MASK;
@@ -216,6 +219,38 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   }
 
   gd_pair_scalars_init(&P.sc, ep->lj0, ep->lj1, ep->el0, ep->el1, ep->a1, ep->b1);
+
+  /* MCC reference contacts (InitRefMatrix_d, calcmcc_d.cu:57-83): replica 0's
+   * ligand conformer at its start pose (movematrix 0, before IS_AWAY) against
+   * replica 0's protein conformer, contact = d <= pmf0 with the pair loop's
+   * own d; stored per protein conformer in that conformer's point order */
+  I.mcc_ref.clear();
+  if (o.mcc && mc) {
+    const int nchunk = pnp_pad / GD_WAVE;
+    const int l0 = I.rep_lig[0], p0 = I.rep_prt[0];
+    const float m0[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float rot[9];
+    gd_rotation(m0, rot);
+    std::vector<char> ref((size_t) sz.lna * sz.pnp);
+    for (int a = 0; a < sz.lna; ++a) {
+      const size_t li = (size_t) l0 * sz.lna + a;
+      float lx, ly, lz;
+      gd_transform(rot, m0, lig->center, lig->x[li], lig->y[li], lig->z[li], &lx, &ly, &lz);
+      for (int q = 0; q < sz.pnp; ++q) {
+        const size_t pi = (size_t) p0 * sz.pnp + q;
+        const float d2 = gd_dist2(lx - prt->x[pi], ly - prt->y[pi], lz - prt->z[pi]);
+        const float d = d2 * gd_rsqrt(d2);
+        ref[(size_t) a * sz.pnp + q] = (d - ep->pmf0[lig->t[a]][prt->t[pi]]) <= 0.0f;
+      }
+    }
+    I.mcc_ref.assign((size_t) sz.n_prt * sz.lna * nchunk, 0ull);
+    for (int cf = 0; cf < sz.n_prt; ++cf)
+      for (int a = 0; a < sz.lna; ++a)
+        for (int j = 0; j < sz.pnp; ++j)
+          if (ref[(size_t) a * sz.pnp + perms[(size_t) cf * sz.pnp + j]])
+            I.mcc_ref[((size_t) cf * sz.lna + a) * nchunk + j / GD_WAVE] |=
+                1ull << (j % GD_WAVE);
+  }
   P.kde2 = ep->kde2;
   P.kde3 = ep->kde3;
   for (int i = 0; i < 3; ++i) {
@@ -285,10 +320,13 @@ GdBlock gd_layout_block(const GdImage &I)
   b.idx_tmp = take(n * 4);
   b.acs_mc = take(n * 4);
   b.acs_x = take(n * 4);
+  b.cmcc = take(n * 4);
+  b.cmcc_best = take(n * 4);
   b.step = take(2 * 4);
   b.pack = take(sizeof(gd_pack));
   b.rec_bytes = (size_t) n * I.P.rec_per_window * GD_REC_WORDS * 4;
   b.rec = I.P.rec_stride > 0 ? take(b.rec_bytes) : 0;
+  b.mcc_ref = I.mcc_ref.empty() ? 0 : take(vbytes(I.mcc_ref));
   b.total = off;
   return b;
 }
@@ -327,6 +365,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   put(b.rep_lig, I.rep_lig.data(), vbytes(I.rep_lig));
   put(b.rep_prt, I.rep_prt.data(), vbytes(I.rep_prt));
   put(b.idx_tmp, I.idx_tmp.data(), vbytes(I.idx_tmp));
+  put(b.mcc_ref, I.mcc_ref.data(), vbytes(I.mcc_ref));
 
   P.pt_pos = (const float4 *) (dev + b.pt_pos);
   P.pt_eh = (const float2 *) (dev + b.pt_eh);
@@ -356,6 +395,9 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.S.idx_tmp = (int *) (dev + b.idx_tmp);
   P.S.acs_mc = (int *) (dev + b.acs_mc);
   P.S.acs_x = (int *) (dev + b.acs_x);
+  P.S.cmcc = (float *) (dev + b.cmcc);
+  P.S.cmcc_best = (float *) (dev + b.cmcc_best);
+  P.mcc_ref = I.mcc_ref.empty() ? nullptr : (const unsigned long long *) (dev + b.mcc_ref);
   P.step = (int *) (dev + b.step);
   P.rec = P.rec_stride > 0 ? (uint32_t *) (dev + b.rec) : nullptr;
   memcpy(host + b.pack, &P, sizeof P);
@@ -394,6 +436,8 @@ void gd_parse_state(const GdImage &I, const GdBlock &b, const char *state, const
   const int *it = (const int *) at(b.idx_tmp);
   const int *acs = (const int *) at(b.acs_mc);
   const int *acx = (const int *) at(b.acs_x);
+  const float *cm = (const float *) at(b.cmcc);
+  const float *cmb = (const float *) at(b.cmcc_best);
   if (results) {
     for (int r = 0; r < n; ++r) {
       gd_replica_result &R = results[r];
@@ -410,6 +454,8 @@ void gd_parse_state(const GdImage &I, const GdBlock &b, const char *state, const
       R.idx_tmp_final = it[r];
       R.acs_mc = acs[r];
       R.acs_exchg = acx[r];
+      R.cmcc_final = cm[r];
+      R.cmcc_best = cmb[r];
     }
   }
   if (mclog) {

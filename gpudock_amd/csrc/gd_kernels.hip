@@ -104,11 +104,13 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
 /* ------------------------------------------------------------------------ */
 /* Energy of one pose, one wavefront (calcenergy_d.cu:22-374).               */
 /* Lane j holds ligand atoms j and j + 64 in (ax0..az0) and (ax1..az1).      */
-/* On return out[] (slot order) is identical in every lane.                  */
+/* On return out[] (slot order) and *cmcc (MCC against the reference       */
+/* contacts, CalcMcc_d calcmcc_d.cu:148-213, or 0) are identical in every   */
+/* lane.  prt = the protein conformer staged in smem.                       */
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void gd_energy_wave(
-    const gd_pack &P, const char *smem, float ax0, float ay0, float az0, float ax1,
-    float ay1, float az1, const float cen[3], float out[GD_E_N])
+    const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
+    float ay1, float az1, const float cen[3], float out[GD_E_N], float *cmcc)
 {
   const float4 *__restrict__ s_pos = (const float4 *) smem;
   const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
@@ -124,6 +126,11 @@ __device__ __forceinline__ void gd_energy_wave(
   gd_pair_acc acc;
   acc.vdw = 0.0f; acc.ele = 0.0f; acc.pmf = 0.0f; acc.psp = 0.0f; acc.hdb = 0.0f;
   float ehpc = 0.0f;
+  /* MCC confusion counts (wave-uniform): contacts now vs reference, from one
+   * ballot per chunk against the reference mask */
+  const unsigned long long *mref =
+      P.mcc_ref ? P.mcc_ref + (size_t) prt * lna * nchunk : nullptr;
+  int tp = 0, fp = 0, fn = 0;
 
   /* pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) */
   for (int l = 0; l < lna; ++l) {
@@ -153,7 +160,14 @@ __device__ __forceinline__ void gd_energy_wave(
       if (d2 <= 81.0f)
         hacc += gd_hpc_pair(d2, eh.y);
       const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
-      gd_pair_terms(d2, lq * eh.x, &pp, &sc, 1, pv, &acc);
+      const int contact = gd_pair_terms(d2, lq * eh.x, &pp, &sc, 1, pv, &acc);
+      if (mref) {
+        const unsigned long long now = __ballot(contact);
+        const unsigned long long ref = mref[(size_t) l * nchunk + k];
+        tp += __popcll(now & ref);
+        fp += __popcll(now & ~ref);
+        fn += __popcll(~now & ref);
+      }
     }
     const float hs = gd_wave_sum(hacc);
     const float z = (hs - P.hpl0[lt]) * P.inv_hpl1[lt];
@@ -227,6 +241,11 @@ __device__ __forceinline__ void gd_energy_wave(
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;
   gd_finalize(sums, (float) lna, P.a_para, P.b_para, P.w, P.term_order, out);
+  *cmcc = 0.0f;
+  if (mref) {
+    const int tn = lna * P.pnp - tp - fp - fn;   /* padding points are never contacts */
+    *cmcc = gd_mcc((float) tp, (float) fn, (float) fp, (float) tn);
+  }
 }
 
 /* Move the conformer held by this lane (move_d.cu:86-132). */
@@ -290,10 +309,12 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
     for (int k = 0; k < 6; ++k)
       m[k] = P.move_scale[k] * 44.5f + m[k];
   }
-  float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N];
+  float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N], cm;
   gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
-  gd_energy_wave(P, smem, ax0, ay0, az0, ax1, ay1, az1, cen, e);
+  gd_energy_wave(P, smem, P.rep_prt[r], ax0, ay0, az0, ax1, ay1, az1, cen, e, &cm);
   if (lane == 0) {
+    S.cmcc[r] = cm;
+    S.cmcc_best[r] = cm;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       S.mv[r * 8 + k] = m[k];
@@ -311,7 +332,8 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
 /* n_steps MC steps from step s0.  The movematrices and energies live in
  * the wave's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in registers. */
 __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *smem,
-                                                 float *ws, int r, int s0, int n_steps)
+                                                 float *ws, int r, int prt, int s0,
+                                                 int n_steps)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const gd_state S = P.S;
@@ -321,6 +343,8 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
     ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
   if (lane < GD_E_N)
     ws[GD_WS_EOLD + lane] = S.e[r * 10 + lane];
+  if (lane == 0)
+    ws[GD_WS_CMCC] = S.cmcc[r];
   __builtin_amdgcn_wave_barrier();
   float e_old_total = S.e[r * 10 + GD_E_TOTAL];
   float best_total = S.e_best[r * 10 + GD_E_TOTAL];
@@ -338,6 +362,8 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
       float fv = 0.0f;
       if (lane >= 4 && lane < 14)
         fv = ws[GD_WS_EOLD + lane - 4];
+      else if (lane == 14)
+        fv = ws[GD_WS_CMCC];
       else if (lane >= 15 && lane < 21)
         fv = ws[GD_WS_MOLD + lane - 15];
       uint32_t v = __float_as_uint(fv);
@@ -360,7 +386,7 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
     m_new[5] = P.move_scale[5] * (2.0f * gd_u01(rb.v[1]) - 1.0f) + ws[GD_WS_MOLD + 5];
     const float u_acc = gd_u01(rb.v[2]);
 
-    float ax0, ay0, az0, ax1, ay1, az1, cen[3], e_new[GD_E_N];
+    float ax0, ay0, az0, ax1, ay1, az1, cen[3], e_new[GD_E_N], cm_new;
     gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
              &az1, cen);
     if (lane == 0) {
@@ -368,7 +394,7 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
       for (int k = 0; k < 6; ++k)
         ws[GD_WS_MNEW + k] = m_new[k];
     }
-    gd_energy_wave(P, smem, ax0, ay0, az0, ax1, ay1, az1, cen, e_new);
+    gd_energy_wave(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, cen, e_new, &cm_new);
 
     const float de = e_new[GD_E_TOTAL] - e_old_total;
     if (u_acc < gd_exp(de * mb)) {
@@ -381,6 +407,7 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
 #pragma unroll
         for (int k = 0; k < 6; ++k)
           ws[GD_WS_MOLD + k] = ws[GD_WS_MNEW + k];
+        ws[GD_WS_CMCC] = cm_new;
       }
       if (e_new[GD_E_TOTAL] < best_total) {
         best_total = e_new[GD_E_TOTAL];
@@ -391,6 +418,7 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
 #pragma unroll
           for (int i = 0; i < GD_E_N; ++i)
             S.e_best[r * 10 + i] = e_new[i];
+          S.cmcc_best[r] = cm_new;
           S.step_best[r] = s;
         }
       }
@@ -402,8 +430,10 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
     S.mv[r * 8 + lane] = ws[GD_WS_MOLD + lane];
   if (lane < GD_E_N)
     S.e[r * 10 + lane] = ws[GD_WS_EOLD + lane];
-  if (lane == 0)
+  if (lane == 0) {
     S.acs_mc[r] += acs;
+    S.cmcc[r] = ws[GD_WS_CMCC];
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -459,7 +489,7 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
       if (w < task.r_count) {
         const int r = task.r_begin + w;
         if (mode == 0)
-          gd_replica_cycle(P, smem, ws, r, s0, n_steps);
+          gd_replica_cycle(P, smem, ws, r, task.prt, s0, n_steps);
         else
           gd_replica_init(P, smem, r);
       }
@@ -570,7 +600,8 @@ gd_score_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ t
     m[k] = move[i * 6 + k];
   float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N];
   gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
-  gd_energy_wave(P, smem, ax0, ay0, az0, ax1, ay1, az1, cen, e);
+  float cm;
+  gd_energy_wave(P, smem, task.prt, ax0, ay0, az0, ax1, ay1, az1, cen, e, &cm);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < GD_E_N; ++k)

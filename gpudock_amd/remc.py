@@ -102,7 +102,8 @@ class McLog(C.Structure):
 class Options(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("complex_id", C.c_uint32), ("exchange", C.c_int),
                 ("away", C.c_int), ("term_order", C.c_int), ("device", C.c_int),
-                ("record_stride", C.c_int), ("record_prefix", C.c_char_p)]
+                ("record_stride", C.c_int), ("record_prefix", C.c_char_p),
+                ("mcc", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -120,7 +121,8 @@ class ReplicaResult(C.Structure):
     _fields_ = [("move_final", F * 6), ("energy_final", F * MAXWEI),
                 ("move_best", F * 6), ("energy_best", F * MAXWEI),
                 ("energy_best_fp64", C.c_double * MAXWEI), ("step_best", C.c_int),
-                ("idx_tmp_final", C.c_int), ("acs_mc", C.c_int), ("acs_exchg", C.c_int)]
+                ("idx_tmp_final", C.c_int), ("acs_mc", C.c_int), ("acs_exchg", C.c_int),
+                ("cmcc_final", F), ("cmcc_best", F)]
 
 
 class Job(C.Structure):
@@ -422,9 +424,10 @@ def optimize_enepara(ep0):
 
 
 def default_options(seed=1, complex_id=0, exchange=1, away=0, term_order=0, device=0,
-                    record_stride=0, record_prefix=None):
+                    record_stride=0, record_prefix=None, mcc=0):
     return Options(seed, complex_id, exchange, away, term_order, device, record_stride,
-                   record_prefix.encode() if isinstance(record_prefix, str) else record_prefix)
+                   record_prefix.encode() if isinstance(record_prefix, str) else record_prefix,
+                   mcc)
 
 
 def mcpara(steps_total, steps_per_exchange, move_scale, steps_per_dump=1000):
@@ -443,6 +446,8 @@ def results_to_numpy(res):
         "idx_tmp_final": np.array([r.idx_tmp_final for r in res], np.int32),
         "acs_mc": np.array([r.acs_mc for r in res], np.int32),
         "acs_exchg": np.array([r.acs_exchg for r in res], np.int32),
+        "cmcc_final": np.array([r.cmcc_final for r in res], np.float32),
+        "cmcc_best": np.array([r.cmcc_best for r in res], np.float32),
     }
     return out
 

@@ -200,14 +200,17 @@ typedef struct gd_options {
   const char *record_prefix; /* gd_run writes <prefix>_%04d.gdtraj per
                                 steps_per_dump window (gpusingle.cu:26-34);
                                 NULL = keep records on the device only */
+  int mcc;              /* IS_CALCU_MCC (toggle.h:18): Matthews correlation of
+                           each pose's contacts against the contacts of
+                           replica 0's ligand at its start pose */
 } gd_options;
 
 /* LigRecordSingleStep (dock.h:487-494): the state of one replica slot
  * before MC step `step` (RecordLigand_d, util_d.cu:72-94).  88 bytes. */
 typedef struct gd_step_record {
   gd_replica replica;          /* idx_rep, idx_prt, idx_tmp held then, idx_lig */
-  float energy[GD_MAXWEI + 1]; /* energy_old e[10] (slot order) + cmcc (0: MCC
-                                  is not computed) */
+  float energy[GD_MAXWEI + 1]; /* energy_old e[10] (slot order) + cmcc (0
+                                  without gd_options.mcc) */
   float movematrix[6];
   int step;
 } gd_step_record;
@@ -239,6 +242,8 @@ typedef struct gd_replica_result {
   int idx_tmp_final;    /* temperature index held at the end */
   int acs_mc;           /* accepted MC moves */
   int acs_exchg;        /* accepted temperature swaps involving this slot */
+  float cmcc_final;     /* MCC of the final pose (gd_options.mcc; else 0) */
+  float cmcc_best;      /* MCC of the best pose */
 } gd_replica_result;
 
 /* ---------------------------------------------------------------------- */

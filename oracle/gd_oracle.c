@@ -585,9 +585,14 @@ static void rp_free(rp_ctx *c)
 }
 
 /* canonical energy of atoms (ax, ay, az)[lna] placed with centre cen */
+/* ref: MCC reference contacts [lna][pnp] (original point order) or NULL;
+ * *cmcc gets the Matthews correlation of this pose's contacts (0 without
+ * ref), calcmcc_d.cu:148-213 */
 static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const float *ay,
-                      const float *az, const float cen[3], float out[10])
+                      const float *az, const float cen[3], float out[10], const char *ref,
+                      float *cmcc)
 {
+  int tp = 0, fp = 0, fn = 0, tn = 0;
   const gd_complex_size sz = c->sz;
   const int lna = sz.lna, pnp = sz.pnp;
   const size_t po = (size_t) prt_conf * pnp;
@@ -617,7 +622,14 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         pp.pad0 = 0.0f; pp.pad1 = 0.0f;
         const int is_c2 = prt->c[q] == 2;
         const float pv = is_c2 ? psp_row[prt->seq3r[q]] : 0.0f;
-        gd_pair_terms(d2, lq * prt->ele[q], &pp, &c->sc, is_c2, pv, &acc[j]);
+        const int now = gd_pair_terms(d2, lq * prt->ele[q], &pp, &c->sc, is_c2, pv, &acc[j]);
+        if (ref) {
+          const int was = ref[(size_t) l * pnp + c->perm[po + p]];
+          tp += was && now;
+          fn += was && !now;
+          fp += !was && now;
+          tn += !was && !now;
+        }
       }
     }
     const float hs = bfly(hacc);
@@ -683,6 +695,8 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;
   gd_finalize(sums, (float) lna, ep->a_para, ep->b_para, ep->w, c->term_order, out);
+  if (cmcc)
+    *cmcc = ref ? gd_mcc((float) tp, (float) fn, (float) fp, (float) tn) : 0.0f;
 }
 
 static void rp_place(const rp_ctx *c, int lig_conf, const float m[6], float *ax, float *ay,
@@ -713,7 +727,7 @@ ORC_EXPORT void orc_replay_energy(const gd_ligand *lig, const gd_protein *prt,
   for (int i = 0; i < n_pose; ++i) {
     float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3];
     rp_place(&c, lig_conf[i], move + (size_t) i * 6, ax, ay, az, cen);
-    rp_energy(&c, prt_conf[i], ax, ay, az, cen, out + (size_t) i * 10);
+    rp_energy(&c, prt_conf[i], ax, ay, az, cen, out + (size_t) i * 10, NULL, NULL);
   }
   rp_free(&c);
 }
@@ -748,6 +762,24 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
     }
     memset(&res[r], 0, sizeof res[r]);
   }
+  /* MCC reference contacts (InitRefMatrix_d, calcmcc_d.cu:57-83): replica
+   * 0's ligand at movematrix 0 against replica 0's protein conformer */
+  char *ref = NULL;
+  float *cm = (float *) calloc((size_t) n, sizeof(float));
+  if (opt->mcc) {
+    ref = (char *) malloc((size_t) sz.lna * sz.pnp);
+    const float m0[6] = {0, 0, 0, 0, 0, 0};
+    float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3];
+    rp_place(&c, lc[0], m0, ax, ay, az, cen);
+    const size_t po = (size_t) pc[0] * sz.pnp;
+    for (int a = 0; a < sz.lna; ++a)
+      for (int q = 0; q < sz.pnp; ++q) {
+        const size_t i = po + q;
+        const float d2 = gd_dist2(ax[a] - prt->x[i], ay[a] - prt->y[i], az[a] - prt->z[i]);
+        const float d = d2 * gd_rsqrt(d2);
+        ref[(size_t) a * sz.pnp + q] = (d - ep->pmf0[lig->t[a]][prt->t[i]]) <= 0.0f;
+      }
+  }
   /* initial state (MonteCarlo_Init_d) */
 #pragma omp parallel for schedule(dynamic, 1)
   for (int r = 0; r < n; ++r) {
@@ -757,7 +789,9 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
         m[k] = mc->move_scale[k] * 44.5f + m[k];
     float ax[GD_MAX_LNA], ay[GD_MAX_LNA], az[GD_MAX_LNA], cen[3], e[10];
     rp_place(&c, lc[r], m, ax, ay, az, cen);
-    rp_energy(&c, pc[r], ax, ay, az, cen, e);
+    rp_energy(&c, pc[r], ax, ay, az, cen, e, ref, &cm[r]);
+    res[r].cmcc_final = cm[r];
+    res[r].cmcc_best = cm[r];
     memcpy(res[r].move_final, m, sizeof m);
     memcpy(res[r].move_best, m, sizeof m);
     memcpy(res[r].energy_final, e, sizeof e);
@@ -781,7 +815,7 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
           q->replica.idx_tmp = it[r];
           q->replica.idx_lig = lc[r];
           memcpy(q->energy, R->energy_final, sizeof(float) * 10);
-          q->energy[10] = 0.0f;
+          q->energy[10] = R->cmcc_final;
           memcpy(q->movematrix, R->move_final, sizeof(float) * 6);
           q->step = s;
         }
@@ -793,15 +827,18 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
         for (int k = 0; k < 6; ++k)
           mn[k] = mc->move_scale[k] * (2.0f * gd_u01(w[k]) - 1.0f) + R->move_final[k];
         rp_place(&c, lc[r], mn, ax, ay, az, cen);
-        rp_energy(&c, pc[r], ax, ay, az, cen, en);
+        float cmn = 0.0f;
+        rp_energy(&c, pc[r], ax, ay, az, cen, en, ref, &cmn);
         const float de = en[GD_E_TOTAL] - R->energy_final[GD_E_TOTAL];
         if (gd_u01(w[6]) < gd_exp(de * mb)) {
           R->acs_mc++;
           memcpy(R->move_final, mn, sizeof mn);
           memcpy(R->energy_final, en, sizeof en);
+          R->cmcc_final = cmn;
           if (en[GD_E_TOTAL] < R->energy_best[GD_E_TOTAL]) {
             memcpy(R->move_best, mn, sizeof mn);
             memcpy(R->energy_best, en, sizeof en);
+            R->cmcc_best = cmn;
             R->step_best = s;
           }
         }
@@ -844,6 +881,8 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
   }
   for (int r = 0; r < n; ++r)
     res[r].idx_tmp_final = it[r];
+  free(ref);
+  free(cm);
   free(lc);
   free(pc);
   free(it);

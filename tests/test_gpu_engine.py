@@ -14,7 +14,7 @@ from gpudock_amd import remc, synth
 pytestmark = pytest.mark.gpu
 
 KEYS = ("move_final", "energy_final", "move_best", "energy_best", "step_best",
-        "idx_tmp_final", "acs_mc", "acs_exchg")
+        "idx_tmp_final", "acs_mc", "acs_exchg", "cmcc_final", "cmcc_best")
 
 
 def _same(a, b, what=""):
@@ -35,7 +35,8 @@ def _uneven_jobs(n, seed=0, steps=(30, 90)):
             n_tmp=int(rng.choice([1, 2, 3, 8])), steps=int(rng.integers(*steps)),
             spe=int(rng.choice([1, 5, 10])))
         opt = remc.default_options(seed=7 + i, complex_id=i, exchange=int(i % 4 != 3),
-                                   away=int(i % 5 == 4), term_order=int(i % 3 == 2))
+                                   away=int(i % 5 == 4), term_order=int(i % 3 == 2),
+                                   mcc=int(i % 2 == 1))
         jobs.append(remc.JobSpec(cx, temps, mc, opt))
     return jobs
 

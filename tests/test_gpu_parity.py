@@ -117,7 +117,7 @@ def test_pose_parity_c4_shape():
 
 def _compare_runs(g, r):
     for k in ("move_final", "energy_final", "move_best", "energy_best", "step_best",
-              "idx_tmp_final", "acs_mc", "acs_exchg"):
+              "idx_tmp_final", "acs_mc", "acs_exchg", "cmcc_final", "cmcc_best"):
         np.testing.assert_array_equal(g[k], r[k], err_msg=k)
 
 

@@ -53,3 +53,24 @@ def test_record_window_must_align():
     cx, temps, mc = _cfg(60, 10, 25)
     with pytest.raises(remc.GDError):
         remc.run(cx, temps, mc, remc.default_options(record_stride=1))
+
+
+@pytest.mark.parametrize("n_prt,away", [(1, 0), (2, 0), (2, 1)])
+def test_mcc_equals_replay(tmp_path, n_prt, away):
+    """MCC per pose (one ballot per chunk against the reference mask on the
+    GPU, per-pair counts on the CPU): final/best values and every record."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=n_prt, n_tmp=4, steps=60,
+                                           spe=10)
+    mc.steps_per_dump = 20
+    prefix = str(tmp_path / "m")
+    opt = remc.default_options(seed=21, mcc=1, away=away, record_stride=1,
+                               record_prefix=prefix)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r, rec = O.replay_run_records(cx, temps, mc,
+                                  remc.default_options(seed=21, mcc=1, away=away), 1)
+    for k in ("cmcc_final", "cmcc_best", "energy_final", "move_final", "acs_mc"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+    allrec = np.concatenate([traj.read_gdtraj(f"{prefix}_{w:04d}.gdtraj")[1]
+                             for w in range(3)], axis=1)
+    np.testing.assert_array_equal(allrec, rec)
+    assert np.any(rec["energy"][..., 10] != 0.0)
