@@ -452,9 +452,108 @@ enum {
 /* term_order 0: parameter k applies to file-order term k (the documented
  * order, cpu_mic calcenergy_cpu.C:220-239); 1: parameter k applies to
  * slot k (rem-1.2 calcenergy_d.cu:301-309,349-358). */
+/* ------------------------------------------------------------------ */
+/* Bayesian total (gpudocksm-bayes-1.0 calcenergy_d.cu:315-367 with the   */
+/* pdfs of util_d.cu:220-303 and the fitted parameters of               */
+/* distribution.h:21-77)                                                 */
+/* ------------------------------------------------------------------ */
+#define GD_PDF_FLOOR 1e-8f   /* the reference's floor for a zero pdf */
+
+/* log10 x = ln x / ln 10 */
+GD_HD float gd_log10(float x) { return gd_log(x) * 0x1.bcb7b2p-2f; }
+
+/* a pdf that is 0, infinite or NaN counts as the floor.  The reference
+ * floors exact zeros only (its infinite and NaN cases poison the total);
+ * with this exp a pdf whose exponential factor is below e^-86 is 0. */
+GD_HD float gd_pdf_floor(float v)
+{
+  return (v > 0.0f && v < gd_as_float(GD_INF_BITS)) ? v : GD_PDF_FLOOR;
+}
+
+GD_HD float gd_norm_pdf(float x, float loc, float scale)
+{
+  const float norm = 1.0f / (scale * 0x1.40d932p+1f);     /* sqrt(2 pi) */
+  const float d = x - loc;
+  return gd_pdf_floor(norm * gd_exp(0.0f - d * d / (2.0f * scale * scale)));
+}
+
+GD_HD float gd_cauchy_pdf(float x, float loc, float scale)
+{
+  const float norm = 1.0f / (0x1.921fb6p+1f * scale);      /* pi */
+  const float t = (x - loc) / scale;
+  return gd_pdf_floor(norm * (1.0f / (1.0f + t * t)));
+}
+
+/* e^-z / (1 + e^-z)^2 is even in z: evaluated at |z| so that e^-|z| <= 1
+ * never overflows (the reference's e^-z does for z < -88) */
+GD_HD float gd_logistic_pdf(float x, float loc, float scale)
+{
+  const float z = (x - loc) / scale;
+  const float e = gd_exp(-__builtin_fabsf(z));
+  const float q = 1.0f + e;
+  return gd_pdf_floor((1.0f / scale) * (e / (q * q)));
+}
+
+GD_HD float gd_wald_pdf(float x, float loc, float scale)
+{
+  const float nx = (x - loc) / scale;
+  if (!(nx > 0.0f))
+    return GD_PDF_FLOOR;
+  const float norm = 1.0f / (gd_sqrt(0x1.921fb6p+2f * (nx * nx * nx)) * scale);   /* 2 pi */
+  const float t = nx - 1.0f;
+  return gd_pdf_floor(norm * gd_exp(-(t * t) / (2.0f * nx)));
+}
+
+/* Fitted (location, scale) of each normalised term for high- and
+ * low-quality decoys, file order vdw ele pmf hpc hdb dst psp kde lhm
+ * (distribution.h:21-77). */
+#define GD_BAYES_HIGH_INIT {                                                   \
+    {0.05286683f, 0.29916642f}, {-0.3653806f, 0.00681102f},                   \
+    {-0.0221777f, 0.11847532f}, {-1.0041231f, 0.1151713f},                    \
+    {0.60986808f, 0.30974999f}, {-0.8246522f, 0.05218529f},                   \
+    {-0.2346999f, 0.09871713f}, {-1.0112055f, 0.09882933f},                   \
+    {-0.3377100f, 0.09282947f}}
+#define GD_BAYES_LOW_INIT {                                                    \
+    {0.15818486f, 0.29032471f}, {-0.3647824f, 0.00534225f},                   \
+    {0.04257942f, 0.08801551f}, {-1.0019512f, 0.10312611f},                   \
+    {0.76967708f, 0.11224029f}, {-0.7509618f, 0.08757299f},                   \
+    {-0.2022064f, 0.07854212f}, {-1.004605f, 0.02967548f},                    \
+    {-0.2387023f, 0.12134983f}}
+
+/* Family of each term's pdf, file order (calcenergy_d.cu:319-342 of
+ * bayes-1.0): 0 normal, 1 Cauchy, 2 logistic, 3 Wald.  Low-decoy psp and
+ * hdb use the logistic and normal forms with the parameters fitted for a
+ * Laplace and a logistic, as the reference does. */
+#define GD_BAYES_FAMILY_INIT {0, 1, 2, 3, 0, 2, 2, 3, 2}
+
+GD_HD float gd_bayes_pdf(int family, float x, float loc, float scale)
+{
+  switch (family) {
+  case 0: return gd_norm_pdf(x, loc, scale);
+  case 1: return gd_cauchy_pdf(x, loc, scale);
+  case 2: return gd_logistic_pdf(x, loc, scale);
+  default: return gd_wald_pdf(x, loc, scale);
+  }
+}
+
+/* sum_k log10 p_low(e_k) - sum_k log10 p_high(e_k), k in file order */
+GD_HD float gd_bayes_total(const float norm_file[GD_F_NTERM])
+{
+  const float hi[GD_F_NTERM][2] = GD_BAYES_HIGH_INIT;
+  const float lo[GD_F_NTERM][2] = GD_BAYES_LOW_INIT;
+  const int fam[GD_F_NTERM] = GD_BAYES_FAMILY_INIT;
+  float ph = 0.0f, pl = 0.0f;
+  for (int k = 0; k < GD_F_NTERM; ++k) {
+    ph += gd_log10(gd_bayes_pdf(fam[k], norm_file[k], hi[k][0], hi[k][1]));
+    pl += gd_log10(gd_bayes_pdf(fam[k], norm_file[k], lo[k][0], lo[k][1]));
+  }
+  return pl - ph;
+}
+
+/* combine: 0 = weighted sum (calcenergy_d.cu:349-358), 1 = Bayesian total */
 GD_HD void gd_finalize(const float sums[GD_F_NTERM], float lna_f,
                        const float a[10], const float b[10], const float w[10],
-                       int term_order, float out[GD_E_N])
+                       int term_order, int combine, float out[GD_E_N])
 {
   float raw[GD_F_NTERM];
   raw[GD_F_VDW] = sums[GD_F_VDW] / lna_f;
@@ -483,6 +582,12 @@ GD_HD void gd_finalize(const float sums[GD_F_NTERM], float lna_f,
       slot[i] = v;
       total = gd_fma(w[i], v, total);
     }
+  }
+  if (combine == 1) {
+    float nf[GD_F_NTERM];
+    for (int k = 0; k < GD_F_NTERM; ++k)
+      nf[k] = slot[f2s[k]];
+    total = gd_bayes_total(nf);
   }
   slot[GD_E_TOTAL] = total;
   for (int i = 0; i < GD_E_N; ++i)

@@ -40,6 +40,7 @@ typedef struct gd_pack {
   int n_res;             /* PSP residues */
   int psp_ld;            /* PSP row length: n_res + 1 (last column all zero) */
   int term_order;
+  int combine;           /* total: 0 weighted sum, 1 Bayesian */
   int pnp_pad;           /* pnp rounded up to a multiple of 64 */
 
   /* run control */

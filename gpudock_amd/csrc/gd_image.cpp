@@ -43,6 +43,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   P.lna = sz.lna; P.pnp = sz.pnp; P.pnk = sz.pnk; P.pos = sz.pos;
   P.n_res = psp->n_res;
   P.term_order = o.term_order;
+  P.combine = o.combine;
   P.steps_total = mc ? mc->steps_total : 0;
   P.spe = mc ? mc->steps_per_exchange : 1;
   P.exchange = o.exchange;

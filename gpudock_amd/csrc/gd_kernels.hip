@@ -240,7 +240,7 @@ __device__ __forceinline__ void gd_energy_wave(
   sums[GD_F_PSP] = gd_wave_sum(acc.psp);
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;
-  gd_finalize(sums, (float) lna, P.a_para, P.b_para, P.w, P.term_order, out);
+  gd_finalize(sums, (float) lna, P.a_para, P.b_para, P.w, P.term_order, P.combine, out);
   *cmcc = 0.0f;
   if (mref) {
     const int tn = lna * P.pnp - tp - fp - fn;   /* padding points are never contacts */
@@ -613,6 +613,46 @@ gd_score_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ t
 /* FP64 re-score: the reference formulas (calcenergy_d.cu:69-309) in double, */
 /* one wavefront per pose, reading the pack straight from HBM.              */
 /* ------------------------------------------------------------------------ */
+/* The Bayesian total in FP64: the same fitted pdfs and the same floor (1e-8
+ * for a zero or non-finite pdf or an exponential factor below e^-86). */
+__device__ double gd_pdf64(int family, double x, double loc, double scale)
+{
+  const double pi = 3.14159265358979323846;
+  double v, arg = 0.0;
+  if (family == 0) {
+    const double d = x - loc;
+    arg = -d * d / (2.0 * scale * scale);
+    v = exp(arg) / (scale * sqrt(2.0 * pi));
+  } else if (family == 1) {
+    const double t = (x - loc) / scale;
+    v = 1.0 / (pi * scale * (1.0 + t * t));
+  } else if (family == 2) {
+    arg = -fabs((x - loc) / scale);
+    const double e = exp(arg);
+    v = e / ((1.0 + e) * (1.0 + e)) / scale;
+  } else {
+    const double nx = (x - loc) / scale;
+    if (!(nx > 0.0))
+      return 1e-8;
+    arg = -(nx - 1.0) * (nx - 1.0) / (2.0 * nx);
+    v = exp(arg) / (sqrt(2.0 * pi * nx * nx * nx) * scale);
+  }
+  return (v > 0.0 && v < HUGE_VAL && arg >= -86.0) ? v : 1e-8;
+}
+
+__device__ double gd_bayes_total64(const double nf[9])
+{
+  const float hi[9][2] = GD_BAYES_HIGH_INIT;
+  const float lo[9][2] = GD_BAYES_LOW_INIT;
+  const int fam[9] = GD_BAYES_FAMILY_INIT;
+  double ph = 0.0, pl = 0.0;
+  for (int k = 0; k < 9; ++k) {
+    ph += log10(gd_pdf64(fam[k], nf[k], hi[k][0], hi[k][1]));
+    pl += log10(gd_pdf64(fam[k], nf[k], lo[k][0], lo[k][1]));
+  }
+  return pl - ph;
+}
+
 __device__ void gd_energy64_wave(const gd_pack &P, int lconf, int pconf, const float *move,
                                  double *__restrict__ out)
 {
@@ -749,6 +789,12 @@ __device__ void gd_energy64_wave(const gd_pack &P, int lconf, int pconf, const f
         slot[k] = v;
         total += (double) P.w[k] * v;
       }
+    }
+    if (P.combine == 1) {
+      double nf[9];
+      for (int k = 0; k < 9; ++k)
+        nf[k] = slot[f2s[k]];
+      total = gd_bayes_total64(nf);
     }
     slot[9] = total;
     for (int k = 0; k < 10; ++k)

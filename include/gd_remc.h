@@ -58,6 +58,13 @@ enum gd_energy_slot {
   GD_E_HPC_SLOT, GD_E_KDE_SLOT, GD_E_LHM_SLOT, GD_E_DST_SLOT, GD_E_TOTAL_SLOT
 };
 
+enum gd_combine {
+  GD_COMBINE_LINEAR = 0,  /* sum of w_k e_k (calcenergy_d.cu:349-358) */
+  GD_COMBINE_BAYES = 1    /* IS_BAYE: sum log10 p_low(e_k) - sum log10 p_high(e_k)
+                             over fitted per-term pdfs (gpudocksm-bayes-1.0
+                             calcenergy_d.cu:315-367, distribution.h:21-77) */
+};
+
 enum gd_term_order {
   GD_TERM_ORDER_FILE = 0, /* parameter k applies to file-order term k */
   GD_TERM_ORDER_SLOT = 1  /* rem-1.2 behaviour: parameter k applies to slot k
@@ -203,6 +210,7 @@ typedef struct gd_options {
   int mcc;              /* IS_CALCU_MCC (toggle.h:18): Matthews correlation of
                            each pose's contacts against the contacts of
                            replica 0's ligand at its start pose */
+  int combine;          /* enum gd_combine: the total energy */
 } gd_options;
 
 /* LigRecordSingleStep (dock.h:487-494): the state of one replica slot

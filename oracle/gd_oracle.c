@@ -511,6 +511,7 @@ typedef struct {
   float *kx, *ky, *kz;
   /* canonical point order per protein conformer (stable by sort key) */
   int *perm;
+  int combine;
 } rp_ctx;
 
 static const gd_protein *g_sort_prt;
@@ -548,7 +549,7 @@ static void rp_init(rp_ctx *c, const gd_ligand *lig, const gd_protein *prt,
                     const gd_enepara *ep, gd_complex_size sz, int term_order)
 {
   c->lig = lig; c->prt = prt; c->psp = psp; c->kde = kde; c->mcs = mcs; c->ep = ep;
-  c->sz = sz; c->term_order = term_order;
+  c->sz = sz; c->term_order = term_order; c->combine = 0;
   gd_pair_scalars_init(&c->sc, ep->lj0, ep->lj1, ep->el0, ep->el1, ep->a1, ep->b1);
   const int n = sz.pnk > 0 ? sz.pnk : 1;
   c->kx = (float *) malloc(sizeof(float) * 3 * n);
@@ -694,7 +695,8 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
   sums[GD_F_PSP] = bfly(t);
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;
-  gd_finalize(sums, (float) lna, ep->a_para, ep->b_para, ep->w, c->term_order, out);
+  gd_finalize(sums, (float) lna, ep->a_para, ep->b_para, ep->w, c->term_order, c->combine,
+              out);
   if (cmcc)
     *cmcc = ref ? gd_mcc((float) tp, (float) fn, (float) fp, (float) tn) : 0.0f;
 }
@@ -749,6 +751,7 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
 {
   rp_ctx c;
   rp_init(&c, lig, prt, psp, kde, mcs, ep, sz, opt->term_order);
+  c.combine = opt->combine;
   const int n = sz.n_rep;
   int *lc = (int *) malloc(sizeof(int) * n), *pc = (int *) malloc(sizeof(int) * n);
   int *it = (int *) malloc(sizeof(int) * n);
@@ -909,6 +912,19 @@ ORC_EXPORT void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, u
 {
   const gd_u32x4 r = gd_philox4x32_10(c0, c1, c2, c3, k0, k1);
   memcpy(out, r.v, sizeof r.v);
+}
+
+/* canonical Bayesian pdfs and total, for the closed-form tests */
+ORC_EXPORT void orc_bayes_pdf(int family, int n, const float *x, float loc, float scale,
+                              float *y)
+{
+  for (int i = 0; i < n; ++i)
+    y[i] = gd_bayes_pdf(family, x[i], loc, scale);
+}
+
+ORC_EXPORT float orc_bayes_total(const float norm_file[9])
+{
+  return gd_bayes_total(norm_file);
 }
 
 ORC_EXPORT void orc_canon_math(int fn, int n, const float *x, float *y, float *y2)

@@ -40,6 +40,9 @@ def lib():
         L.orc_ref_run.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int,
                                         C.c_int, IP, IP, IP, FP, FP]
+        L.orc_bayes_pdf.argtypes = [C.c_int, C.c_int, FP, C.c_float, C.c_float, FP]
+        L.orc_bayes_total.argtypes = [FP]
+        L.orc_bayes_total.restype = C.c_float
         L.orc_philox.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]
         L.orc_canon_math.argtypes = [C.c_int, C.c_int, FP, FP, FP]
         _L = L
@@ -133,3 +136,15 @@ def canon_math(fn, x):
     y2 = np.zeros_like(x)
     lib().orc_canon_math(fn, len(x), _fp(x), _fp(y), _fp(y2))
     return (y, y2) if fn == 2 else y
+
+
+def bayes_pdf(family, x, loc, scale):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x)
+    lib().orc_bayes_pdf(family, len(x), _fp(x), loc, scale, _fp(y))
+    return y
+
+
+def bayes_total(norm_file):
+    v = np.ascontiguousarray(norm_file, np.float32)
+    return lib().orc_bayes_total(_fp(v))

@@ -188,3 +188,20 @@ def test_acceptance_rises_with_temperature():
     acs = g["acs_mc"].reshape(n_prt, n_tmp, n_lig).sum(axis=(0, 2))
     assert acs[-1] > acs[0]
     assert np.corrcoef(np.arange(n_tmp), acs)[0, 1] > 0.8
+
+
+@pytest.mark.parametrize("term_order", [0, 1])
+def test_bayes_total_run_matches_replay(term_order):
+    """IS_BAYE total: every decision equals the replay's; the FP64 re-score
+    of each best pose carries the float64 Bayesian total of its FP64
+    normalised terms."""
+    import test_bayes as TB
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=2, n_tmp=4, steps=150)
+    opt = remc.default_options(seed=6, combine=1, term_order=term_order)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    slot_of_file = [0, 1, 2, 5, 4, 8, 3, 6, 7]
+    for e64 in g["energy_best_fp64"]:
+        ref = TB.bayes64(e64[slot_of_file])
+        assert abs(e64[9] - ref) <= 1e-9 * max(1.0, abs(ref))
