@@ -3,8 +3,8 @@
 The library holds the HIP kernels (csrc/gd_kernels.hip), the host image of a
 complex (csrc/gd_image.cpp), the engine and the C ABI entry points
 (csrc/gd_engine.cpp), the serving loop (csrc/gd_server.cpp), shared host utilities
-(csrc/gd_host.cpp) and the host preparation / parameter parsing
-(csrc/gd_prep.cpp).  All device code is compiled with contraction off
+(csrc/gd_host.cpp), the host preparation / parameter parsing
+(csrc/gd_prep.cpp) and the input-file loaders (csrc/gd_load.cpp).  All device code is compiled with contraction off
 (the canonical arithmetic relies on it) and without packed FP32 ops.
 """
 import os
@@ -35,7 +35,8 @@ def build(verbose=True):
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src, dev in (("gd_kernels.hip", True), ("gd_image.cpp", False), ("gd_engine.cpp", False),
-                     ("gd_server.cpp", False), ("gd_host.cpp", False), ("gd_prep.cpp", False)):
+                     ("gd_server.cpp", False), ("gd_host.cpp", False), ("gd_prep.cpp", False),
+                     ("gd_load.cpp", False)):
         obj = os.path.join(objdir, src + ".o")
         cmd = [HIPCC] + COMMON + (DEVICE if dev else [f"--offload-arch={ARCH}"]) + [
             "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]

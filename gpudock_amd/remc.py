@@ -198,6 +198,21 @@ class ServerStats(C.Structure):
                 ("cycles", C.c_longlong), ("rebuilds", C.c_longlong)]
 
 
+class InputFiles(C.Structure):
+    _fields_ = [("ligand_sdf", C.c_char_p), ("molid", C.c_char_p),
+                ("protein_pdb", C.c_char_p), ("lhm_ff", C.c_char_p),
+                ("ligand_id", C.c_char_p), ("flags", C.c_int)]
+
+
+LOAD_PLATE_ALL_MODELS = 1
+
+
+class InputsView(C.Structure):
+    _fields_ = [("size", ComplexSize), ("lig", Ligand), ("prt", Protein), ("psp", Psp),
+                ("kde", Kde), ("mcs", Mcs), ("lens_rmsd", FP), ("prt_d", IP),
+                ("prt_r", IP), ("ligand_name", C.c_char_p)]
+
+
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 
 _LIB = None
@@ -261,6 +276,10 @@ SYMBOLS = {
     "gd_optimize_enepara": (C.c_int, [C.POINTER(EnePara0), C.POINTER(EnePara)]),
     "gd_set_temperature": (C.c_int, [C.POINTER(Temp), C.c_int, C.c_float, C.c_float]),
     "gd_set_replica": (C.c_int, [C.POINTER(Replica), ComplexSize]),
+    "gd_load_inputs": (C.c_int, [C.POINTER(InputFiles), C.POINTER(EnePara0),
+                                 C.POINTER(C.c_void_p)]),
+    "gd_inputs_get": (C.c_int, [C.c_void_p, C.POINTER(InputsView)]),
+    "gd_inputs_free": (None, [C.c_void_p]),
     "gd_load_enepara": (C.c_int, [C.c_char_p, C.POINTER(EnePara0)]),
     "gd_load_weight": (C.c_int, [C.c_char_p, C.POINTER(EnePara0)]),
     "gd_load_norpara": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(EnePara0)]),
@@ -415,6 +434,74 @@ def load_enepara0(ff_path, weight_path=None, nor_a=None, nor_b=None):
     if nor_a and nor_b:
         check(lib().gd_load_norpara(nor_a.encode(), nor_b.encode(), C.byref(ep0)))
     return ep0
+
+
+def _copy(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def load_complex(ligand_sdf, protein_pdb, lhm_ff, ligand_id, enepara0, molid="MOLID",
+                 flags=0):
+    """Read one complex from its SD, PDB and LHM files and prepare it.
+
+    The sequence of the reference's dock.C:40-42,70-74: loadLigand,
+    loadProtein, loadLHM (load.C:99-1126), then OptimizeLigand,
+    OptimizeProtein, OptimizePsp, OptimizeKde and OptimizeMcs
+    (util.C:199-402), all in libgdremc (gd_load_inputs).  enepara0 is the
+    EnePara0 of load_enepara0(); the returned Complex carries
+    optimize_enepara(enepara0).  Extra attributes: lens_rmsd [n_lig],
+    prt_d and prt_r [n_prt, pnp] (residue code and index of each point),
+    ligand_name.  flags: LOAD_PLATE_ALL_MODELS builds the peptide-plate
+    points of every PDB model from N, C and O; by default models from the
+    third on follow the reference (load.C:886), which takes C and O of
+    residue serial 0 there.
+    """
+    files = InputFiles(os.fsencode(ligand_sdf), molid.encode() if molid else None,
+                       os.fsencode(protein_pdb), os.fsencode(lhm_ff),
+                       ligand_id.encode() if ligand_id is not None else None, flags)
+    h = C.c_void_p()
+    check(lib().gd_load_inputs(C.byref(files), C.byref(enepara0), C.byref(h)))
+    try:
+        v = InputsView()
+        check(lib().gd_inputs_get(h, C.byref(v)))
+        sz = v.size
+        n_lig, lna, n_prt, pnp = sz.n_lig, sz.lna, sz.n_prt, sz.pnp
+        nl, npt, nf = n_lig * lna, n_prt * pnp, v.mcs.n_field
+        lig_xyz = np.stack([_copy(v.lig.x, nl, np.float32), _copy(v.lig.y, nl, np.float32),
+                            _copy(v.lig.z, nl, np.float32)], -1).reshape(n_lig, lna, 3)
+        prt_xyz = np.stack([_copy(v.prt.x, npt, np.float32), _copy(v.prt.y, npt, np.float32),
+                            _copy(v.prt.z, npt, np.float32)], -1).reshape(n_prt, pnp, 3)
+        pnk, pos = sz.pnk, sz.pos
+        kde_xyz = np.stack([_copy(v.kde.x, pnk, np.float32), _copy(v.kde.y, pnk, np.float32),
+                            _copy(v.kde.z, pnk, np.float32)], -1).reshape(pnk, 3)
+        mcs_xyz = np.stack([_copy(v.mcs.x, pos * nf, np.float32),
+                            _copy(v.mcs.y, pos * nf, np.float32),
+                            _copy(v.mcs.z, pos * nf, np.float32)], -1).reshape(pos, nf, 3)
+        shp = (n_prt, pnp)
+        cx = Complex(
+            lig_xyz=lig_xyz, lig_t=_copy(v.lig.t, lna, np.int32),
+            lig_c=_copy(v.lig.c, lna, np.float32), lig_n=_copy(v.lig.n, lna, np.int32),
+            lig_center=np.array(v.lig.center[:], np.float32), prt_xyz=prt_xyz,
+            prt_t=_copy(v.prt.t, npt, np.int32).reshape(shp),
+            prt_c=_copy(v.prt.c, npt, np.int32).reshape(shp),
+            prt_ele=_copy(v.prt.ele, npt, np.float32).reshape(shp),
+            prt_seq3r=_copy(v.prt.seq3r, npt, np.int32).reshape(shp),
+            prt_hpcflag=_copy(v.prt.c0_and_d12_or_c2, npt, np.int32).reshape(shp),
+            prt_hpp=_copy(v.prt.hpp, npt, np.float32).reshape(shp),
+            pocket_center=np.array(v.prt.pocket_center[:], np.float32),
+            psp=_copy(v.psp.psp, MAXTP2 * v.psp.n_res, np.float32).reshape(MAXTP2,
+                                                                             v.psp.n_res),
+            kde_xyz=kde_xyz, kde_t=_copy(v.kde.t, pnk, np.int32), mcs_xyz=mcs_xyz,
+            mcs_tcc=_copy(v.mcs.tcc, pos, np.float32), enepara=optimize_enepara(enepara0))
+        cx.lens_rmsd = _copy(v.lens_rmsd, n_lig, np.float32)
+        cx.prt_d = _copy(v.prt_d, npt, np.int32).reshape(shp)
+        cx.prt_r = _copy(v.prt_r, npt, np.int32).reshape(shp)
+        cx.ligand_name = (v.ligand_name or b"").decode(errors="replace")
+        return cx
+    finally:
+        lib().gd_inputs_free(h)
 
 
 def optimize_enepara(ep0):

@@ -518,6 +518,56 @@ int gd_set_temperature(gd_temp *temp, int num_temp, float floor_temp,
 int gd_set_replica(gd_replica *replica, gd_complex_size size);
 
 /* ---------------------------------------------------------------------- */
+/* Input files of one complex (load.C:99-1126) and their preparation        */
+/* (util.C:199-402)                                                         */
+/* ---------------------------------------------------------------------- */
+/* The per-complex files of a docking run: InputFiles (dock.h:62-75), set by
+ * the -l, -p, -s and -id options (util.C:28-195). */
+typedef struct gd_input_files {
+  const char *ligand_sdf;   /* SD file: atom block, OB_ATOM_TYPES,
+                               OB_ATOMIC_CHARGES, ENSEMBLE_COORDS */
+  const char *molid;        /* tag of the ligand-name property; NULL = "MOLID" */
+  const char *protein_pdb;  /* PDB file, one MODEL ... ENDMDL per conformer */
+  const char *lhm_ff;       /* LHM file: KDE, PSP and MCS lines */
+  const char *ligand_id;    /* the MCS lines of this id are read (-id) */
+  int flags;                /* GD_LOAD_* */
+} gd_input_files;
+
+enum {
+  /* Peptide-plate points of every PDB model average the N of their residue
+   * and the C and O of the previous one.  Without it, models from the third
+   * on take C and O of residue serial 0 instead, as the reference's
+   * loadProtein does (load.C:886 reads an unfilled conformer). */
+  GD_LOAD_PLATE_ALL_MODELS = 1
+};
+
+typedef struct gd_inputs gd_inputs;   /* host-memory prepared complex */
+
+/* What Run() takes about one complex, in memory owned by the gd_inputs
+ * object (valid until gd_inputs_free). */
+typedef struct gd_inputs_view {
+  gd_complex_size size;     /* n_lig n_prt lna pnp pnk pos; n_tmp = n_rep = 0 */
+  gd_ligand lig;
+  gd_protein prt;
+  gd_psp psp;
+  gd_kde kde;
+  gd_mcs mcs;               /* n_field = lna + 2 */
+  const float *lens_rmsd;   /* [n_lig] RMSD to the native pose (Ligand0.lens_rmsd) */
+  const int *prt_d;         /* [n_prt * pnp] residue code of each point (Protein0.d) */
+  const int *prt_r;         /* [n_prt * pnp] residue index of each point (Protein0.r) */
+  const char *ligand_name;  /* the line after the molid tag; "" if absent */
+} gd_inputs_view;
+
+/* loadLigand + loadProtein + loadLHM (load.C:99-419, 595-983, 1010-1126),
+ * then OptimizeLigand/OptimizeProtein/OptimizePsp/OptimizeKde/OptimizeMcs
+ * (util.C:199-402), the sequence of dock.C:40-42,70-74.  enepara0 supplies
+ * the ELE and HPP tables of the protein points (util.C:263-273). */
+int gd_load_inputs(const gd_input_files *files, const gd_enepara0 *enepara0,
+                   gd_inputs **out);
+int gd_inputs_get(const gd_inputs *in, gd_inputs_view *view);
+void gd_inputs_free(gd_inputs *in);
+
+/* ---------------------------------------------------------------------- */
 /* Parameter files (load.C:1138-1419)                                       */
 /* ---------------------------------------------------------------------- */
 /* loadEnePara (load.C:1138-1324): gpudocksm.ff; zero-fills first. */

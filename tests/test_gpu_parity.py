@@ -205,3 +205,19 @@ def test_bayes_total_run_matches_replay(term_order):
     for e64 in g["energy_best_fp64"]:
         ref = TB.bayes64(e64[slot_of_file])
         assert abs(e64[9] - ref) <= 1e-9 * max(1.0, abs(ref))
+
+
+def test_g1_and_run_from_input_files():
+    """1a07C1 read by the library's own loaders from its SD/PDB/LHM files:
+    the published initial energy line on the GPU, then a short REMC run over
+    all 21 ligand x 3 protein conformers that equals the replay."""
+    from test_loaders import load_1a07c1_files
+    cx = load_1a07c1_files()
+    e, e64 = remc.score_poses(cx, [0], [0], np.zeros((1, 6)))
+    np.testing.assert_allclose(e[0], fixtures.g1_slots(), rtol=0, atol=5.001e-5)
+    temps = remc.temperatures(2, 0.044, 0.036)
+    mc = remc.mcpara(60, 10, [0.02, 0.02, 0.02, 0.08, 0.08, 0.08])
+    opt = remc.default_options(seed=5, mcc=1)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
