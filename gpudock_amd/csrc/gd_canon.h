@@ -301,11 +301,9 @@ typedef struct __attribute__((aligned(16))) {
 #define GD_ROW_STRIDE 32
 #define GD_PAD_TYPE 30   /* point type of padding points: all-zero entry */
 
-/* Packed per-point flags. */
-#define GD_PT_TYPE_MASK 0x1fu
-#define GD_PT_C2 0x20u          /* point class == 2 (PSP applies) */
-#define GD_PT_HPC 0x40u         /* (class 0 and residue GLY) or class 2 */
-#define GD_PT_SEQ_SHIFT 8
+/* A pocket point's fourth word holds its type as the byte offset of its
+ * entry in a table row: type << GD_PT_TYPE_SHIFT (sizeof(gd_pair_param)). */
+#define GD_PT_TYPE_SHIFT 5
 
 /* Scalars shared by every pair.  The short-range electrostatics
  * el0 + a1 s^2 + b1 s^3, s = el1 d (calcenergy_d.cu:99-105) is evaluated
@@ -356,8 +354,10 @@ GD_HD float gd_hpc_pair(float d2, float hpp)
 /* The five directly summed pair terms.  psp_val is the PSP table value
  * psp[ligand type][residue] of the point (any value for points that are
  * not class 2); it is added for class-2 points inside the PMF radius
- * (calcenergy_d.cu:123-126). */
-GD_HD int gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
+ * (calcenergy_d.cu:123-126).  acc->ele collects ele_point * g for the
+ * current ligand atom; the caller multiplies that sum by the atom's charge
+ * once per atom (q sum(ele g) = sum(q ele g), calcenergy_d.cu:99-105). */
+GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
                         const gd_pair_scalars *sc, int is_c2, float psp_val,
                         gd_pair_acc *acc)
 {
@@ -377,7 +377,7 @@ GD_HD int gd_pair_terms(float d2, float lq_ele, const gd_pair_param *pp,
   /* distance-dependent electrostatics */
   const float g = (d2 < sc->dcut2) ? gd_fma(d2, gd_fma(d, sc->eb, sc->ea), sc->el0)
                                    : r1 * sc->inv_el1;
-  acc->ele = gd_fma(lq_ele, g, acc->ele);
+  acc->ele = gd_fma(ele_pt, g, acc->ele);
 
   /* PMF contact denominator */
   const float dm = d - pp->pmf0;
@@ -423,13 +423,16 @@ GD_HD uint32_t gd_hdb_point_mask(const float *hdb0, int n_lt, int n_pt)
 }
 
 /* Order of the pocket points inside the canonical sums: stable by
- * (H-bond-capable type first, point type, class-2 flag, hydrophobic flag),
- * so that the lanes of one wavefront mostly see one point type (coherent
- * branches, broadcast table reads) and the H-bond branch is skipped by every
- * wavefront past the H-bond-capable points. */
+ * (hydrophobic class first, then H-bond-capable type first, point type,
+ * class-2 flag), so that the hydrophobic points fill the leading
+ * wavefront-wide chunks (the rest skip that term), the lanes of one
+ * wavefront mostly see one point type (coherent branches, broadcast table
+ * reads) and the H-bond branch is skipped by wavefronts outside the
+ * H-bond-capable groups. */
 GD_HD int gd_point_sort_key(int type, int is_c2, int hpc, uint32_t hdb_mask)
 {
-  return (((hdb_mask >> type) & 1u) ? 0 : 128) + type * 4 + (is_c2 ? 2 : 0) + (hpc ? 1 : 0);
+  return (hpc ? 0 : 256) + (((hdb_mask >> type) & 1u) ? 0 : 128) + type * 4 +
+         (is_c2 ? 2 : 0) + (hpc ? 1 : 0);
 }
 
 /* Raw (averaged, not normalised) terms in the parameter-file order used by

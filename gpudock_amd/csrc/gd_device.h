@@ -31,6 +31,14 @@ typedef struct gd_state {
   float *cmcc_best; /* [n_rep] cmcc of the best pose */
 } gd_state;
 
+/* A pocket point as staged in LDS: one 24-byte record, read from one
+ * address per point. */
+typedef struct __attribute__((aligned(8))) gd_lds_point {
+  float2 xy;
+  float2 zw;    /* z, bits(type << GD_PT_TYPE_SHIFT) */
+  float2 eh;    /* ele, hpp (0 outside the hydrophobic class) */
+} gd_lds_point;
+
 /* One complex. */
 typedef struct gd_pack {
   /* sizes */
@@ -38,7 +46,7 @@ typedef struct gd_pack {
   int lna, pnp, pnk, pos;
   int n_rows;            /* distinct ligand atom types = table rows */
   int n_res;             /* PSP residues */
-  int psp_ld;            /* PSP row length: n_res + 1 (last column all zero) */
+  int pad_psp;
   int term_order;
   int combine;           /* total: 0 weighted sum, 1 Bayesian */
   int pnp_pad;           /* pnp rounded up to a multiple of 64 */
@@ -58,7 +66,7 @@ typedef struct gd_pack {
 
   /* pocket, per protein conformer: [n_prt][pnp_pad]; points pnp..pnp_pad-1
    * are padding that contributes exactly zero to every term */
-  const float4 *pt_pos;  /* x, y, z, bits(type | flags | seq << 8) */
+  const float4 *pt_pos;  /* x, y, z, bits(type << GD_PT_TYPE_SHIFT) */
   const float2 *pt_eh;   /* ele, hpp (0 outside the hydrophobic class) */
   /* pair table, [n_rows][GD_ROW_STRIDE] */
   const gd_pair_param *tab;
@@ -74,9 +82,12 @@ typedef struct gd_pack {
   /* position restraints: targets per (m, atom) [pos][lna] (x, y, z, valid) */
   const float4 *mcs;
   const float *mcs_tcc;  /* [pos] */
-  /* PSP table [GD_MAXTP2][psp_ld]; point bits hold the column (the zero
-   * column n_res for points outside class 2) */
-  const float *psp;
+  /* PSP value per (protein conformer, table row, point):
+   * [n_prt][n_rows][pnp_pad], 0 outside class 2 and for padding */
+  const float *psp_pt;
+  /* [n_prt] chunks of 64 points, from the first, that hold every point of
+   * the hydrophobic class (the canonical order puts them first) */
+  const int *hpc_chunks;
   /* per ligand type (24 entries each) */
   const float *hpl0, *hpl1, *hpl2;
   const float *inv_hpl1; /* 1.0f / hpl1 */

@@ -104,7 +104,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   P.pnp_pad = pnp_pad;
   const size_t np = (size_t) sz.n_prt * pnp_pad;
   {
-    const uint32_t pbits = GD_PAD_TYPE;
+    const uint32_t pbits = (uint32_t) GD_PAD_TYPE << GD_PT_TYPE_SHIFT;
     float pw;
     memcpy(&pw, &pbits, 4);
     I.pt_pos.assign(np, make_float4(1.0e3f, 1.0e3f, 1.0e3f, pw));
@@ -113,6 +113,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   const uint32_t hdb_mask = gd_hdb_point_mask(&ep->hdb0[0][0], GD_MAXTP2, GD_MAXTP1);
   std::vector<int> perm(sz.pnp);
   std::vector<int> perms((size_t) sz.n_prt * sz.pnp);   /* sorted slot -> point */
+  I.hpc_chunks.assign(sz.n_prt, 0);
   for (int cf = 0; cf < sz.n_prt; ++cf) {
     const size_t base = (size_t) cf * sz.pnp;
     const size_t obase = (size_t) cf * pnp_pad;
@@ -126,16 +127,13 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     });
     for (int j = 0; j < sz.pnp; ++j)
       perms[(size_t) cf * sz.pnp + j] = perm[j];
+    int n_hpc = 0;
+    for (int j = 0; j < sz.pnp; ++j)
+      n_hpc += prt->c0_and_d12_or_c2[base + j] != 0;
+    I.hpc_chunks[cf] = (n_hpc + GD_WAVE - 1) / GD_WAVE;
     for (int j = 0; j < sz.pnp; ++j) {
       const size_t i = base + perm[j];
-      uint32_t bits = (uint32_t) prt->t[i] & GD_PT_TYPE_MASK;
-      if (prt->c[i] == 2)
-        bits |= GD_PT_C2;
-      if (prt->c0_and_d12_or_c2[i])
-        bits |= GD_PT_HPC;
-      /* PSP column: the residue for class-2 points, else the all-zero
-       * column n_res, so the energy loop gathers without a class test */
-      bits |= (uint32_t) (prt->c[i] == 2 ? prt->seq3r[i] : psp->n_res) << GD_PT_SEQ_SHIFT;
+      const uint32_t bits = (uint32_t) prt->t[i] << GD_PT_TYPE_SHIFT;
       float w;
       memcpy(&w, &bits, 4);
       I.pt_pos[obase + j] = make_float4(prt->x[i], prt->y[i], prt->z[i], w);
@@ -180,12 +178,21 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     }
   }
 
-  /* PSP [24][n_res + 1]: one extra all-zero column (see the point bits) */
-  P.psp_ld = psp->n_res + 1;
-  I.psp.assign((size_t) GD_MAXTP2 * P.psp_ld, 0.0f);
-  for (int t = 0; t < GD_MAXTP2; ++t)
-    memcpy(&I.psp[(size_t) t * P.psp_ld], psp->psp + (size_t) t * psp->n_res,
-           (size_t) psp->n_res * sizeof(float));
+  /* PSP value of every (protein conformer, table row, sorted point):
+   * psp[row type][seq3r] for class-2 points, 0 for the others and for
+   * padding, so the energy loop reads one contiguous row per ligand atom
+   * and adds it under the PMF-radius test alone */
+  I.psp_pt.assign((size_t) sz.n_prt * P.n_rows * pnp_pad, 0.0f);
+  for (int cf = 0; cf < sz.n_prt; ++cf)
+    for (int r = 0; r < P.n_rows; ++r) {
+      float *dst = &I.psp_pt[((size_t) cf * P.n_rows + r) * pnp_pad];
+      const float *src = psp->psp + (size_t) row_type[r] * psp->n_res;
+      for (int j = 0; j < sz.pnp; ++j) {
+        const size_t i = (size_t) cf * sz.pnp + perms[(size_t) cf * sz.pnp + j];
+        if (prt->c[i] == 2)
+          dst[j] = src[prt->seq3r[i]];
+      }
+    }
   I.hpl.resize(4 * GD_MAXTP2);
   for (int t = 0; t < GD_MAXTP2; ++t) {
     I.hpl[t] = ep->hpl0[t];
@@ -268,8 +275,9 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
 
   /* LDS carve-up */
   auto al16 = [](int v) { return (v + 15) & ~15; };
-  P.off_eh = al16(pnp_pad * 16);
-  P.off_tab = al16(P.off_eh + pnp_pad * 8);
+  /* pocket points staged as 24-byte records (position + type, ele + hpp) */
+  P.off_eh = 0;
+  P.off_tab = al16(pnp_pad * (int) sizeof(gd_lds_point));
   P.off_kde = al16(P.off_tab + P.n_rows * GD_ROW_STRIDE * (int) sizeof(gd_pair_param));
   const int with_kde = al16(P.off_kde + P.pnk * 16);
   const int reserve = 16 * GD_WSLOT_FLOATS * 4 + 64;   /* wave slots of up to 16 waves */
@@ -308,7 +316,8 @@ GdBlock gd_layout_block(const GdImage &I)
   b.kde_inv = take(vbytes(I.kde_inv));
   b.mcs = take(vbytes(I.mcs));
   b.tcc = take(vbytes(I.tcc));
-  b.psp = take(vbytes(I.psp));
+  b.psp_pt = take(vbytes(I.psp_pt));
+  b.hpc_chunks = take(vbytes(I.hpc_chunks));
   b.hpl = take(vbytes(I.hpl));
   b.mb = take(vbytes(I.mb));
   b.rep_lig = take(n * 4);
@@ -360,7 +369,8 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   put(b.kde_inv, I.kde_inv.data(), vbytes(I.kde_inv));
   put(b.mcs, I.mcs.data(), vbytes(I.mcs));
   put(b.tcc, I.tcc.data(), vbytes(I.tcc));
-  put(b.psp, I.psp.data(), vbytes(I.psp));
+  put(b.psp_pt, I.psp_pt.data(), vbytes(I.psp_pt));
+  put(b.hpc_chunks, I.hpc_chunks.data(), vbytes(I.hpc_chunks));
   put(b.hpl, I.hpl.data(), vbytes(I.hpl));
   put(b.mb, I.mb.data(), vbytes(I.mb));
   put(b.rep_lig, I.rep_lig.data(), vbytes(I.rep_lig));
@@ -380,7 +390,8 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.kde_inv_n = (const float *) (dev + b.kde_inv);
   P.mcs = (const float4 *) (dev + b.mcs);
   P.mcs_tcc = (const float *) (dev + b.tcc);
-  P.psp = (const float *) (dev + b.psp);
+  P.psp_pt = (const float *) (dev + b.psp_pt);
+  P.hpc_chunks = (const int *) (dev + b.hpc_chunks);
   P.hpl0 = (const float *) (dev + b.hpl);
   P.hpl1 = P.hpl0 + GD_MAXTP2;
   P.hpl2 = P.hpl0 + 2 * GD_MAXTP2;

@@ -36,6 +36,18 @@
 #include "gd_device.h"
 #include "gd_launch.h"
 
+template <bool B>
+struct gd_bool {
+  static constexpr bool value = B;
+};
+
+/* A read of read-only, wave-uniform data through the scalar unit. */
+template <typename T>
+__device__ __forceinline__ T gd_uload(const T *p)
+{
+  return *(const __attribute__((address_space(4))) T *) p;
+}
+
 __device__ __forceinline__ float gd_bcast(float v, int lane)
 {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -79,15 +91,16 @@ __device__ __forceinline__ double gd_wave_sum64(double v)
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
 {
-  float4 *s_pos = (float4 *) smem;
-  float2 *s_eh = (float2 *) (smem + P.off_eh);
+  gd_lds_point *s_pt = (gd_lds_point *) smem;
   float4 *s_tab = (float4 *) (smem + P.off_tab);
   const int pnp_pad = P.pnp_pad;
   const float4 *g_pos = P.pt_pos + (size_t) prt * pnp_pad;
   const float2 *g_eh = P.pt_eh + (size_t) prt * pnp_pad;
   for (int i = threadIdx.x; i < pnp_pad; i += blockDim.x) {
-    s_pos[i] = g_pos[i];
-    s_eh[i] = g_eh[i];
+    const float4 q = g_pos[i];
+    s_pt[i].xy = make_float2(q.x, q.y);
+    s_pt[i].zw = make_float2(q.z, q.w);
+    s_pt[i].eh = g_eh[i];
   }
   const float4 *g_tab = (const float4 *) P.tab;
   const int ntab4 = P.n_rows * GD_ROW_STRIDE * 2;
@@ -112,25 +125,33 @@ __device__ __forceinline__ void gd_energy_wave(
     const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
     float ay1, float az1, const float cen[3], float out[GD_E_N], float *cmcc)
 {
-  const float4 *__restrict__ s_pos = (const float4 *) smem;
-  const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
+  const gd_lds_point *__restrict__ s_pt = (const gd_lds_point *) smem;
   const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
   const float4 *__restrict__ kde =
       P.kde_in_lds ? (const float4 *) (smem + P.off_kde) : P.kde;
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
-  const int psp_ld = P.psp_ld;
+  const int n_rows = P.n_rows;
   const gd_pair_scalars sc = P.sc;
+  /* the near-field coefficients sit beside other scalars in one fma: held
+   * in vector registers, the loop needs no copy per pair */
+  gd_pair_scalars sc_v = sc;
+  asm volatile("" : "+v"(sc_v.ea), "+v"(sc_v.eb));
 
   gd_pair_acc acc;
   acc.vdw = 0.0f; acc.ele = 0.0f; acc.pmf = 0.0f; acc.psp = 0.0f; acc.hdb = 0.0f;
+  float eele = 0.0f;
   float ehpc = 0.0f;
   /* MCC confusion counts (wave-uniform): contacts now vs reference, from one
-   * ballot per chunk against the reference mask */
-  const unsigned long long *mref =
-      P.mcc_ref ? P.mcc_ref + (size_t) prt * lna * nchunk : nullptr;
+   * ballot per chunk against the reference mask, read through scalar loads */
+  typedef const __attribute__((address_space(4))) unsigned long long *mask_ptr;
+  const mask_ptr mref =
+      P.mcc_ref ? (mask_ptr) (P.mcc_ref + (size_t) prt * lna * nchunk) : nullptr;
   int tp = 0, fp = 0, fn = 0;
+  typedef const __attribute__((address_space(1))) float *gfloat_ptr;
+  const gfloat_ptr psp_prt = (gfloat_ptr) (P.psp_pt + (size_t) prt * n_rows * P.pnp_pad);
+  const int kh = gd_uload(P.hpc_chunks + prt);
 
   /* pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) */
   for (int l = 0; l < lna; ++l) {
@@ -139,36 +160,43 @@ __device__ __forceinline__ void gd_energy_wave(
     const float lx = gd_bcast(hi ? ax1 : ax0, src);
     const float ly = gd_bcast(hi ? ay1 : ay0, src);
     const float lz = gd_bcast(hi ? az1 : az0, src);
-    const int row = P.lig_row[l];
-    const int lt = P.lig_type[l];
-    const float lq = P.lig_q[l];
+    const int row = gd_uload(P.lig_row + l);
+    const float lq = gd_uload(P.lig_q + l);
     const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
-    const __attribute__((address_space(1))) float *psp_row =
-        (const __attribute__((address_space(1))) float *) (P.psp + (size_t) lt * psp_ld);
+    const gfloat_ptr psp_row = psp_prt + (size_t) row * P.pnp_pad;
     float hacc = 0.0f;
-    for (int k = 0; k < nchunk; ++k) {
-      const int p = lane + k * GD_WAVE;
-      const float4 q = s_pos[p];
-      const uint32_t bits = __float_as_uint(q.w);
-      /* issue the PSP gather early; it is consumed at the end of the pair.
-       * Points outside class 2 read the zero PSP column and hydrophobic
-       * weights outside the class are 0: both add exact zeros, bit for
-       * bit what skipping them gives */
-      const float pv = psp_row[bits >> GD_PT_SEQ_SHIFT];
-      const float2 eh = s_eh[p];
-      const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      if (d2 <= 81.0f)
-        hacc += gd_hpc_pair(d2, eh.y);
-      const gd_pair_param pp = trow[bits & GD_PT_TYPE_MASK];
-      const int contact = gd_pair_terms(d2, lq * eh.x, &pp, &sc, 1, pv, &acc);
-      if (mref) {
-        const unsigned long long now = __ballot(contact);
-        const unsigned long long ref = mref[(size_t) l * nchunk + k];
-        tp += __popcll(now & ref);
-        fp += __popcll(now & ~ref);
-        fn += __popcll(~now & ref);
+    acc.ele = 0.0f;
+    /* chunks [0, kh) hold the hydrophobic-class points (the canonical order
+     * puts them first); beyond them every hydrophobic weight is 0, whose
+     * term would add exact zeros, so that loop leaves the term out */
+    auto chunks = [&](auto hpc_tag, int k0, int k1) {
+      for (int k = k0; k < k1; ++k) {
+        const int p = lane + k * GD_WAVE;
+        const float2 qxy = s_pt[p].xy, qzw = s_pt[p].zw;
+        const float2 eh = s_pt[p].eh;
+        const uint32_t bits = __float_as_uint(qzw.y);
+        /* the PSP read (chunk base uniform, lane offset fixed) is consumed
+         * at the end of the pair; points outside class 2 hold PSP 0 */
+        const float pv = (psp_row + k * GD_WAVE)[lane];
+        const float d2 = gd_dist2(lx - qxy.x, ly - qxy.y, lz - qzw.x);
+        if (decltype(hpc_tag)::value && d2 <= 81.0f)
+          hacc += gd_hpc_pair(d2, eh.y);
+        const gd_pair_param pp =
+            *(const gd_pair_param *) ((const char *) trow + bits);
+        const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, 1, pv, &acc);
+        if (mref) {
+          const unsigned long long now = __ballot(contact);
+          const unsigned long long ref = mref[(size_t) l * nchunk + k];
+          tp += __popcll(now & ref);
+          fp += __popcll(now & ~ref);
+          fn += __popcll(~now & ref);
+        }
       }
-    }
+    };
+    chunks(gd_bool<true>(), 0, kh);
+    chunks(gd_bool<false>(), kh, nchunk);
+    eele = gd_fma(lq, acc.ele, eele);
+    const int lt = gd_uload(P.lig_type + l);
     const float hs = gd_wave_sum(hacc);
     const float z = (hs - P.hpl0[lt]) * P.inv_hpl1[lt];
     ehpc += 0.5f * z * z - P.hpl2[lt];
@@ -232,7 +260,7 @@ __device__ __forceinline__ void gd_energy_wave(
 
   float sums[GD_F_NTERM];
   sums[GD_F_VDW] = gd_wave_sum(acc.vdw);
-  sums[GD_F_ELE] = gd_wave_sum(acc.ele);
+  sums[GD_F_ELE] = gd_wave_sum(eele);
   sums[GD_F_PMF] = gd_wave_sum(acc.pmf);
   sums[GD_F_HPC] = ehpc;
   sums[GD_F_HDB] = gd_wave_sum(acc.hdb);
@@ -311,7 +339,7 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
   }
   float ax0, ay0, az0, ax1, ay1, az1, cen[3], e[GD_E_N], cm;
   gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
-  gd_energy_wave(P, smem, P.rep_prt[r], ax0, ay0, az0, ax1, ay1, az1, cen, e, &cm);
+  gd_energy_wave(P, smem, gd_uload(P.rep_prt + r), ax0, ay0, az0, ax1, ay1, az1, cen, e, &cm);
   if (lane == 0) {
     S.cmcc[r] = cm;
     S.cmcc_best[r] = cm;
@@ -454,8 +482,8 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
 {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   __shared__ int s_task;
-  float *ws = (float *) (smem + wslot_off) + (threadIdx.x / GD_WAVE) * GD_WSLOT_FLOATS;
-  const int w = threadIdx.x / GD_WAVE;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / GD_WAVE);
+  float *ws = (float *) (smem + wslot_off) + w * GD_WSLOT_FLOATS;
   int staged_cx = -1, staged_prt = -1;
   int t = queue ? -1 : (int) blockIdx.x;
 
@@ -464,7 +492,9 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
       if (threadIdx.x == 0)
         s_task = atomicAdd(queue, 1);
       __syncthreads();
-      t = s_task;
+      /* the task index is workgroup-uniform: keep it (and every pack field
+       * read through it) in scalar registers */
+      t = __builtin_amdgcn_readfirstlane(s_task);
       __syncthreads();
     } else if (t != (int) blockIdx.x) {
       break;  /* grid form: one task per workgroup */
@@ -587,7 +617,7 @@ gd_score_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ t
   const gd_pack &P = packs[task.cx];
   gd_stage(P, task.prt, smem);
   __syncthreads();
-  const int w = threadIdx.x / GD_WAVE;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / GD_WAVE);
   if (w >= task.r_count)
     return;
   const int lane = threadIdx.x & (GD_WAVE - 1);
@@ -690,11 +720,11 @@ __device__ void gd_energy64_wave(const gd_pack &P, int lconf, int pconf, const f
       const float4 q = pp[p];
       const float2 eh = pe[p];
       const uint32_t bits = __float_as_uint(q.w);
-      const gd_pair_param tp = trow[bits & GD_PT_TYPE_MASK];
+      const gd_pair_param tp = trow[bits >> GD_PT_TYPE_SHIFT];
       const double dx = lx - q.x, dy = ly - q.y, dz = lz - q.z;
       const double d2 = dx * dx + dy * dy + dz * dz;
       const double d = sqrt(d2);
-      if ((bits & GD_PT_HPC) && d2 <= 81.0) {
+      if (eh.y != 0.0f && d2 <= 81.0) {
         const double u = d2 / 81.0;
         hpc1 += (double) eh.y * (1.0 - (3.5 * u - 4.5 * u * u + 2.5 * u * u * u - 0.5 * u * u * u * u));
       }
@@ -707,8 +737,8 @@ __device__ void gd_energy64_wave(const gd_pack &P, int lconf, int pconf, const f
       eele += lq * (double) eh.x * g;
       const double dm = d - (double) tp.pmf0;
       epmf += (double) tp.pmf1 / (1.0 + exp((-0.5 * d + 6.0) * dm));
-      if ((bits & GD_PT_C2) && dm <= 0.0)
-        epsp += (double) P.psp[(size_t) lt * P.psp_ld + (bits >> GD_PT_SEQ_SHIFT)];
+      if (dm <= 0.0)
+        epsp += (double) P.psp_pt[((size_t) pconf * P.n_rows + row) * P.pnp_pad + p];
       if (tp.hdb0 > 0.1f) {
         const double h = (d - (double) tp.hdb0) * (double) tp.hdb1;
         ehdb += (double) tp.hdb1 * exp(-0.5 * h * h);

@@ -601,6 +601,9 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
   const gd_protein *prt = c->prt;
   gd_pair_acc acc[W];
   memset(acc, 0, sizeof acc);
+  float eele[W];
+  for (int j = 0; j < W; ++j)
+    eele[j] = 0.0f;
   float ehpc = 0.0f;
   for (int l = 0; l < lna; ++l) {
     const float lx = ax[l], ly = ay[l], lz = az[l];
@@ -610,6 +613,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     float hacc[W];
     for (int j = 0; j < W; ++j) {
       hacc[j] = 0.0f;
+      acc[j].ele = 0.0f;
       for (int p = j; p < pnp; p += W) {
         const size_t q = po + c->perm[po + p];
         const float d2 = gd_dist2(lx - prt->x[q], ly - prt->y[q], lz - prt->z[q]);
@@ -623,7 +627,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         pp.pad0 = 0.0f; pp.pad1 = 0.0f;
         const int is_c2 = prt->c[q] == 2;
         const float pv = is_c2 ? psp_row[prt->seq3r[q]] : 0.0f;
-        const int now = gd_pair_terms(d2, lq * prt->ele[q], &pp, &c->sc, is_c2, pv, &acc[j]);
+        const int now = gd_pair_terms(d2, prt->ele[q], &pp, &c->sc, is_c2, pv, &acc[j]);
         if (ref) {
           const int was = ref[(size_t) l * pnp + c->perm[po + p]];
           tp += was && now;
@@ -633,6 +637,8 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         }
       }
     }
+    for (int j = 0; j < W; ++j)
+      eele[j] = gd_fma(lq, acc[j].ele, eele[j]);
     const float hs = bfly(hacc);
     const float z = (hs - ep->hpl0[lt]) * (1.0f / ep->hpl1[lt]);
     ehpc += 0.5f * z * z - ep->hpl2[lt];
@@ -683,8 +689,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
   float sums[GD_F_NTERM];
   for (int j = 0; j < W; ++j) t[j] = acc[j].vdw;
   sums[GD_F_VDW] = bfly(t);
-  for (int j = 0; j < W; ++j) t[j] = acc[j].ele;
-  sums[GD_F_ELE] = bfly(t);
+  sums[GD_F_ELE] = bfly(eele);
   for (int j = 0; j < W; ++j) t[j] = acc[j].pmf;
   sums[GD_F_PMF] = bfly(t);
   sums[GD_F_HPC] = ehpc;
