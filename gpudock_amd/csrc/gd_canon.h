@@ -356,10 +356,12 @@ GD_HD float gd_hpc_pair(float d2, float hpp)
  * not class 2); it is added for class-2 points inside the PMF radius
  * (calcenergy_d.cu:123-126).  acc->ele collects ele_point * g for the
  * current ligand atom; the caller multiplies that sum by the atom's charge
- * once per atom (q sum(ele g) = sum(q ele g), calcenergy_d.cu:99-105). */
+ * once per atom (q sum(ele g) = sum(q ele g), calcenergy_d.cu:99-105).
+ * hdb_row = 0 promises that no entry of this ligand type's row has
+ * hdb0 > 0.1, so the H-bond test can be left out. */
 GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
                         const gd_pair_scalars *sc, int is_c2, float psp_val,
-                        gd_pair_acc *acc)
+                        int hdb_row, gd_pair_acc *acc)
 {
   /* all inverse powers from one reciprocal square root */
   const float r1 = gd_rsqrt(d2);
@@ -392,7 +394,7 @@ GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
   acc->psp += (is_c2 && dm <= 0.0f) ? psp_val : 0.0f;
 
   /* hydrogen bond */
-  if (pp->hdb0 > 0.1f) {
+  if (hdb_row && pp->hdb0 > 0.1f) {
     const float h = (d - pp->hdb0) * pp->hdb1;
     acc->hdb = gd_fma(pp->hdb1, gd_exp_hot((-0.5f * h) * h), acc->hdb);
   }

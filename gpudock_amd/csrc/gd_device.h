@@ -88,6 +88,9 @@ typedef struct gd_pack {
   /* [n_prt] chunks of 64 points, from the first, that hold every point of
    * the hydrophobic class (the canonical order puts them first) */
   const int *hpc_chunks;
+  /* [n_rows] 1 if some point type pairs with the row's ligand type through
+   * an H-bond term (hdb0 > 0.1) */
+  const int *row_hdb;
   /* per ligand type (24 entries each) */
   const float *hpl0, *hpl1, *hpl2;
   const float *inv_hpl1; /* 1.0f / hpl1 */

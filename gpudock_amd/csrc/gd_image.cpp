@@ -97,6 +97,11 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
       q.hdb1 = ep->hdb1[lt][pt];
     }
   }
+  I.row_hdb.assign(P.n_rows, 0);
+  for (int r = 0; r < P.n_rows; ++r)
+    for (int pt = 0; pt < GD_MAXTP1; ++pt)
+      if (ep->hdb0[row_type[r]][pt] > 0.1f)
+        I.row_hdb[r] = 1;
 
   /* pocket points per conformer, canonical order (stable by
    * gd_point_sort_key), padded to a multiple of 64 with zero points */
@@ -318,6 +323,7 @@ GdBlock gd_layout_block(const GdImage &I)
   b.tcc = take(vbytes(I.tcc));
   b.psp_pt = take(vbytes(I.psp_pt));
   b.hpc_chunks = take(vbytes(I.hpc_chunks));
+  b.row_hdb = take(vbytes(I.row_hdb));
   b.hpl = take(vbytes(I.hpl));
   b.mb = take(vbytes(I.mb));
   b.rep_lig = take(n * 4);
@@ -371,6 +377,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   put(b.tcc, I.tcc.data(), vbytes(I.tcc));
   put(b.psp_pt, I.psp_pt.data(), vbytes(I.psp_pt));
   put(b.hpc_chunks, I.hpc_chunks.data(), vbytes(I.hpc_chunks));
+  put(b.row_hdb, I.row_hdb.data(), vbytes(I.row_hdb));
   put(b.hpl, I.hpl.data(), vbytes(I.hpl));
   put(b.mb, I.mb.data(), vbytes(I.mb));
   put(b.rep_lig, I.rep_lig.data(), vbytes(I.rep_lig));
@@ -392,6 +399,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.mcs_tcc = (const float *) (dev + b.tcc);
   P.psp_pt = (const float *) (dev + b.psp_pt);
   P.hpc_chunks = (const int *) (dev + b.hpc_chunks);
+  P.row_hdb = (const int *) (dev + b.row_hdb);
   P.hpl0 = (const float *) (dev + b.hpl);
   P.hpl1 = P.hpl0 + GD_MAXTP2;
   P.hpl2 = P.hpl0 + 2 * GD_MAXTP2;

@@ -36,6 +36,10 @@
 #include "gd_device.h"
 #include "gd_launch.h"
 
+/* Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data
+ * format, no swizzle): reads past num_records return 0. */
+#define GD_BUFFER_DWORD3 0x00020000
+
 template <bool B>
 struct gd_bool {
   static constexpr bool value = B;
@@ -152,6 +156,7 @@ __device__ __forceinline__ void gd_energy_wave(
   typedef const __attribute__((address_space(1))) float *gfloat_ptr;
   const gfloat_ptr psp_prt = (gfloat_ptr) (P.psp_pt + (size_t) prt * n_rows * P.pnp_pad);
   const int kh = gd_uload(P.hpc_chunks + prt);
+  const unsigned ulane = (unsigned) lane;
 
   /* pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) */
   for (int l = 0; l < lna; ++l) {
@@ -163,13 +168,15 @@ __device__ __forceinline__ void gd_energy_wave(
     const int row = gd_uload(P.lig_row + l);
     const float lq = gd_uload(P.lig_q + l);
     const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
-    const gfloat_ptr psp_row = psp_prt + (size_t) row * P.pnp_pad;
+    /* the PSP row as a buffer: lane offset fixed, chunk offset scalar */
+    const __amdgpu_buffer_rsrc_t psp_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) (psp_prt + (size_t) row * P.pnp_pad), 0, P.pnp_pad * 4, GD_BUFFER_DWORD3);
     float hacc = 0.0f;
     acc.ele = 0.0f;
     /* chunks [0, kh) hold the hydrophobic-class points (the canonical order
      * puts them first); beyond them every hydrophobic weight is 0, whose
      * term would add exact zeros, so that loop leaves the term out */
-    auto chunks = [&](auto hpc_tag, int k0, int k1) {
+    auto chunks = [&](auto hpc_tag, auto hdb_tag, int k0, int k1) {
       for (int k = k0; k < k1; ++k) {
         const int p = lane + k * GD_WAVE;
         const float2 qxy = s_pt[p].xy, qzw = s_pt[p].zw;
@@ -177,13 +184,15 @@ __device__ __forceinline__ void gd_energy_wave(
         const uint32_t bits = __float_as_uint(qzw.y);
         /* the PSP read (chunk base uniform, lane offset fixed) is consumed
          * at the end of the pair; points outside class 2 hold PSP 0 */
-        const float pv = (psp_row + k * GD_WAVE)[lane];
+        const float pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+            psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
         const float d2 = gd_dist2(lx - qxy.x, ly - qxy.y, lz - qzw.x);
         if (decltype(hpc_tag)::value && d2 <= 81.0f)
           hacc += gd_hpc_pair(d2, eh.y);
         const gd_pair_param pp =
             *(const gd_pair_param *) ((const char *) trow + bits);
-        const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, 1, pv, &acc);
+        const int contact =
+            gd_pair_terms(d2, eh.x, &pp, &sc_v, 1, pv, decltype(hdb_tag)::value, &acc);
         if (mref) {
           const unsigned long long now = __ballot(contact);
           const unsigned long long ref = mref[(size_t) l * nchunk + k];
@@ -193,8 +202,14 @@ __device__ __forceinline__ void gd_energy_wave(
         }
       }
     };
-    chunks(gd_bool<true>(), 0, kh);
-    chunks(gd_bool<false>(), kh, nchunk);
+    /* rows of ligand types without any H-bond pair leave that test out */
+    if (gd_uload(P.row_hdb + row)) {
+      chunks(gd_bool<true>(), gd_bool<true>(), 0, kh);
+      chunks(gd_bool<false>(), gd_bool<true>(), kh, nchunk);
+    } else {
+      chunks(gd_bool<true>(), gd_bool<false>(), 0, kh);
+      chunks(gd_bool<false>(), gd_bool<false>(), kh, nchunk);
+    }
     eele = gd_fma(lq, acc.ele, eele);
     const int lt = gd_uload(P.lig_type + l);
     const float hs = gd_wave_sum(hacc);

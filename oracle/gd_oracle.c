@@ -627,7 +627,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         pp.pad0 = 0.0f; pp.pad1 = 0.0f;
         const int is_c2 = prt->c[q] == 2;
         const float pv = is_c2 ? psp_row[prt->seq3r[q]] : 0.0f;
-        const int now = gd_pair_terms(d2, prt->ele[q], &pp, &c->sc, is_c2, pv, &acc[j]);
+        const int now = gd_pair_terms(d2, prt->ele[q], &pp, &c->sc, is_c2, pv, 1, &acc[j]);
         if (ref) {
           const int was = ref[(size_t) l * pnp + c->perm[po + p]];
           tp += was && now;
