@@ -45,6 +45,15 @@
 #define GD_HD static inline
 #endif
 
+/* Inside a branch whose condition is wavefront-uniform: keeps the compiler
+ * from evaluating the branch body unconditionally (it has no effect on
+ * the values computed). */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GD_KEEP_BRANCH() asm volatile("")
+#else
+#define GD_KEEP_BRANCH() ((void) 0)
+#endif
+
 /* ------------------------------------------------------------------ */
 /* bit casts                                                           */
 /* ------------------------------------------------------------------ */
@@ -132,8 +141,9 @@ GD_HD float gd_exp(float x)
  * so 2^n is one shift-and-add of those bits); then a one-constant
  * Cody-Waite step (|r| error <= 2.4e-7 at |x| = 86) and a degree-6
  * polynomial fitted here for minimax relative error on |r| <= ln2/2
- * (6.5e-8 in FP32); total error below 4 ulp. */
-GD_HD float gd_exp_hot(float x)
+ * (6.5e-8 in FP32); total error below 4 ulp.  Returns the polynomial p and
+ * sets *scale = 2^n; e^x = p * 2^n. */
+GD_HD float gd_exp_hot_split(float x, float *scale)
 {
   const float xc = __builtin_fmaxf(x, -86.0f);
   const float t = gd_fma(xc, 0x1.715476p+0f, 0x1.8p+23f);
@@ -147,7 +157,18 @@ GD_HD float gd_exp_hot(float x)
   p = gd_fma(p, r, 1.0f);
   p = gd_fma(p, r, 1.0f);
   /* bits(t) = 0x4b400000 + n: (bits << 23) + (127 << 23) = bits of 2^n */
-  return p * gd_as_float((gd_as_uint(t) << 23) + 0x3f800000u);
+  *scale = gd_as_float((gd_as_uint(t) << 23) + 0x3f800000u);
+  return p;
+}
+
+/* e^x = p * 2^n of gd_exp_hot_split.  The product is exact (n >= -124 keeps
+ * it normal), so c + e^x rounds like gd_fma(p, 2^n, c): callers that add
+ * the exponential to something use that form, one op fewer. */
+GD_HD float gd_exp_hot(float x)
+{
+  float s;
+  const float p = gd_exp_hot_split(x, &s);
+  return p * s;
 }
 
 /* ------------------------------------------------------------------ */
@@ -362,10 +383,11 @@ GD_HD float gd_hpc_pair(float d2, float hpp)
  * current ligand atom; the caller multiplies that sum by the atom's charge
  * once per atom (q sum(ele g) = sum(q ele g), calcenergy_d.cu:99-105).
  * hdb_row = 0 promises that no entry of this ligand type's row has
- * hdb0 > 0.1, so the H-bond test can be left out. */
+ * hdb0 > 0.1, so the H-bond test can be left out; near_any = 0 promises
+ * d2 >= dcut2 (no short-range electrostatics). */
 GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
                         const gd_pair_scalars *sc, int is_c2, float psp_val,
-                        int hdb_row, gd_pair_acc *acc)
+                        int hdb_row, int near_any, gd_pair_acc *acc)
 {
   /* all inverse powers from one reciprocal square root */
   const float r1 = gd_rsqrt(d2);
@@ -380,14 +402,21 @@ GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
   const float p2 = pp->p2a * r6;
   const float p4 = gd_fma(p1 * sc->lj0, gd_fma(sc->lj1, d2, 1.0f), 1.0f);
 
-  /* distance-dependent electrostatics */
-  const float g = (d2 < sc->dcut2) ? gd_fma(d2, gd_fma(d, sc->eb, sc->ea), sc->el0)
-                                   : r1 * sc->inv_el1;
+  /* distance-dependent electrostatics; the short-range form only where
+   * some pair of the call's group (near_any) is inside it */
+  float g = r1 * sc->inv_el1;
+  if (near_any) {
+    GD_KEEP_BRANCH();
+    if (d2 < sc->dcut2)
+      g = gd_fma(d2, gd_fma(d, sc->eb, sc->ea), sc->el0);
+  }
   acc->ele = gd_fma(ele_pt, g, acc->ele);
 
   /* PMF contact denominator */
   const float dm = d - pp->pmf0;
-  const float q1 = 1.0f + gd_exp_hot(gd_fma(-0.5f, d, 6.0f) * dm);
+  float es;
+  const float ep = gd_exp_hot_split(gd_fma(-0.5f, d, 6.0f) * dm, &es);
+  const float q1 = gd_fma(ep, es, 1.0f);   /* 1 + e^z, one rounding as before */
 
   /* one reciprocal serves both quotients: 1/(p4 q1) */
   const float rr = gd_rcp(p4 * q1);

@@ -191,8 +191,9 @@ __device__ __forceinline__ void gd_energy_wave(
           hacc += gd_hpc_pair(d2, eh.y);
         const gd_pair_param pp =
             *(const gd_pair_param *) ((const char *) trow + bits);
-        const int contact =
-            gd_pair_terms(d2, eh.x, &pp, &sc_v, 1, pv, decltype(hdb_tag)::value, &acc);
+        const int near_any = __ballot(d2 < sc_v.dcut2) != 0;
+        const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, 1, pv,
+                                          decltype(hdb_tag)::value, near_any, &acc);
         if (mref) {
           const unsigned long long now = __ballot(contact);
           const unsigned long long ref = mref[(size_t) l * nchunk + k];
@@ -233,7 +234,9 @@ __device__ __forceinline__ void gd_energy_wave(
     for (int i = s0 + lane; i < s1; i += GD_WAVE) {
       const float4 q = kde[i];
       const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      v += gd_exp_hot(kde2 * d2);
+      float es;
+      const float ep = gd_exp_hot_split(kde2 * d2, &es);
+      v = gd_fma(ep, es, v);   /* v + e^x: the product is exact */
     }
     v = gd_wave_sum(v);
     if (s1 > s0)

@@ -627,7 +627,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         pp.pad0 = 0.0f; pp.pad1 = 0.0f;
         const int is_c2 = prt->c[q] == 2;
         const float pv = is_c2 ? psp_row[prt->seq3r[q]] : 0.0f;
-        const int now = gd_pair_terms(d2, prt->ele[q], &pp, &c->sc, is_c2, pv, 1, &acc[j]);
+        const int now = gd_pair_terms(d2, prt->ele[q], &pp, &c->sc, is_c2, pv, 1, 1, &acc[j]);
         if (ref) {
           const int was = ref[(size_t) l * pnp + c->perm[po + p]];
           tp += was && now;
@@ -652,8 +652,12 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     for (int j = 0; j < W; ++j) {
       v[j] = 0.0f;
       for (int i = s0 + j; i < s1; i += W)
-        v[j] += gd_exp_hot(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
-                                           az[l] - c->kz[i]));
+      {
+        float es;
+        const float e = gd_exp_hot_split(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
+                                                             az[l] - c->kz[i]), &es);
+        v[j] = gd_fma(e, es, v[j]);
+      }
     }
     const float vs = bfly(v);
     const int n = s1 - s0;
