@@ -627,10 +627,16 @@ int gd_engine::autotune(gd_tune_report *rep)
     he = hipEventCreate(&a);
   if (he == hipSuccess)
     he = hipEventCreate(&b);
-  /* one timed trial of a configuration (after a warm-up launch) */
-  auto trial = [&](int wpg, int pers, int kg, int occ) {
+  struct Cand {
+    float ms;
+    int wpg, pers, kg, occ;
+  };
+  std::vector<Cand> cands;
+  /* time `reps` cycles of a configuration after a warm-up cycle; returns
+   * the mean cycle time in ms, or a negative value on error */
+  auto time_cfg = [&](int wpg, int pers, int kg, int occ, int reps) -> float {
     if (rc || he != hipSuccess)
-      return;
+      return -1.0f;
     kde_global = kg;
     lc.wpg = wpg;
     lc.persistent = pers;
@@ -640,7 +646,7 @@ int gd_engine::autotune(gd_tune_report *rep)
       rc = trial_cycle();
     if (!rc)
       he = hipEventRecord(a, stream);
-    if (!rc && he == hipSuccess)
+    for (int i = 0; i < reps && !rc && he == hipSuccess; ++i)
       rc = trial_cycle();
     if (!rc && he == hipSuccess)
       he = hipEventRecord(b, stream);
@@ -650,8 +656,15 @@ int gd_engine::autotune(gd_tune_report *rep)
     if (!rc && he == hipSuccess)
       he = hipEventElapsedTime(&ms, a, b);
     if (rc || he != hipSuccess)
-      return;
+      return -1.0f;
     reap();
+    return ms / (float) reps;
+  };
+  /* one trial of the staged search */
+  auto trial = [&](int wpg, int pers, int kg, int occ) {
+    const float ms = time_cfg(wpg, pers, kg, occ, 2);
+    if (ms < 0.0f)
+      return;
     if (rep && n < GD_TUNE_MAX) {
       rep->wpg[n] = wpg;
       rep->persistent[n] = pers;
@@ -660,6 +673,7 @@ int gd_engine::autotune(gd_tune_report *rep)
       rep->ms[n] = ms;
     }
     ++n;
+    cands.push_back({ms, wpg, pers, kg, occ});
     if (ms < best) {
       best = ms;
       best_c = lc;
@@ -688,6 +702,32 @@ int gd_engine::autotune(gd_tune_report *rep)
       for (int pers = 0; pers < 2; ++pers)
         if ((wpgs[wi] != c.wpg || pers != c.persistent) && (c.occ != occ0 || kg != kg0))
           trial(wpgs[wi], pers, kg, c.occ);
+  }
+  /* the staged trials differ by a few per cent, about what one short trial
+   * can resolve: the three fastest distinct configurations are timed again
+   * over more cycles, in turn, and the fastest of that round is kept */
+  if (!rc && he == hipSuccess && cands.size() > 1) {
+    std::stable_sort(cands.begin(), cands.end(),
+                     [](const Cand &x, const Cand &y) { return x.ms < y.ms; });
+    std::vector<Cand> top;
+    for (const Cand &c : cands) {
+      bool dup = false;
+      for (const Cand &t : top)
+        dup |= t.wpg == c.wpg && t.pers == c.pers && t.kg == c.kg && t.occ == c.occ;
+      if (!dup)
+        top.push_back(c);
+      if (top.size() == 3)
+        break;
+    }
+    float conf_best = 3.0e38f;
+    for (const Cand &c : top) {
+      const float ms = time_cfg(c.wpg, c.pers, c.kg, c.occ, 4);
+      if (ms >= 0.0f && ms < conf_best) {
+        conf_best = ms;
+        best_c = lc;
+        best_kg = c.kg;
+      }
+    }
   }
   if (a) (void) hipEventDestroy(a);
   if (b) (void) hipEventDestroy(b);

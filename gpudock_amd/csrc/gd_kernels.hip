@@ -131,8 +131,7 @@ __device__ __forceinline__ void gd_energy_wave(
 {
   const gd_lds_point *__restrict__ s_pt = (const gd_lds_point *) smem;
   const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
-  const float4 *__restrict__ kde =
-      P.kde_in_lds ? (const float4 *) (smem + P.off_kde) : P.kde;
+  const int kde_lds = P.kde_in_lds;
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
@@ -227,20 +226,30 @@ __device__ __forceinline__ void gd_energy_wave(
     const float lx = gd_bcast(hi ? ax1 : ax0, src);
     const float ly = gd_bcast(hi ? ay1 : ay0, src);
     const float lz = gd_bcast(hi ? az1 : az0, src);
-    const int row = P.lig_row[l];
-    const int s0 = P.kde_seg[2 * row];
-    const int s1 = P.kde_seg[2 * row + 1];
+    const int row = gd_uload(P.lig_row + l);
+    const int s0 = gd_uload(P.kde_seg + 2 * row);
+    const int s1 = gd_uload(P.kde_seg + 2 * row + 1);
     float v = 0.0f;
-    for (int i = s0 + lane; i < s1; i += GD_WAVE) {
-      const float4 q = kde[i];
-      const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      float es;
-      const float ep = gd_exp_hot_split(kde2 * d2, &es);
-      v = gd_fma(ep, es, v);   /* v + e^x: the product is exact */
-    }
+    /* wavefront-uniform trip count; lanes past the segment sit out the
+     * last round */
+    auto segment = [&](const auto *pts) {
+      for (int i0 = s0; i0 < s1; i0 += GD_WAVE) {
+        const int i = i0 + lane;
+        if (lane < s1 - i0) {
+          const float d2 = gd_dist2(lx - pts[4 * i], ly - pts[4 * i + 1], lz - pts[4 * i + 2]);
+          float es;
+          const float ep = gd_exp_hot_split(kde2 * d2, &es);
+          v = gd_fma(ep, es, v);   /* v + e^x: the product is exact */
+        }
+      }
+    };
+    if (kde_lds)
+      segment((const __attribute__((address_space(3))) float *) (smem + P.off_kde));
+    else
+      segment((const __attribute__((address_space(1))) float *) P.kde);
     v = gd_wave_sum(v);
     if (s1 > s0)
-      ekde += v * P.kde_inv_n[row];
+      ekde += v * gd_uload(P.kde_inv_n + row);
   }
   ekde = ekde / P.kde3;
 
