@@ -31,14 +31,6 @@ typedef struct gd_state {
   float *cmcc_best; /* [n_rep] cmcc of the best pose */
 } gd_state;
 
-/* A pocket point as staged in LDS: one 24-byte record, read from one
- * address per point. */
-typedef struct __attribute__((aligned(8))) gd_lds_point {
-  float2 xy;
-  float2 zw;    /* z, bits(type << GD_PT_TYPE_SHIFT) */
-  float2 eh;    /* ele, hpp (0 outside the hydrophobic class) */
-} gd_lds_point;
-
 /* One complex. */
 typedef struct gd_pack {
   /* sizes */

@@ -280,9 +280,10 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
 
   /* LDS carve-up */
   auto al16 = [](int v) { return (v + 15) & ~15; };
-  /* pocket points staged as 24-byte records (position + type, ele + hpp) */
-  P.off_eh = 0;
-  P.off_tab = al16(pnp_pad * (int) sizeof(gd_lds_point));
+  /* pocket points staged as two planes, positions + type (16 B) and
+   * ele + hpp (8 B): contiguous per lane, so free of bank conflicts */
+  P.off_eh = pnp_pad * 16;
+  P.off_tab = al16(P.off_eh + pnp_pad * 8);
   P.off_kde = al16(P.off_tab + P.n_rows * GD_ROW_STRIDE * (int) sizeof(gd_pair_param));
   const int with_kde = al16(P.off_kde + P.pnk * 16);
   const int reserve = 16 * GD_WSLOT_FLOATS * 4 + 64;   /* wave slots of up to 16 waves */

@@ -95,16 +95,16 @@ __device__ __forceinline__ double gd_wave_sum64(double v)
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
 {
-  gd_lds_point *s_pt = (gd_lds_point *) smem;
+  float4 *s_pos = (float4 *) smem;
+  float2 *s_eh = (float2 *) (smem + P.off_eh);
   float4 *s_tab = (float4 *) (smem + P.off_tab);
   const int pnp_pad = P.pnp_pad;
   const float4 *g_pos = P.pt_pos + (size_t) prt * pnp_pad;
   const float2 *g_eh = P.pt_eh + (size_t) prt * pnp_pad;
   for (int i = threadIdx.x; i < pnp_pad; i += blockDim.x) {
     const float4 q = g_pos[i];
-    s_pt[i].xy = make_float2(q.x, q.y);
-    s_pt[i].zw = make_float2(q.z, q.w);
-    s_pt[i].eh = g_eh[i];
+    s_pos[i] = q;
+    s_eh[i] = g_eh[i];
   }
   const float4 *g_tab = (const float4 *) P.tab;
   const int ntab4 = P.n_rows * GD_ROW_STRIDE * 2;
@@ -129,7 +129,8 @@ __device__ __forceinline__ void gd_energy_wave(
     const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
     float ay1, float az1, const float cen[3], float out[GD_E_N], float *cmcc)
 {
-  const gd_lds_point *__restrict__ s_pt = (const gd_lds_point *) smem;
+  const float4 *__restrict__ s_pos = (const float4 *) smem;
+  const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
   const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
   const int kde_lds = P.kde_in_lds;
   const int lane = threadIdx.x & (GD_WAVE - 1);
@@ -178,14 +179,14 @@ __device__ __forceinline__ void gd_energy_wave(
     auto chunks = [&](auto hpc_tag, auto hdb_tag, int k0, int k1) {
       for (int k = k0; k < k1; ++k) {
         const int p = lane + k * GD_WAVE;
-        const float2 qxy = s_pt[p].xy, qzw = s_pt[p].zw;
-        const float2 eh = s_pt[p].eh;
-        const uint32_t bits = __float_as_uint(qzw.y);
+        const float4 q = s_pos[p];
+        const float2 eh = s_eh[p];
+        const uint32_t bits = __float_as_uint(q.w);
         /* the PSP read (chunk base uniform, lane offset fixed) is consumed
          * at the end of the pair; points outside class 2 hold PSP 0 */
         const float pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
             psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
-        const float d2 = gd_dist2(lx - qxy.x, ly - qxy.y, lz - qzw.x);
+        const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
         if (decltype(hpc_tag)::value && d2 <= 81.0f)
           hacc += gd_hpc_pair(d2, eh.y);
         const gd_pair_param pp =
@@ -392,8 +393,7 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const gd_state S = P.S;
-  float bx0, by0, bz0, bx1, by1, bz1;
-  gd_load_conformer(P, P.rep_lig[r], &bx0, &by0, &bz0, &bx1, &by1, &bz1);
+  const int lconf = gd_uload(P.rep_lig + r);
   if (lane < 6)
     ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
   if (lane < GD_E_N)
@@ -441,6 +441,10 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
     m_new[5] = P.move_scale[5] * (2.0f * gd_u01(rb.v[1]) - 1.0f) + ws[GD_WS_MOLD + 5];
     const float u_acc = gd_u01(rb.v[2]);
 
+    /* the conformer is re-read every step (an L1 hit) rather than held in
+     * registers across the energy evaluation */
+    float bx0, by0, bz0, bx1, by1, bz1;
+    gd_load_conformer(P, lconf, &bx0, &by0, &bz0, &bx1, &by1, &bz1);
     float ax0, ay0, az0, ax1, ay1, az1, cen[3], e_new[GD_E_N], cm_new;
     gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
              &az1, cen);
