@@ -40,6 +40,8 @@
  * format, no swizzle): reads past num_records return 0. */
 #define GD_BUFFER_DWORD3 0x00020000
 
+typedef float gd_f4 __attribute__((ext_vector_type(4)));
+
 template <bool B>
 struct gd_bool {
   static constexpr bool value = B;
@@ -237,7 +239,8 @@ __device__ __forceinline__ void gd_energy_wave(
       for (int i0 = s0; i0 < s1; i0 += GD_WAVE) {
         const int i = i0 + lane;
         if (lane < s1 - i0) {
-          const float d2 = gd_dist2(lx - pts[4 * i], ly - pts[4 * i + 1], lz - pts[4 * i + 2]);
+          const gd_f4 q = pts[i];   /* one 16-byte read per lane */
+          const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
           float es;
           const float ep = gd_exp_hot_split(kde2 * d2, &es);
           v = gd_fma(ep, es, v);   /* v + e^x: the product is exact */
@@ -245,9 +248,9 @@ __device__ __forceinline__ void gd_energy_wave(
       }
     };
     if (kde_lds)
-      segment((const __attribute__((address_space(3))) float *) (smem + P.off_kde));
+      segment((const __attribute__((address_space(3))) gd_f4 *) (smem + P.off_kde));
     else
-      segment((const __attribute__((address_space(1))) float *) P.kde);
+      segment((const __attribute__((address_space(1))) gd_f4 *) P.kde);
     v = gd_wave_sum(v);
     if (s1 > s0)
       ekde += v * gd_uload(P.kde_inv_n + row);
