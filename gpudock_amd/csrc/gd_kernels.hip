@@ -506,7 +506,7 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
 /* passes n_tasks (every workgroup exits once the list is drained).          */
 /* ------------------------------------------------------------------------ */
 /* WPG: waves (replicas) per workgroup.  MINW: waves per SIMD the register
- * allocation must allow (1 = the compiler's choice, 92 VGPRs, 5 waves;
+ * allocation must allow (1 = the compiler's choice, about 90 VGPRs, 5 waves;
  * 6 = 80 VGPRs; 8 = 64 VGPRs with a few spill slots) -- the launch shape
  * and this are picked by timing. */
 template <int WPG, int MINW>
