@@ -4,7 +4,8 @@ The library holds the HIP kernels (csrc/gd_kernels.hip), the host image of a
 complex (csrc/gd_image.cpp), the engine and the C ABI entry points
 (csrc/gd_engine.cpp), the serving loop (csrc/gd_server.cpp), shared host utilities
 (csrc/gd_host.cpp), the host preparation / parameter parsing
-(csrc/gd_prep.cpp) and the input-file loaders (csrc/gd_load.cpp).  All device code is compiled with contraction off
+(csrc/gd_prep.cpp) and the input-file loaders (csrc/gd_load.cpp); then the
+command-line front end gd_dock (csrc/gd_dock.cpp) beside it.  All device code is compiled with contraction off
 (the canonical arithmetic relies on it) and without packed FP32 ops.
 """
 import os
@@ -14,6 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgdremc.so")
+DOCK = os.path.join(HERE, "gd_dock")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GD_OFFLOAD_ARCH", "gfx950")
 
@@ -45,6 +47,11 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the command-line front end, linked against the library beside it
+    tmp = DOCK + ".tmp"
+    _run([HIPCC] + COMMON + ["-I", CSRC, os.path.join(CSRC, "gd_dock.cpp"), "-o", tmp,
+                             "-L", HERE, "-lgdremc", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DOCK)
     return LIB
 
 
