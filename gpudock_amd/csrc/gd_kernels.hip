@@ -135,6 +135,8 @@ __device__ __forceinline__ void gd_energy_wave(
   const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
   const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
   const int kde_lds = P.kde_in_lds;
+  const __amdgpu_buffer_rsrc_t kde_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *) P.kde, 0, P.pnk * 16, GD_BUFFER_DWORD3);
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
@@ -235,11 +237,10 @@ __device__ __forceinline__ void gd_energy_wave(
     float v = 0.0f;
     /* wavefront-uniform trip count; lanes past the segment sit out the
      * last round */
-    auto segment = [&](const auto *pts) {
+    auto segment = [&](auto read) {
       for (int i0 = s0; i0 < s1; i0 += GD_WAVE) {
-        const int i = i0 + lane;
         if (lane < s1 - i0) {
-          const gd_f4 q = pts[i];   /* one 16-byte read per lane */
+          const gd_f4 q = read(i0);   /* one 16-byte read per lane */
           const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
           float es;
           const float ep = gd_exp_hot_split(kde2 * d2, &es);
@@ -247,10 +248,17 @@ __device__ __forceinline__ void gd_energy_wave(
         }
       }
     };
-    if (kde_lds)
-      segment((const __attribute__((address_space(3))) gd_f4 *) (smem + P.off_kde));
-    else
-      segment((const __attribute__((address_space(1))) gd_f4 *) P.kde);
+    if (kde_lds) {
+      const __attribute__((address_space(3))) gd_f4 *pts =
+          (const __attribute__((address_space(3))) gd_f4 *) (smem + P.off_kde);
+      segment([&](int i0) { return pts[i0 + lane]; });
+    } else {
+      /* from HBM as a raw buffer: lane offset fixed, segment offset scalar */
+      segment([&](int i0) {
+        return __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(
+                                             kde_rsrc, ulane * 16u, i0 * 16, 0));
+      });
+    }
     v = gd_wave_sum(v);
     if (s1 > s0)
       ekde += v * gd_uload(P.kde_inv_n + row);
