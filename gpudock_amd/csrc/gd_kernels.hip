@@ -224,7 +224,7 @@ __device__ __forceinline__ void gd_energy_wave(
 
   /* KDE (calcenergy_d.cu:175-213) over the type-grouped segment */
   const float kde2 = P.kde2;
-  float ekde = 0.0f;
+  float ekde = 0.0f, ekde_lane = 0.0f;
   for (int l = 0; l < lna; ++l) {
     const int src = l & (GD_WAVE - 1);
     const bool hi = l >= GD_WAVE;
@@ -259,11 +259,11 @@ __device__ __forceinline__ void gd_energy_wave(
                                              kde_rsrc, ulane * 16u, i0 * 16, 0));
       });
     }
-    v = gd_wave_sum(v);
-    if (s1 > s0)
-      ekde += v * gd_uload(P.kde_inv_n + row);
+    /* the per-atom mean is linear: lanes keep their share across atoms and
+     * the wavefront sums once */
+    ekde_lane = gd_fma(v, gd_uload(P.kde_inv_n + row), ekde_lane);
   }
-  ekde = ekde / P.kde3;
+  ekde = gd_wave_sum(ekde_lane) / P.kde3;
 
   /* position restraints (calcenergy_d.cu:229-270) */
   float elhm = 0.0f;

@@ -644,7 +644,9 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     ehpc += 0.5f * z * z - ep->hpl2[lt];
   }
 
-  float ekde = 0.0f;
+  float ekde = 0.0f, ekde_lane[W];
+  for (int j = 0; j < W; ++j)
+    ekde_lane[j] = 0.0f;
   for (int l = 0; l < lna; ++l) {
     const int lt = c->lig->t[l];
     const int s0 = c->kseg[lt], s1 = c->kseg[lt + 1];
@@ -659,12 +661,12 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         v[j] = gd_fma(e, es, v[j]);
       }
     }
-    const float vs = bfly(v);
     const int n = s1 - s0;
-    if (n > 0)
-      ekde += vs * (1.0f / (float) n);
+    const float inv = n > 0 ? 1.0f / (float) n : 0.0f;
+    for (int j = 0; j < W; ++j)
+      ekde_lane[j] = gd_fma(v[j], inv, ekde_lane[j]);
   }
-  ekde = ekde / ep->kde3;
+  ekde = bfly(ekde_lane) / ep->kde3;
 
   float elhm = 0.0f;
   for (int m = 0; m < sz.pos; ++m) {

@@ -97,3 +97,13 @@ def test_no_gpu_is_an_error_not_a_fallback():
         remc.run(cx, temps, mc)
     with pytest.raises(remc.GDError, match="no HIP device"):
         remc.score_poses(cx, [0], [0], np.zeros((1, 6)))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
