@@ -26,7 +26,10 @@
  * the smaller loops); the ligand atom of the outer loop is broadcast from
  * the lane that holds it with v_readlane, so it sits in scalar registers.
  * The replica's uniform state (movematrices, energies) lives in a per-wave
- * LDS slot, not in vector registers.  Per-term sums are per-lane FP32
+ * LDS slot, not in vector registers; read-only uniform data (the task, pack
+ * fields, each atom's table row, type and charge) is read through the
+ * scalar unit, and the per-atom PSP rows and HBM-resident KDE points through
+ * raw buffer resources with a fixed lane offset and a scalar chunk offset.  Per-term sums are per-lane FP32
  * accumulators combined with the near-first pairing tree the CPU replay
  * reproduces bit for bit.  The step index of each complex lives in device
  * memory, so the three launches of a cycle (MC, exchange, advance) replay

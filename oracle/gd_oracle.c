@@ -370,14 +370,13 @@ ORC_EXPORT void orc_pose_ref32(const gd_ligand *lig, const gd_protein *prt,
  * to the GPU's.  Runs steps [0, n_steps) from the loaded pose (IS_AWAY 0);
  * exchange after every steps_per_exchange steps when n_tmp > 1 and
  * exchange != 0.  acs: [n_rep] accepted moves; e_final: [n_rep * 10]. */
-ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
-                           const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
-                           const gd_enepara *ep, const gd_temp *temp,
-                           gd_complex_size sz, const float move_scale[6], int n_steps,
-                           int steps_per_exchange, int exchange, uint32_t seed,
-                           uint32_t cid, int term_order, int n_threads, int *acs,
-                           int *acs_x, int *idx_tmp_out, float *e_final,
-                           float *mv_final)
+static int ref_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
+                   const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *ep,
+                   const gd_temp *temp, gd_complex_size sz, const float move_scale[6],
+                   int n_steps, int steps_per_exchange, int exchange, uint32_t seed,
+                   uint32_t cid, int term_order, int n_threads, int *acs, int *acs_x,
+                   int *idx_tmp_out, float *e_final, float *mv_final, float *e_samp,
+                   int *t_samp, int samp_stride, int n_samp)
 {
   const int n = sz.n_rep, lna = sz.lna;
   float *mv = (float *) calloc((size_t) n * 6, sizeof(float));
@@ -412,6 +411,11 @@ ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
       float *mo = mv + (size_t) r * 6, *eo = e + (size_t) r * 10;
       const float mb = temp[it[r]].minus_beta;
       for (int s = s0; s < s1; ++s) {
+        if (e_samp && s % samp_stride == 0 && s / samp_stride < n_samp) {
+          /* the state before step s, as the per-step records hold it */
+          e_samp[(size_t) r * n_samp + s / samp_stride] = eo[9];
+          t_samp[(size_t) r * n_samp + s / samp_stride] = it[r];
+        }
         const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
         const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
         const uint32_t w[7] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3], rb.v[0], rb.v[1], rb.v[2]};
@@ -476,6 +480,39 @@ ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
   free(e);
   free(it);
   return 0;
+}
+
+ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
+                           const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                           const gd_enepara *ep, const gd_temp *temp,
+                           gd_complex_size sz, const float move_scale[6], int n_steps,
+                           int steps_per_exchange, int exchange, uint32_t seed,
+                           uint32_t cid, int term_order, int n_threads, int *acs,
+                           int *acs_x, int *idx_tmp_out, float *e_final,
+                           float *mv_final)
+{
+  return ref_run(lig, prt, psp, kde, mcs, ep, temp, sz, move_scale, n_steps,
+                 steps_per_exchange, exchange, seed, cid, term_order, n_threads, acs, acs_x,
+                 idx_tmp_out, e_final, mv_final, NULL, NULL, 1, 0);
+}
+
+/* orc_ref_run plus samples of the trajectory: for replica r, the total
+ * energy and temperature index of the state before every samp_stride-th
+ * step, e_samp/t_samp[r * n_samp + k] for step k * samp_stride. */
+ORC_EXPORT int orc_ref_run_samples(const gd_ligand *lig, const gd_protein *prt,
+                                   const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                                   const gd_enepara *ep, const gd_temp *temp,
+                                   gd_complex_size sz, const float move_scale[6],
+                                   int n_steps, int steps_per_exchange, int exchange,
+                                   uint32_t seed, uint32_t cid, int term_order,
+                                   int n_threads, int *acs, float *e_samp, int *t_samp,
+                                   int samp_stride, int n_samp)
+{
+  if (samp_stride < 1)
+    return -1;
+  return ref_run(lig, prt, psp, kde, mcs, ep, temp, sz, move_scale, n_steps,
+                 steps_per_exchange, exchange, seed, cid, term_order, n_threads, acs, NULL,
+                 NULL, NULL, NULL, e_samp, t_samp, samp_stride, n_samp);
 }
 
 /* ====================================================================== */

@@ -40,6 +40,10 @@ def lib():
         L.orc_ref_run.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP, C.c_int,
                                         C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int,
                                         C.c_int, IP, IP, IP, FP, FP]
+        L.orc_ref_run_samples.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP,
+                                                C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                                C.c_uint32, C.c_int, C.c_int, IP, FP, IP,
+                                                C.c_int, C.c_int]
         L.orc_bayes_pdf.argtypes = [C.c_int, C.c_int, FP, C.c_float, C.c_float, FP]
         L.orc_bayes_total.argtypes = [FP]
         L.orc_bayes_total.restype = C.c_float
@@ -122,6 +126,24 @@ def ref_run(cx, temps, scale, n_steps, spe, exchange=1, seed=1, cid=0, term_orde
     lib().orc_ref_run(*cx.args(), temps, size, _fp(sc), n_steps, spe, exchange, seed, cid,
                       term_order, threads, _ip(acs), _ip(acx), _ip(it), _fp(e), _fp(mv))
     return dict(acs_mc=acs, acs_exchg=acx, idx_tmp_final=it, energy_final=e, move_final=mv)
+
+
+def ref_run_samples(cx, temps, scale, n_steps, spe, stride, exchange=0, seed=1, cid=0,
+                    term_order=0, threads=0):
+    """Reference-order run sampled every `stride` steps:
+    (acs_mc [n_rep], energies [n_rep, n_samp], temperature index [n_rep, n_samp])."""
+    size = cx.sized(len(temps))
+    n = size.n_rep
+    n_samp = (n_steps + stride - 1) // stride
+    acs = np.zeros(n, np.int32)
+    es = np.zeros((n, n_samp), np.float32)
+    ts = np.zeros((n, n_samp), np.int32)
+    sc = np.ascontiguousarray(scale, np.float32)
+    rc = lib().orc_ref_run_samples(*cx.args(), temps, size, _fp(sc), n_steps, spe, exchange,
+                                   seed, cid, term_order, threads, _ip(acs), _fp(es), _ip(ts),
+                                   stride, n_samp)
+    assert rc == 0
+    return acs, es, ts
 
 
 def philox(ctr, key):
