@@ -221,3 +221,27 @@ def test_g1_and_run_from_input_files():
     g, _ = remc.run(cx, temps, mc, opt)
     r = O.replay_run(cx, temps, mc, opt)
     _compare_runs(g, r)
+
+
+def test_c4_trajectory_poses_against_fp64(tmp_path):
+    """The large-pocket stress shape (pocket 4,000, ligand 128, 5,000 KDE points,
+    32 temperatures; 8 of the 128 ligand conformers): the kernel's FP32 energies
+    of every 100th pose along the trajectories, per term, against the GPU FP64
+    re-score of the same poses."""
+    from gpudock_amd import traj
+    cx, temps, mc, cfg = synth.make_config("c4", n_lig=8, steps=300)
+    mc.steps_per_dump = 300
+    prefix = str(tmp_path / "c4")
+    opt = remc.default_options(seed=3, record_stride=100, record_prefix=prefix)
+    remc.run(cx, temps, mc, opt)
+    _, rec = traj.read_gdtraj(prefix + "_0000.gdtraj")
+    rec = rec.reshape(-1)
+    lc = rec["idx_lig"].astype(np.int32)
+    pc = rec["idx_prt"].astype(np.int32)
+    mv = np.ascontiguousarray(rec["movematrix"], np.float32)
+    e32 = rec["energy"][:, :10]
+    e, e64 = remc.score_poses(cx, lc, pc, mv)
+    np.testing.assert_array_equal(e, e32)      # the same kernel arithmetic re-run
+    tol = tolerance(cx, e64, 0)
+    assert np.all(np.abs(e32 - e64) <= tol), np.max(np.abs(e32 - e64) / tol)
+    assert len(rec) == 8 * 32 * 3
