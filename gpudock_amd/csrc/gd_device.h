@@ -54,7 +54,7 @@ typedef struct gd_pack {
   /* LDS carve-up for this complex (bytes) */
   int off_eh, off_tab, off_kde, lds_bytes;
   int kde_in_lds;
-  int pad0;
+  int pocket_in_lds;     /* 0: pocket points read through the caches */
 
   /* pocket, per protein conformer: [n_prt][pnp_pad]; points pnp..pnp_pad-1
    * are padding that contributes exactly zero to every term */

@@ -281,12 +281,21 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   /* LDS carve-up */
   auto al16 = [](int v) { return (v + 15) & ~15; };
   /* pocket points staged as two planes, positions + type (16 B) and
-   * ele + hpp (8 B): contiguous per lane, so free of bank conflicts */
+   * ele + hpp (8 B): contiguous per lane, so free of bank conflicts; a
+   * pocket too large for LDS beside the table rows is read through the
+   * caches instead (whole-protein pockets) */
+  const int reserve = 16 * GD_WSLOT_FLOATS * 4 + 64;   /* wave slots of up to 16 waves */
+  const int tab_bytes = P.n_rows * GD_ROW_STRIDE * (int) sizeof(gd_pair_param);
+  P.pocket_in_lds = 1;
   P.off_eh = pnp_pad * 16;
   P.off_tab = al16(P.off_eh + pnp_pad * 8);
-  P.off_kde = al16(P.off_tab + P.n_rows * GD_ROW_STRIDE * (int) sizeof(gd_pair_param));
+  if (P.off_tab + tab_bytes + reserve > max_lds) {
+    P.pocket_in_lds = 0;
+    P.off_eh = 0;
+    P.off_tab = 0;
+  }
+  P.off_kde = al16(P.off_tab + tab_bytes);
   const int with_kde = al16(P.off_kde + P.pnk * 16);
-  const int reserve = 16 * GD_WSLOT_FLOATS * 4 + 64;   /* wave slots of up to 16 waves */
   if (kde_policy != GD_KDE_GLOBAL && with_kde + reserve <= max_lds) {
     P.kde_in_lds = 1;
     P.lds_bytes = with_kde;
@@ -295,8 +304,8 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     P.lds_bytes = P.off_kde;
   }
   if (P.lds_bytes + reserve > max_lds)
-    return gd_fail(GD_ERR_ARG, "pocket of %d points needs %d B of LDS (device has %d)",
-                   sz.pnp, P.lds_bytes + reserve, max_lds);
+    return gd_fail(GD_ERR_ARG, "%d ligand atom types need %d B of LDS (device has %d)",
+                   P.n_rows, P.lds_bytes + reserve, max_lds);
   return GD_OK;
 }
 

@@ -44,6 +44,7 @@
 #define GD_BUFFER_DWORD3 0x00020000
 
 typedef float gd_f4 __attribute__((ext_vector_type(4)));
+typedef float gd_f2 __attribute__((ext_vector_type(2)));
 
 template <bool B>
 struct gd_bool {
@@ -106,11 +107,12 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
   const int pnp_pad = P.pnp_pad;
   const float4 *g_pos = P.pt_pos + (size_t) prt * pnp_pad;
   const float2 *g_eh = P.pt_eh + (size_t) prt * pnp_pad;
-  for (int i = threadIdx.x; i < pnp_pad; i += blockDim.x) {
-    const float4 q = g_pos[i];
-    s_pos[i] = q;
-    s_eh[i] = g_eh[i];
-  }
+  if (P.pocket_in_lds)
+    for (int i = threadIdx.x; i < pnp_pad; i += blockDim.x) {
+      const float4 q = g_pos[i];
+      s_pos[i] = q;
+      s_eh[i] = g_eh[i];
+    }
   const float4 *g_tab = (const float4 *) P.tab;
   const int ntab4 = P.n_rows * GD_ROW_STRIDE * 2;
   for (int i = threadIdx.x; i < ntab4; i += blockDim.x)
@@ -164,6 +166,11 @@ __device__ __forceinline__ void gd_energy_wave(
   const gfloat_ptr psp_prt = (gfloat_ptr) (P.psp_pt + (size_t) prt * n_rows * P.pnp_pad);
   const int kh = gd_uload(P.hpc_chunks + prt);
   const unsigned ulane = (unsigned) lane;
+  const int pocket_lds = P.pocket_in_lds;
+  const __amdgpu_buffer_rsrc_t pos_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *) (P.pt_pos + (size_t) prt * P.pnp_pad), 0, P.pnp_pad * 16, GD_BUFFER_DWORD3);
+  const __amdgpu_buffer_rsrc_t eh_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *) (P.pt_eh + (size_t) prt * P.pnp_pad), 0, P.pnp_pad * 8, GD_BUFFER_DWORD3);
 
   /* pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) */
   for (int l = 0; l < lna; ++l) {
@@ -183,11 +190,11 @@ __device__ __forceinline__ void gd_energy_wave(
     /* chunks [0, kh) hold the hydrophobic-class points (the canonical order
      * puts them first); beyond them every hydrophobic weight is 0, whose
      * term would add exact zeros, so that loop leaves the term out */
-    auto chunks = [&](auto hpc_tag, auto hdb_tag, int k0, int k1) {
+    auto chunks = [&](auto read, auto hpc_tag, auto hdb_tag, int k0, int k1) {
       for (int k = k0; k < k1; ++k) {
-        const int p = lane + k * GD_WAVE;
-        const float4 q = s_pos[p];
-        const float2 eh = s_eh[p];
+        float4 q;
+        float2 eh;
+        read(k, q, eh);
         const uint32_t bits = __float_as_uint(q.w);
         /* the PSP read (chunk base uniform, lane offset fixed) is consumed
          * at the end of the pair; points outside class 2 hold PSP 0 */
@@ -211,12 +218,31 @@ __device__ __forceinline__ void gd_energy_wave(
       }
     };
     /* rows of ligand types without any H-bond pair leave that test out */
-    if (gd_uload(P.row_hdb + row)) {
-      chunks(gd_bool<true>(), gd_bool<true>(), 0, kh);
-      chunks(gd_bool<false>(), gd_bool<true>(), kh, nchunk);
+    auto rows = [&](auto read) {
+      if (gd_uload(P.row_hdb + row)) {
+        chunks(read, gd_bool<true>(), gd_bool<true>(), 0, kh);
+        chunks(read, gd_bool<false>(), gd_bool<true>(), kh, nchunk);
+      } else {
+        chunks(read, gd_bool<true>(), gd_bool<false>(), 0, kh);
+        chunks(read, gd_bool<false>(), gd_bool<false>(), kh, nchunk);
+      }
+    };
+    if (pocket_lds) {
+      rows([&](int k, float4 &q, float2 &eh) {
+        q = s_pos[lane + k * GD_WAVE];
+        eh = s_eh[lane + k * GD_WAVE];
+      });
     } else {
-      chunks(gd_bool<true>(), gd_bool<false>(), 0, kh);
-      chunks(gd_bool<false>(), gd_bool<false>(), kh, nchunk);
+      /* whole-protein pockets: the conformer's points as raw buffers, lane
+       * offset fixed, chunk offset scalar */
+      rows([&](int k, float4 &q, float2 &eh) {
+        const gd_f4 v = __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                      pos_rsrc, ulane * 16u, k * GD_WAVE * 16, 0));
+        const gd_f2 w = __builtin_bit_cast(gd_f2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                      eh_rsrc, ulane * 8u, k * GD_WAVE * 8, 0));
+        q = make_float4(v.x, v.y, v.z, v.w);
+        eh = make_float2(w.x, w.y);
+      });
     }
     eele = gd_fma(lq, acc.ele, eele);
     const int lt = gd_uload(P.lig_type + l);

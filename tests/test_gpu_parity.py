@@ -245,3 +245,24 @@ def test_c4_trajectory_poses_against_fp64(tmp_path):
     tol = tolerance(cx, e64, 0)
     assert np.all(np.abs(e32 - e64) <= tol), np.max(np.abs(e32 - e64) / tol)
     assert len(rec) == 8 * 32 * 3
+
+
+def test_whole_protein_pocket_read_through_caches():
+    """A pocket of 7,000 points (168 KB of points, more than LDS holds beside
+    the table rows) is read through the caches: poses and a short run equal
+    the replay."""
+    cx = synth.make_complex(pnp=7000, radius=None, lna=20, n_lig=2, n_prt=2, pnk=300, pos=2,
+                            seed=21)
+    rng = np.random.default_rng(21)
+    lc, pc, mv = _poses(rng, 24, 2, 2, tsig=0.4, rsig=0.6)
+    _check_pose_parity(cx, lc, pc, mv)
+    temps = remc.temperatures(2, 0.3, 3.0)
+    mc = remc.mcpara(30, 10, [0.2, 0.2, 0.2, 0.05, 0.05, 0.05])
+    opt = remc.default_options(seed=4, mcc=1)
+    s = remc.Session(cx, temps, mc, opt)
+    s.init()
+    assert s.layout()["lds_bytes"] < 7000 * 24
+    s.close()
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
