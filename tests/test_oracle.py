@@ -159,6 +159,20 @@ def test_canonical_rsqrt_rcp_accuracy():
     assert np.max(_ulp_err(O.canon_math(5, x), 1.0 / xd)) < 2.0
 
 
+def test_canonical_rsqrt_exhaustive_over_two_binades():
+    """Every float in [1, 4): the seed and the step depend only on the
+    mantissa and the exponent's parity, so this covers every positive
+    normal input up to the power-of-two scaling."""
+    x = np.arange(0x3F800000, 0x40800000, dtype=np.uint32).view(np.float32)
+    err = _ulp_err(O.canon_math(4, x), 1.0 / np.sqrt(x.astype(np.float64)))
+    assert err.max() < 1.2
+    assert np.mean(err > 1.0) < 0.002
+    # scaling by 4 scales the result by exactly 1/2
+    xs = x[::4099]
+    assert np.array_equal(O.canon_math(4, xs * np.float32(4.0)) * np.float32(2.0),
+                          O.canon_math(4, xs))
+
+
 # ---------------------------------------------------------- closed forms
 def _single_pair_complex(d, lig_t=3, prt_t=5, cls=2, hpcflag=1, q=0.4, ele=-0.3,
                          psp_val=-0.7, kde_at=None, mcs_at=None):
