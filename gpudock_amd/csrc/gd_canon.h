@@ -425,8 +425,11 @@ GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
   acc->vdw = gd_fma((p1 - p2) * q1, rr, acc->vdw);
   acc->pmf = gd_fma(pp->pmf1 * p4, rr, acc->pmf);
 
-  /* pocket-specific (adding 0 leaves the sum's bits unchanged) */
-  acc->psp += (is_c2 && dm <= 0.0f) ? psp_val : 0.0f;
+  /* pocket-specific; is_c2 = 0 promises psp_val = 0 for every point of the
+   * call (a sum that starts at +0 never becomes -0, so adding 0 leaves its
+   * bits unchanged and may be left out) */
+  if (is_c2)
+    acc->psp += (dm <= 0.0f) ? psp_val : 0.0f;
 
   /* hydrogen bond */
   if (hdb_row && pp->hdb0 > 0.1f) {

@@ -77,9 +77,11 @@ typedef struct gd_pack {
   /* PSP value per (protein conformer, table row, point):
    * [n_prt][n_rows][pnp_pad], 0 outside class 2 and for padding */
   const float *psp_pt;
-  /* [n_prt] chunks of 64 points, from the first, that hold every point of
-   * the hydrophobic class (the canonical order puts them first) */
-  const int *hpc_chunks;
+  /* [n_prt][2]: kh = chunks of 64 points, from the first, that hold every
+   * point of the hydrophobic class (the canonical order puts them first);
+   * kp >= kh = chunks from the first that hold every class-2 point, the
+   * only points with a PSP value */
+  const int *chunk_bounds;
   /* [n_rows] 1 if some point type pairs with the row's ligand type through
    * an H-bond term (hdb0 > 0.1) */
   const int *row_hdb;

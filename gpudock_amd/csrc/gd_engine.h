@@ -29,7 +29,7 @@ struct GdImage {
   std::vector<float4> mcs;
   std::vector<float> tcc;
   std::vector<float> psp_pt;     /* [n_prt][n_rows][pnp_pad] */
-  std::vector<int> hpc_chunks;   /* [n_prt] */
+  std::vector<int> chunk_bounds; /* [n_prt][2]: kh, kp */
   std::vector<int> row_hdb;      /* [n_rows] */
   std::vector<float> hpl;
   std::vector<float> mb;
@@ -42,7 +42,7 @@ struct GdImage {
  * as one copy. */
 struct GdBlock {
   size_t pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_seg, kde_inv, mcs,
-      tcc, psp_pt, hpc_chunks, row_hdb, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
+      tcc, psp_pt, chunk_bounds, row_hdb, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
       acs_x, cmcc, cmcc_best, step, pack, rec, rec_bytes, mcc_ref, total;
 };
 

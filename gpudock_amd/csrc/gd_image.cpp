@@ -118,7 +118,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   const uint32_t hdb_mask = gd_hdb_point_mask(&ep->hdb0[0][0], GD_MAXTP2, GD_MAXTP1);
   std::vector<int> perm(sz.pnp);
   std::vector<int> perms((size_t) sz.n_prt * sz.pnp);   /* sorted slot -> point */
-  I.hpc_chunks.assign(sz.n_prt, 0);
+  I.chunk_bounds.assign((size_t) 2 * sz.n_prt, 0);
   for (int cf = 0; cf < sz.n_prt; ++cf) {
     const size_t base = (size_t) cf * sz.pnp;
     const size_t obase = (size_t) cf * pnp_pad;
@@ -132,10 +132,19 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     });
     for (int j = 0; j < sz.pnp; ++j)
       perms[(size_t) cf * sz.pnp + j] = perm[j];
-    int n_hpc = 0;
-    for (int j = 0; j < sz.pnp; ++j)
+    int n_hpc = 0, end_c2 = 0;
+    for (int j = 0; j < sz.pnp; ++j) {
       n_hpc += prt->c0_and_d12_or_c2[base + j] != 0;
-    I.hpc_chunks[cf] = (n_hpc + GD_WAVE - 1) / GD_WAVE;
+      if (prt->c[base + perm[j]] == 2)
+        end_c2 = j + 1;
+    }
+    /* [0, kh) holds every hydrophobic-class point; [0, kp) every class-2
+     * point (the only ones with a PSP value); class 2 implies the
+     * hydrophobic class in every input the loaders build, so kp = kh there */
+    const int kh = (n_hpc + GD_WAVE - 1) / GD_WAVE;
+    const int kc = (end_c2 + GD_WAVE - 1) / GD_WAVE;
+    I.chunk_bounds[2 * cf] = kh;
+    I.chunk_bounds[2 * cf + 1] = kc > kh ? kc : kh;
     for (int j = 0; j < sz.pnp; ++j) {
       const size_t i = base + perm[j];
       const uint32_t bits = (uint32_t) prt->t[i] << GD_PT_TYPE_SHIFT;
@@ -332,7 +341,7 @@ GdBlock gd_layout_block(const GdImage &I)
   b.mcs = take(vbytes(I.mcs));
   b.tcc = take(vbytes(I.tcc));
   b.psp_pt = take(vbytes(I.psp_pt));
-  b.hpc_chunks = take(vbytes(I.hpc_chunks));
+  b.chunk_bounds = take(vbytes(I.chunk_bounds));
   b.row_hdb = take(vbytes(I.row_hdb));
   b.hpl = take(vbytes(I.hpl));
   b.mb = take(vbytes(I.mb));
@@ -386,7 +395,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   put(b.mcs, I.mcs.data(), vbytes(I.mcs));
   put(b.tcc, I.tcc.data(), vbytes(I.tcc));
   put(b.psp_pt, I.psp_pt.data(), vbytes(I.psp_pt));
-  put(b.hpc_chunks, I.hpc_chunks.data(), vbytes(I.hpc_chunks));
+  put(b.chunk_bounds, I.chunk_bounds.data(), vbytes(I.chunk_bounds));
   put(b.row_hdb, I.row_hdb.data(), vbytes(I.row_hdb));
   put(b.hpl, I.hpl.data(), vbytes(I.hpl));
   put(b.mb, I.mb.data(), vbytes(I.mb));
@@ -408,7 +417,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.mcs = (const float4 *) (dev + b.mcs);
   P.mcs_tcc = (const float *) (dev + b.tcc);
   P.psp_pt = (const float *) (dev + b.psp_pt);
-  P.hpc_chunks = (const int *) (dev + b.hpc_chunks);
+  P.chunk_bounds = (const int *) (dev + b.chunk_bounds);
   P.row_hdb = (const int *) (dev + b.row_hdb);
   P.hpl0 = (const float *) (dev + b.hpl);
   P.hpl1 = P.hpl0 + GD_MAXTP2;

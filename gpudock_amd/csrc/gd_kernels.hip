@@ -164,7 +164,8 @@ __device__ __forceinline__ void gd_energy_wave(
   int tp = 0, fp = 0, fn = 0;
   typedef const __attribute__((address_space(1))) float *gfloat_ptr;
   const gfloat_ptr psp_prt = (gfloat_ptr) (P.psp_pt + (size_t) prt * n_rows * P.pnp_pad);
-  const int kh = gd_uload(P.hpc_chunks + prt);
+  const int kh = gd_uload(P.chunk_bounds + 2 * prt);
+  const int kp = gd_uload(P.chunk_bounds + 2 * prt + 1);
   const unsigned ulane = (unsigned) lane;
   const int pocket_lds = P.pocket_in_lds;
   const __amdgpu_buffer_rsrc_t pos_rsrc = __builtin_amdgcn_make_buffer_rsrc(
@@ -188,9 +189,10 @@ __device__ __forceinline__ void gd_energy_wave(
     float hacc = 0.0f;
     acc.ele = 0.0f;
     /* chunks [0, kh) hold the hydrophobic-class points (the canonical order
-     * puts them first); beyond them every hydrophobic weight is 0, whose
-     * term would add exact zeros, so that loop leaves the term out */
-    auto chunks = [&](auto read, auto hpc_tag, auto hdb_tag, int k0, int k1) {
+     * puts them first) and [0, kp) the class-2 points; beyond them every
+     * hydrophobic weight and every PSP value is 0, whose terms would add
+     * exact zeros, so those loops leave the terms (and the PSP read) out */
+    auto chunks = [&](auto read, auto hpc_tag, auto hdb_tag, auto psp_tag, int k0, int k1) {
       for (int k = k0; k < k1; ++k) {
         float4 q;
         float2 eh;
@@ -198,16 +200,18 @@ __device__ __forceinline__ void gd_energy_wave(
         const uint32_t bits = __float_as_uint(q.w);
         /* the PSP read (chunk base uniform, lane offset fixed) is consumed
          * at the end of the pair; points outside class 2 hold PSP 0 */
-        const float pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-            psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
+        float pv = 0.0f;
+        if (decltype(psp_tag)::value)
+          pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+              psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
         const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
         if (decltype(hpc_tag)::value && d2 <= 81.0f)
           hacc += gd_hpc_pair(d2, eh.y);
         const gd_pair_param pp =
             *(const gd_pair_param *) ((const char *) trow + bits);
         const int near_any = __ballot(d2 < sc_v.dcut2) != 0;
-        const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, 1, pv,
-                                          decltype(hdb_tag)::value, near_any, &acc);
+        const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, decltype(psp_tag)::value,
+                                          pv, decltype(hdb_tag)::value, near_any, &acc);
         if (mref) {
           const unsigned long long now = __ballot(contact);
           const unsigned long long ref = mref[(size_t) l * nchunk + k];
@@ -219,12 +223,16 @@ __device__ __forceinline__ void gd_energy_wave(
     };
     /* rows of ligand types without any H-bond pair leave that test out */
     auto rows = [&](auto read) {
+      const gd_bool<true> on;
+      const gd_bool<false> off;
       if (gd_uload(P.row_hdb + row)) {
-        chunks(read, gd_bool<true>(), gd_bool<true>(), 0, kh);
-        chunks(read, gd_bool<false>(), gd_bool<true>(), kh, nchunk);
+        chunks(read, on, on, on, 0, kh);
+        chunks(read, off, on, on, kh, kp);
+        chunks(read, off, on, off, kp, nchunk);
       } else {
-        chunks(read, gd_bool<true>(), gd_bool<false>(), 0, kh);
-        chunks(read, gd_bool<false>(), gd_bool<false>(), kh, nchunk);
+        chunks(read, on, off, on, 0, kh);
+        chunks(read, off, off, on, kh, kp);
+        chunks(read, off, off, off, kp, nchunk);
       }
     };
     if (pocket_lds) {

@@ -266,3 +266,24 @@ def test_whole_protein_pocket_read_through_caches():
     g, _ = remc.run(cx, temps, mc, opt)
     r = O.replay_run(cx, temps, mc, opt)
     _compare_runs(g, r)
+
+
+def test_class2_points_outside_the_hydrophobic_class():
+    """Class-2 points (the only ones with a PSP value) whose hydrophobic flag
+    is clear sort after the hydrophobic chunks, so the chunks that read PSP
+    run past them (the loaders never build such input: class 2 implies the
+    hydrophobic class).  Poses and a short run equal the replay."""
+    cx = synth.make_complex(pnp=300, radius=12.0, lna=20, n_lig=2, n_prt=2, pnk=200, pos=2,
+                            seed=23)
+    pc, pj = np.nonzero(cx.prt_c == 2)
+    assert len(pc) > 40 and np.all(cx.prt_hpcflag[pc, pj] != 0)
+    cx.prt_hpcflag[pc[::2], pj[::2]] = 0          # the struct points at this array
+    rng = np.random.default_rng(23)
+    lc, pcf, mv = _poses(rng, 64, 2, 2, tsig=0.4, rsig=0.6)
+    _check_pose_parity(cx, lc, pcf, mv)
+    temps = remc.temperatures(2, 0.3, 3.0)
+    mc = remc.mcpara(30, 10, [0.2, 0.2, 0.2, 0.05, 0.05, 0.05])
+    opt = remc.default_options(seed=5, mcc=1)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
