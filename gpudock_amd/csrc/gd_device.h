@@ -42,6 +42,7 @@ typedef struct gd_pack {
   int term_order;
   int combine;           /* total: 0 weighted sum, 1 Bayesian */
   int pnp_pad;           /* pnp rounded up to a multiple of 64 */
+  int kde_n_lo, kde_n_hi;/* (atom, KDE point) pairs of atoms < 64 and >= 64 */
 
   /* run control */
   int steps_total;       /* MC steps of the job */
@@ -67,8 +68,13 @@ typedef struct gd_pack {
   const int *lig_row;    /* [lna] table row of each atom */
   const int *lig_type;   /* [lna] atom type */
   const float *lig_q;    /* [lna] charge */
-  /* KDE points grouped by table row; row r spans [kde_seg[2r], kde_seg[2r+1]) */
+  /* KDE points grouped by table row; row r spans [kde_seg[2r], kde_seg[2r+1]);
+   * (x, y, z, 1 / that segment's length) */
   const float4 *kde;
+  /* (atom, KDE point) pairs of a pose: atoms 0..63, each atom's segment in
+   * order, padded with zeros to a multiple of 64; then atoms 64.. the same
+   * way.  Entry = point << 16 | (atom & 63) << 2 */
+  const uint32_t *kde_pairs;
   const int *kde_seg;
   const float *kde_inv_n;/* [n_rows] 1.0f / (float) segment length (0 if empty) */
   /* position restraints: targets per (m, atom) [pos][lna] (x, y, z, valid) */

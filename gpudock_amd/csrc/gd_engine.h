@@ -24,6 +24,7 @@ struct GdImage {
   std::vector<int> lig_row, lig_type;
   std::vector<float> lig_q;
   std::vector<float4> kde;
+  std::vector<uint32_t> kde_pairs;
   std::vector<int> kde_seg;
   std::vector<float> kde_inv;
   std::vector<float4> mcs;
@@ -41,7 +42,7 @@ struct GdImage {
  * state [mv, step) is contiguous, so it is saved, restored and read back
  * as one copy. */
 struct GdBlock {
-  size_t pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_seg, kde_inv, mcs,
+  size_t pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_pairs, kde_seg, kde_inv, mcs,
       tcc, psp_pt, chunk_bounds, row_hdb, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
       acs_x, cmcc, cmcc_best, step, pack, rec, rec_bytes, mcc_ref, total;
 };

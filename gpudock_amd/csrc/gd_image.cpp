@@ -173,11 +173,36 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     I.kde_seg[2 * r + 1] = (int) I.kde.size();
   }
   P.pnk = (int) I.kde.size();
+  if (P.pnk > 0xffff)
+    return gd_fail(GD_ERR_ARG, "%d KDE points of the ligand's atom types (at most 65535)",
+                   P.pnk);
   I.kde_inv.resize(P.n_rows);
   for (int r = 0; r < P.n_rows; ++r) {
     const int n = I.kde_seg[2 * r + 1] - I.kde_seg[2 * r];
     I.kde_inv[r] = n > 0 ? 1.0f / (float) n : 0.0f;
+    /* the weight of a pair rides with its KDE point: every point of the
+     * segment pairs with atoms of this row only */
+    for (int k = I.kde_seg[2 * r]; k < I.kde_seg[2 * r + 1]; ++k)
+      I.kde[k].w = I.kde_inv[r];
   }
+  /* the pose's (atom, KDE point) pairs, atoms < 64 then atoms >= 64 (lane
+   * a holds atoms a and a + 64), each group padded to whole wavefronts */
+  I.kde_pairs.clear();
+  for (int g = 0; g < 2; ++g) {
+    const int la = g ? GD_WAVE : 0;
+    const int lb = g ? sz.lna : std::min(sz.lna, GD_WAVE);
+    const size_t start = I.kde_pairs.size();
+    for (int a = la; a < lb; ++a) {
+      const int r = I.lig_row[a];
+      for (int k = I.kde_seg[2 * r]; k < I.kde_seg[2 * r + 1]; ++k)
+        I.kde_pairs.push_back((uint32_t) k << 16 | (uint32_t) (a & (GD_WAVE - 1)) << 2);
+    }
+    const int n = (int) (I.kde_pairs.size() - start);
+    (g ? P.kde_n_hi : P.kde_n_lo) = n;
+    I.kde_pairs.resize(start + (size_t) (n + GD_WAVE - 1) / GD_WAVE * GD_WAVE, 0u);
+  }
+  if (I.kde_pairs.empty())
+    I.kde_pairs.push_back(0u);
 
   I.mcs.resize((size_t) sz.pos * sz.lna);
   I.tcc.resize(sz.pos);
@@ -336,6 +361,7 @@ GdBlock gd_layout_block(const GdImage &I)
   b.lig_type = take(vbytes(I.lig_type));
   b.lig_q = take(vbytes(I.lig_q));
   b.kde = take(vbytes(I.kde));
+  b.kde_pairs = take(vbytes(I.kde_pairs));
   b.kde_seg = take(vbytes(I.kde_seg));
   b.kde_inv = take(vbytes(I.kde_inv));
   b.mcs = take(vbytes(I.mcs));
@@ -390,6 +416,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   put(b.lig_type, I.lig_type.data(), vbytes(I.lig_type));
   put(b.lig_q, I.lig_q.data(), vbytes(I.lig_q));
   put(b.kde, I.kde.data(), vbytes(I.kde));
+  put(b.kde_pairs, I.kde_pairs.data(), vbytes(I.kde_pairs));
   put(b.kde_seg, I.kde_seg.data(), vbytes(I.kde_seg));
   put(b.kde_inv, I.kde_inv.data(), vbytes(I.kde_inv));
   put(b.mcs, I.mcs.data(), vbytes(I.mcs));
@@ -412,6 +439,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.lig_type = (const int *) (dev + b.lig_type);
   P.lig_q = (const float *) (dev + b.lig_q);
   P.kde = (const float4 *) (dev + b.kde);
+  P.kde_pairs = (const uint32_t *) (dev + b.kde_pairs);
   P.kde_seg = (const int *) (dev + b.kde_seg);
   P.kde_inv_n = (const float *) (dev + b.kde_inv);
   P.mcs = (const float4 *) (dev + b.mcs);

@@ -259,46 +259,57 @@ __device__ __forceinline__ void gd_energy_wave(
     ehpc += 0.5f * z * z - P.hpl2[lt];
   }
 
-  /* KDE (calcenergy_d.cu:175-213) over the type-grouped segment */
+  /* KDE (calcenergy_d.cu:175-213): sum over atoms of the mean of
+   * e^(kde2 d^2) over the KDE points of the atom's type, taken as one list of
+   * (atom, point) pairs, 64 to a round, so that no lane idles at the end of
+   * an atom's short segment.  Lane j of a round takes pair 64 t + j: its
+   * atom's coordinates come from the lane that holds the atom (atoms < 64
+   * from (ax0, ay0, az0), then atoms >= 64 from (ax1, ay1, az1)), the point
+   * carries the pair's weight 1 / (segment length) in .w, and each lane
+   * keeps its weighted sum across the whole list. */
   const float kde2 = P.kde2;
   float ekde = 0.0f, ekde_lane = 0.0f;
-  for (int l = 0; l < lna; ++l) {
-    const int src = l & (GD_WAVE - 1);
-    const bool hi = l >= GD_WAVE;
-    const float lx = gd_bcast(hi ? ax1 : ax0, src);
-    const float ly = gd_bcast(hi ? ay1 : ay0, src);
-    const float lz = gd_bcast(hi ? az1 : az0, src);
-    const int row = gd_uload(P.lig_row + l);
-    const int s0 = gd_uload(P.kde_seg + 2 * row);
-    const int s1 = gd_uload(P.kde_seg + 2 * row + 1);
-    float v = 0.0f;
-    /* wavefront-uniform trip count; lanes past the segment sit out the
-     * last round */
-    auto segment = [&](auto read) {
-      for (int i0 = s0; i0 < s1; i0 += GD_WAVE) {
-        if (lane < s1 - i0) {
-          const gd_f4 q = read(i0);   /* one 16-byte read per lane */
-          const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-          float es;
-          const float ep = gd_exp_hot_split(kde2 * d2, &es);
-          v = gd_fma(ep, es, v);   /* v + e^x: the product is exact */
-        }
+  {
+    const int n_lo = P.kde_n_lo, n_hi = P.kde_n_hi;
+    const int hi0 = (n_lo + GD_WAVE - 1) & ~(GD_WAVE - 1);
+    const int n_all = hi0 + ((n_hi + GD_WAVE - 1) & ~(GD_WAVE - 1));
+    const __amdgpu_buffer_rsrc_t pairs_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.kde_pairs, 0, n_all * 4, GD_BUFFER_DWORD3);
+    auto group = [&](auto read, float gx, float gy, float gz, int p0, int n) {
+      for (int p = 0; p < n; p += GD_WAVE) {
+        /* every lane runs the permutes (padding entries name lane 0 and
+         * point 0); only the lanes with a pair keep the sum */
+        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
+                                                                (p0 + p) * 4, 0);
+        const int src = (int) (e & 0xffu);
+        const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
+        const float ly = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gy)));
+        const float lz = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gz)));
+        const gd_f4 q = read(e >> 12);   /* byte offset of the point */
+        const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+        float es;
+        const float ep = gd_exp_hot_split(kde2 * d2, &es);
+        const float sum = gd_fma(ep * es, q.w, ekde_lane);   /* ep * 2^n is exact */
+        ekde_lane = (int) ulane < n - p ? sum : ekde_lane;
       }
     };
+    auto both = [&](auto read) {
+      group(read, ax0, ay0, az0, 0, n_lo);
+      group(read, ax1, ay1, az1, hi0, n_hi);
+    };
     if (kde_lds) {
-      const __attribute__((address_space(3))) gd_f4 *pts =
-          (const __attribute__((address_space(3))) gd_f4 *) (smem + P.off_kde);
-      segment([&](int i0) { return pts[i0 + lane]; });
+      const __attribute__((address_space(3))) char *pts =
+          (const __attribute__((address_space(3))) char *) (smem + P.off_kde);
+      both([&](uint32_t off) {
+        return *(const __attribute__((address_space(3))) gd_f4 *) (pts + off);
+      });
     } else {
-      /* from HBM as a raw buffer: lane offset fixed, segment offset scalar */
-      segment([&](int i0) {
+      /* from HBM as a raw buffer */
+      both([&](uint32_t off) {
         return __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(
-                                             kde_rsrc, ulane * 16u, i0 * 16, 0));
+                                             kde_rsrc, off, 0, 0));
       });
     }
-    /* the per-atom mean is linear: lanes keep their share across atoms and
-     * the wavefront sums once */
-    ekde_lane = gd_fma(v, gd_uload(P.kde_inv_n + row), ekde_lane);
   }
   ekde = gd_wave_sum(ekde_lane) / P.kde3;
 

@@ -681,27 +681,27 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     ehpc += 0.5f * z * z - ep->hpl2[lt];
   }
 
+  /* KDE as one list of (atom, point) pairs: atoms < W, each atom's points
+   * in segment order, then atoms >= W the same way; pair q of a group goes
+   * to lane q % W, weighted by 1 / (segment length) */
   float ekde = 0.0f, ekde_lane[W];
   for (int j = 0; j < W; ++j)
     ekde_lane[j] = 0.0f;
-  for (int l = 0; l < lna; ++l) {
-    const int lt = c->lig->t[l];
-    const int s0 = c->kseg[lt], s1 = c->kseg[lt + 1];
-    float v[W];
-    for (int j = 0; j < W; ++j) {
-      v[j] = 0.0f;
-      for (int i = s0 + j; i < s1; i += W)
-      {
+  for (int g = 0; g < 2; ++g) {
+    const int la = g ? W : 0, lb = g ? lna : (lna < W ? lna : W);
+    int q = 0;
+    for (int l = la; l < lb; ++l) {
+      const int lt = c->lig->t[l];
+      const int s0 = c->kseg[lt], s1 = c->kseg[lt + 1];
+      const int n = s1 - s0;
+      const float w = n > 0 ? 1.0f / (float) n : 0.0f;
+      for (int i = s0; i < s1; ++i, ++q) {
         float es;
         const float e = gd_exp_hot_split(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
                                                              az[l] - c->kz[i]), &es);
-        v[j] = gd_fma(e, es, v[j]);
+        ekde_lane[q % W] = gd_fma(e * es, w, ekde_lane[q % W]);
       }
     }
-    const int n = s1 - s0;
-    const float inv = n > 0 ? 1.0f / (float) n : 0.0f;
-    for (int j = 0; j < W; ++j)
-      ekde_lane[j] = gd_fma(v[j], inv, ekde_lane[j]);
   }
   ekde = bfly(ekde_lane) / ep->kde3;
 
