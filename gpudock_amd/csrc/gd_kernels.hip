@@ -51,6 +51,9 @@ struct gd_bool {
   static constexpr bool value = B;
 };
 
+/* The wave's slot in LDS, addressed as LDS (not through flat accesses). */
+typedef __attribute__((address_space(3))) float gd_lds_f32;
+
 /* A read of read-only, wave-uniform data through the scalar unit. */
 template <typename T>
 __device__ __forceinline__ T gd_uload(const T *p)
@@ -255,8 +258,8 @@ __device__ __forceinline__ void gd_energy_wave(
     eele = gd_fma(lq, acc.ele, eele);
     const int lt = gd_uload(P.lig_type + l);
     const float hs = gd_wave_sum(hacc);
-    const float z = (hs - P.hpl0[lt]) * P.inv_hpl1[lt];
-    ehpc += 0.5f * z * z - P.hpl2[lt];
+    const float z = (hs - gd_uload(P.hpl0 + lt)) * gd_uload(P.inv_hpl1 + lt);
+    ehpc += 0.5f * z * z - gd_uload(P.hpl2 + lt);
   }
 
   /* KDE (calcenergy_d.cu:175-213): sum over atoms of the mean of
@@ -336,7 +339,7 @@ __device__ __forceinline__ void gd_energy_wave(
     s = gd_wave_sum(s);
     c = gd_wave_sum(c);
     if (c > 0.0f)
-      elhm += P.mcs_tcc[m] * gd_sqrt(s / c);
+      elhm += gd_uload(P.mcs_tcc + m) * gd_sqrt(s / c);
   }
   elhm = (pos > 0) ? gd_log(elhm / (float) pos) : 0.0f;
 
@@ -447,7 +450,7 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
 /* n_steps MC steps from step s0.  The movematrices and energies live in
  * the wave's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in registers. */
 __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *smem,
-                                                 float *ws, int r, int prt, int s0,
+                                                 gd_lds_f32 *ws, int r, int prt, int s0,
                                                  int n_steps)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
@@ -573,7 +576,7 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
   extern __shared__ __attribute__((aligned(16))) char smem[];
   __shared__ int s_task;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / GD_WAVE);
-  float *ws = (float *) (smem + wslot_off) + w * GD_WSLOT_FLOATS;
+  gd_lds_f32 *ws = (gd_lds_f32 *) (smem + wslot_off) + w * GD_WSLOT_FLOATS;
   int staged_cx = -1, staged_prt = -1;
   int t = queue ? -1 : (int) blockIdx.x;
 
