@@ -84,6 +84,22 @@ def test_bad_inputs_are_rejected_without_a_gpu():
     assert rc == -1 and b"type" in remc.lib().gd_last_error()
 
 
+def test_kde_point_limit():
+    """KDE points of the ligand's atom types are indexed in 16 bits (the
+    reference holds at most MAXKDE = 10,000, size.h:73): more is an error
+    from the host image, before any device work."""
+    temps = remc.temperatures(1, 0.3, 0.3)
+    mc = remc.mcpara(10, 10, [0.1] * 6)
+    cx = synth.make_complex(pnp=100, radius=10.0, lna=8, n_lig=1, n_prt=1, pnk=200000, pos=0,
+                            seed=3)
+    assert np.isin(cx.kde_t, cx.lig_t).sum() > 65535
+    with pytest.raises(remc.GDError, match="KDE points"):
+        remc.JobSpec(cx, temps, mc).footprint()
+    cx = synth.make_complex(pnp=100, radius=10.0, lna=8, n_lig=1, n_prt=1, pnk=10000, pos=0,
+                            seed=3)
+    assert remc.JobSpec(cx, temps, mc).footprint() > 0
+
+
 def test_load_enepara_missing_file():
     ep0 = remc.EnePara0()
     rc = remc.lib().gd_load_enepara(b"/nonexistent/gpudocksm.ff", C.byref(ep0))
