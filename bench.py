@@ -343,12 +343,24 @@ def main():
     poses = n_rep * spe * args.steps
     value = poses / elapsed
     traffic = None
+    valu_per_launch = None
     tfile = os.path.join(ROOT, "profiles", "gd_mc_kernel_16384rep_traffic.json")
     if args.config == "c1" and os.path.exists(tfile):
         # HBM bytes per cycle launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-        # passes (calibrated against a device copy), kept with the profile
+        # passes (calibrated against a device copy), and the VALU instructions
+        # per launch (SQ_INSTS_VALU), kept with the profile
         with open(tfile) as f:
-            traffic = json.load(f)["bytes_per_launch"]
+            prof = json.load(f)
+        traffic = prof["bytes_per_launch"]
+        valu_per_launch = prof.get("valu_insts_per_launch")
+    # the device's FP32 VALU issue ceiling, measured now (about a millisecond)
+    probe_ns, probe_tflops, n_simd = remc.valu_probe(8)
+    valu_issue = {"probe_ns_per_wave_inst": probe_ns, "probe_fma_tflops": probe_tflops,
+                  "n_simd": n_simd}
+    if valu_per_launch and mc_n:
+        kern_ns = (mc_ms / mc_n) * 1e6 / (valu_per_launch / n_simd)
+        valu_issue.update(kernel_valu_insts_per_launch=valu_per_launch,
+                          kernel_ns_per_wave_inst=kern_ns, issue_frac=probe_ns / kern_ns)
     avg_mc_s = (mc_ms / mc_n) / 1e3 if mc_n else float("nan")
     achieved = f_pose * n_rep * spe / avg_mc_s / 1e12
     out = {
@@ -397,6 +409,7 @@ def main():
             "waves_per_group": layout["wpg"],
             "persistent": layout["persistent"],
             "grid": layout["grid"],
+            "valu_issue": valu_issue,
         },
         "cpu_baseline": None,
         "autotune": tune,

@@ -1488,3 +1488,44 @@ extern "C" int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
     return gd_fail(GD_ERR_HIP, "score: %s", hipGetErrorString(he));
   return GD_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* C ABI: VALU issue-rate probe                                              */
+/* ------------------------------------------------------------------------ */
+extern "C" int gd_valu_probe(int device, int waves_per_simd, double *ns_per_inst, double *tflops,
+                             int *n_simd)
+{
+  if (!ns_per_inst || !tflops || !n_simd || waves_per_simd < 1 || waves_per_simd > 8)
+    return gd_fail(GD_ERR_ARG, "waves_per_simd must be 1..8 and outputs non-null");
+  if (gd_device_count() <= 0)
+    return gd_fail(GD_ERR_NODEV, "no HIP device");
+  GD_HIP_TRY(hipSetDevice(device));
+  int n_cu = 0;
+  GD_HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+  /* 256-lane workgroups = 4 wavefronts, one per SIMD of a CU */
+  const int blocks = n_cu * waves_per_simd;
+  const int iters = 400;
+  float *out = nullptr;
+  GD_HIP_TRY(hipMalloc(&out, (size_t) blocks * 256 * sizeof(float)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = gd_launch_valu_probe(out, blocks, iters, 0);   /* warm-up */
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  if (e == hipSuccess) e = gd_launch_valu_probe(out, blocks, iters, 0);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void) hipEventDestroy(e0);
+  if (e1) (void) hipEventDestroy(e1);
+  (void) hipFree(out);
+  if (e != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "valu probe: %s", hipGetErrorString(e));
+  const double insts = (double) blocks * 4.0 * iters * 16.0 * 8.0;   /* wavefront FMAs */
+  const double simds = (double) n_cu * 4.0;
+  *ns_per_inst = (double) ms * 1e6 / (insts / simds);
+  *tflops = insts * 64.0 * 2.0 / ((double) ms * 1e-3) / 1e12;
+  *n_simd = n_cu * 4;
+  return GD_OK;
+}

@@ -1097,3 +1097,38 @@ hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_
   default: return occ_wpg<8>(C.occ, stage_bytes, blocks_per_cu);
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* VALU issue-rate probe: eight independent FMA chains per lane, every       */
+/* operand in a vector register (v_fmac_f32_e32), 16 x iters rounds; one     */
+/* store per lane keeps the chains live.  Calibrates the FP32 VALU ceiling   */
+/* that the MC kernel's instruction rate is read against.                    */
+/* ------------------------------------------------------------------------ */
+__global__ void __launch_bounds__(256) gd_valu_probe_kernel(float *out, int iters)
+{
+  const float a = 0.999f + (float) threadIdx.x * 1e-7f;
+  const float b = 1e-3f + (float) threadIdx.x * 1e-7f;
+  float x[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    x[c] = (float) threadIdx.x * 1e-3f + (float) c;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        x[c] = gd_fma(a, b, x[c]);
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    s += x[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t gd_launch_valu_probe(float *out, int blocks, int iters, hipStream_t st)
+{
+  hipLaunchKernelGGL(gd_valu_probe_kernel, dim3(blocks), dim3(256), 0, st, out, iters);
+  return hipGetLastError();
+}

@@ -285,6 +285,8 @@ SYMBOLS = {
     "gd_load_norpara": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(EnePara0)]),
     "gd_last_error": (C.c_char_p, []),
     "gd_device_count": (C.c_int, []),
+    "gd_valu_probe": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "gd_version": (C.c_char_p, []),
 }
 
@@ -551,6 +553,14 @@ def run(cx, temps, mc, opt=None, n_tmp=None):
     check(lib().gd_run(*cx.args(), temps, None, C.byref(mc), C.byref(log), size,
                        C.byref(opt), res))
     return results_to_numpy(res), log
+
+
+def valu_probe(waves_per_simd=8, device=0):
+    """gd_valu_probe: (ns per wavefront FMA instruction per SIMD, device FMA
+    TFLOP/s, SIMDs of the device) with every operand in vector registers."""
+    ns, tf, n = C.c_double(), C.c_double(), C.c_int()
+    check(lib().gd_valu_probe(device, waves_per_simd, C.byref(ns), C.byref(tf), C.byref(n)))
+    return ns.value, tf.value, n.value
 
 
 def score_poses(cx, lig_conf, prt_conf, moves, term_order=0, device=0, fp64=True):

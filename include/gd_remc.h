@@ -579,6 +579,17 @@ int gd_load_norpara(const char *path_a, const char *path_b,
                     gd_enepara0 *enepara0);
 
 /* ---------------------------------------------------------------------- */
+/* Measurement (no counterpart in the reference)                             */
+/* ---------------------------------------------------------------------- */
+/* FP32 VALU issue rate of the device: waves_per_simd wavefronts on every
+ * SIMD run independent FMA chains with all operands in vector registers;
+ * *ns_per_inst = time per wavefront FMA instruction per SIMD, *tflops =
+ * the FMA rate over the whole device, *n_simd = SIMDs of the device.  The
+ * ceiling the MC kernel's instruction rate is read against. */
+int gd_valu_probe(int device, int waves_per_simd, double *ns_per_inst, double *tflops,
+                  int *n_simd);
+
+/* ---------------------------------------------------------------------- */
 const char *gd_last_error(void);
 int gd_device_count(void);
 const char *gd_version(void);

@@ -100,6 +100,24 @@ def test_kde_point_limit():
     assert remc.JobSpec(cx, temps, mc).footprint() > 0
 
 
+def test_valu_probe_arguments():
+    ns, tf, n = C.c_double(), C.c_double(), C.c_int()
+    rc = remc.lib().gd_valu_probe(0, 0, C.byref(ns), C.byref(tf), C.byref(n))
+    assert rc == -1 and b"waves_per_simd" in remc.lib().gd_last_error()
+    if remc.lib().gd_device_count() == 0:
+        rc = remc.lib().gd_valu_probe(0, 8, C.byref(ns), C.byref(tf), C.byref(n))
+        assert rc == -4
+
+
+@pytest.mark.gpu
+def test_valu_probe_on_gpu():
+    """The FP32 FMA issue ceiling: about one wavefront instruction per 1.1 ns
+    per SIMD on MI355X (1,024 SIMDs); only its plausibility is checked."""
+    ns, tflops, n_simd = remc.valu_probe(8)
+    assert n_simd >= 4 and 0.2 < ns < 20.0
+    assert abs(tflops - n_simd * 128 / ns / 1e3) < 1e-6 * tflops
+
+
 def test_load_enepara_missing_file():
     ep0 = remc.EnePara0()
     rc = remc.lib().gd_load_enepara(b"/nonexistent/gpudocksm.ff", C.byref(ep0))
