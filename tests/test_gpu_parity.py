@@ -88,7 +88,8 @@ def test_pose_parity_c0():
 
 @pytest.mark.parametrize("lna,pnp,pnk,pos", [
     (1, 1, 0, 0), (1, 64, 5, 1), (63, 65, 64, 3), (64, 63, 1, 11), (65, 129, 130, 2),
-    (128, 200, 700, 11), (24, 1000, 0, 0), (7, 300, 300, 0)])
+    (128, 200, 700, 11), (24, 1000, 0, 0), (7, 300, 300, 0), (64, 150, 600, 1),
+    (127, 100, 3000, 0)])
 def test_pose_parity_edge_sizes(lna, pnp, pnk, pos):
     cx = synth.make_complex(pnp=pnp, radius=max(6.0, 3.0 * lna ** (1 / 3) + 4),
                             lna=lna, n_lig=2, n_prt=2, pnk=pnk, pos=pos, seed=lna + pnp)
