@@ -31,6 +31,16 @@ typedef struct gd_state {
   float *cmcc_best; /* [n_rep] cmcc of the best pose */
 } gd_state;
 
+/* Per ligand atom, what the pair loop reads once per atom: one 32-byte
+ * record, one scalar load. */
+typedef struct gd_atom_rec {
+  int row;                       /* table row of the atom's type */
+  float q;                       /* charge */
+  int hdb;                       /* 1 if that row has an H-bond pair */
+  float hpl0, inv_hpl1, hpl2;    /* hydrophobic restraint of the atom's type */
+  int pad0, pad1;
+} gd_atom_rec;
+
 /* One complex. */
 typedef struct gd_pack {
   /* sizes */
@@ -91,6 +101,7 @@ typedef struct gd_pack {
   /* [n_rows] 1 if some point type pairs with the row's ligand type through
    * an H-bond term (hdb0 > 0.1) */
   const int *row_hdb;
+  const gd_atom_rec *atom; /* [lna] */
   /* per ligand type (24 entries each) */
   const float *hpl0, *hpl1, *hpl2;
   const float *inv_hpl1; /* 1.0f / hpl1 */

@@ -32,6 +32,7 @@ struct GdImage {
   std::vector<float> psp_pt;     /* [n_prt][n_rows][pnp_pad] */
   std::vector<int> chunk_bounds; /* [n_prt][2]: kh, kp */
   std::vector<int> row_hdb;      /* [n_rows] */
+  std::vector<gd_atom_rec> atom; /* [lna] */
   std::vector<float> hpl;
   std::vector<float> mb;
   std::vector<int> rep_lig, rep_prt, idx_tmp;
@@ -43,7 +44,7 @@ struct GdImage {
  * as one copy. */
 struct GdBlock {
   size_t pt_pos, pt_eh, tab, lig_xyz, lig_row, lig_type, lig_q, kde, kde_pairs, kde_seg, kde_inv, mcs,
-      tcc, psp_pt, chunk_bounds, row_hdb, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
+      tcc, psp_pt, chunk_bounds, row_hdb, atom, hpl, mb, rep_lig, rep_prt, mv, e, mv_best, e_best, step_best, idx_tmp, acs_mc,
       acs_x, cmcc, cmcc_best, step, pack, rec, rec_bytes, mcc_ref, total;
 };
 

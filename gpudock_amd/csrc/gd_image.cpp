@@ -239,6 +239,18 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     I.hpl[2 * GD_MAXTP2 + t] = ep->hpl2[t];
     I.hpl[3 * GD_MAXTP2 + t] = 1.0f / ep->hpl1[t];
   }
+  I.atom.assign(sz.lna, gd_atom_rec());
+  for (int a = 0; a < sz.lna; ++a) {
+    gd_atom_rec &ar = I.atom[a];
+    const int t = I.lig_type[a];
+    ar.row = I.lig_row[a];
+    ar.q = I.lig_q[a];
+    ar.hdb = I.row_hdb[ar.row];
+    ar.hpl0 = I.hpl[t];
+    ar.inv_hpl1 = I.hpl[3 * GD_MAXTP2 + t];
+    ar.hpl2 = I.hpl[2 * GD_MAXTP2 + t];
+    ar.pad0 = ar.pad1 = 0;
+  }
   I.mb.assign(std::max(sz.n_tmp, 1), 0.0f);
   if (temp)
     for (int t = 0; t < sz.n_tmp; ++t)
@@ -369,6 +381,7 @@ GdBlock gd_layout_block(const GdImage &I)
   b.psp_pt = take(vbytes(I.psp_pt));
   b.chunk_bounds = take(vbytes(I.chunk_bounds));
   b.row_hdb = take(vbytes(I.row_hdb));
+  b.atom = take(vbytes(I.atom));
   b.hpl = take(vbytes(I.hpl));
   b.mb = take(vbytes(I.mb));
   b.rep_lig = take(n * 4);
@@ -424,6 +437,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   put(b.psp_pt, I.psp_pt.data(), vbytes(I.psp_pt));
   put(b.chunk_bounds, I.chunk_bounds.data(), vbytes(I.chunk_bounds));
   put(b.row_hdb, I.row_hdb.data(), vbytes(I.row_hdb));
+  put(b.atom, I.atom.data(), vbytes(I.atom));
   put(b.hpl, I.hpl.data(), vbytes(I.hpl));
   put(b.mb, I.mb.data(), vbytes(I.mb));
   put(b.rep_lig, I.rep_lig.data(), vbytes(I.rep_lig));
@@ -447,6 +461,7 @@ void gd_pack_image(GdImage &I, char *host, char *dev, GdBlock &b)
   P.psp_pt = (const float *) (dev + b.psp_pt);
   P.chunk_bounds = (const int *) (dev + b.chunk_bounds);
   P.row_hdb = (const int *) (dev + b.row_hdb);
+  P.atom = (const gd_atom_rec *) (dev + b.atom);
   P.hpl0 = (const float *) (dev + b.hpl);
   P.hpl1 = P.hpl0 + GD_MAXTP2;
   P.hpl2 = P.hpl0 + 2 * GD_MAXTP2;

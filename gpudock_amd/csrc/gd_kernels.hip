@@ -61,6 +61,13 @@ __device__ __forceinline__ T gd_uload(const T *p)
   return *(const __attribute__((address_space(4))) T *) p;
 }
 
+/* One atom record through the scalar unit (a single 32-byte load). */
+typedef int gd_i8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ gd_atom_rec gd_load_atom(const gd_atom_rec *p)
+{
+  return __builtin_bit_cast(gd_atom_rec, *(const __attribute__((address_space(4))) gd_i8 *) p);
+}
+
 __device__ __forceinline__ float gd_bcast(float v, int lane)
 {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -183,8 +190,9 @@ __device__ __forceinline__ void gd_energy_wave(
     const float lx = gd_bcast(hi ? ax1 : ax0, src);
     const float ly = gd_bcast(hi ? ay1 : ay0, src);
     const float lz = gd_bcast(hi ? az1 : az0, src);
-    const int row = gd_uload(P.lig_row + l);
-    const float lq = gd_uload(P.lig_q + l);
+    const gd_atom_rec ar = gd_load_atom(P.atom + l);
+    const int row = ar.row;
+    const float lq = ar.q;
     const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
     /* the PSP row as a buffer: lane offset fixed, chunk offset scalar */
     const __amdgpu_buffer_rsrc_t psp_rsrc = __builtin_amdgcn_make_buffer_rsrc(
@@ -228,7 +236,7 @@ __device__ __forceinline__ void gd_energy_wave(
     auto rows = [&](auto read) {
       const gd_bool<true> on;
       const gd_bool<false> off;
-      if (gd_uload(P.row_hdb + row)) {
+      if (ar.hdb) {
         chunks(read, on, on, on, 0, kh);
         chunks(read, off, on, on, kh, kp);
         chunks(read, off, on, off, kp, nchunk);
@@ -256,10 +264,9 @@ __device__ __forceinline__ void gd_energy_wave(
       });
     }
     eele = gd_fma(lq, acc.ele, eele);
-    const int lt = gd_uload(P.lig_type + l);
     const float hs = gd_wave_sum(hacc);
-    const float z = (hs - gd_uload(P.hpl0 + lt)) * gd_uload(P.inv_hpl1 + lt);
-    ehpc += 0.5f * z * z - gd_uload(P.hpl2 + lt);
+    const float z = (hs - ar.hpl0) * ar.inv_hpl1;
+    ehpc += 0.5f * z * z - ar.hpl2;
   }
 
   /* KDE (calcenergy_d.cu:175-213): sum over atoms of the mean of
