@@ -1,7 +1,7 @@
 /*
  * gd_oracle.c -- CPU oracle of the REMC docking core.  TEST INFRASTRUCTURE:
  * only the tests, the smoke check and the CPU-baseline leg of the benchmark
- * load this library, as the checker; the product never links it.
+ * load this library, to check results against; the product never links it.
  *
  * Three independent pieces:
  *

@@ -1,6 +1,6 @@
 """ctypes access to the CPU oracle (oracle/liboracle.so) for the tests.
 
-The oracle is the checker; nothing in the product imports this module.
+The oracle is what the tests compare against; nothing in the product imports this module.
 """
 import ctypes as C
 import os
