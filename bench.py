@@ -301,7 +301,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} "
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} "
               "requested); the REMC chains shard by whole temperature ladders, "
               "see the sharding notes", file=sys.stderr)
         sys.exit(2)

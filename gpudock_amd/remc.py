@@ -297,8 +297,7 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
-                f"{LIB_PATH} is not built: run python -c 'import __graft_entry__ as g; "
-                "g.build()' (or python gpudock_amd/build.py)")
+                f"{LIB_PATH} is not built: run python gpudock_amd/build.py")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SYMBOLS.items():
             fn = getattr(L, name)

@@ -225,7 +225,7 @@ typedef struct gd_step_record {
 
 /* .gdtraj file: this header, then n_rep x n_step records, replica-major
  * (the reference's LigRecord[n_rep].step[] order); the reference writes
- * the same records as HDF5 (hdf5io.C:10-30), which this build does not
+ * the same records as HDF5 (hdf5io.C:10-30), which this library does not
  * link. */
 typedef struct gd_traj_header {
   char magic[8];               /* "GDTRAJ01" */
