@@ -55,8 +55,11 @@ typedef struct gd_pack {
   int kde_n_lo, kde_n_hi;/* (atom, KDE point) pairs of atoms < 64 and >= 64 */
 
   /* run control */
-  int steps_total;       /* MC steps of the job */
-  int spe;               /* steps per exchange */
+  int steps_run;         /* MC steps the job runs: whole windows of whole
+                            launches (gpusingle.cu:8-20) */
+  int spe;               /* steps per exchange = MC steps per cycle */
+  int lpw;               /* cycles (launches) per steps_per_dump window: the
+                            exchange mode restarts at 0 in every window */
   int exchange;          /* temperature swaps on/off */
   int away;              /* start 44.5 * move_scale away (IS_AWAY) */
   uint32_t seed, cid;    /* Philox key */

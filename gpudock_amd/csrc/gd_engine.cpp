@@ -461,7 +461,7 @@ int gd_engine::add_image(std::unique_ptr<GdImage> &img, int *handle)
   s->bytes = bytes;
   s->pack_dev = (gd_pack *) (arena.base() + off + s->blk.pack);
   s->handle = next_handle++;
-  s->total_steps = I.P.steps_total;
+  s->total_steps = I.P.steps_run;
   s->spe = I.P.spe;
   /* initial energies: queued with the next task-list build, one launch for
    * every complex admitted since the last one */
@@ -1091,7 +1091,7 @@ extern "C" int gd_session_create(const gd_ligand *lig, const gd_protein *prt,
   if (rc)
     return rc;
   s->spe = mcpara->steps_per_exchange;
-  s->total = mcpara->steps_total;
+  s->total = s->eng.slots[0]->total_steps;
   *out = s.release();
   return GD_OK;
 }
@@ -1255,15 +1255,22 @@ extern "C" int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp 
                              opt, &s);
   if (rc)
     return rc;
+  /* the reference's schedule: whole steps_per_dump windows of whole
+   * launches (gpusingle.cu:8-20), steps_run / spe cycles in all */
   const int spe = mcpara->steps_per_exchange;
-  const int n_cycles = (mcpara->steps_total + spe - 1) / spe;
+  const int n_cycles = s->total / spe;
   double t_io = 0.0;
   if (opt && opt->record_stride > 0 && opt->record_prefix) {
-    /* one window of steps_per_dump steps at a time, dumped after it ends */
+    /* one window at a time, dumped after it ends (records require spe to
+     * divide steps_per_dump, so a window is steps_per_dump / spe cycles) */
     const int per_window = mcpara->steps_per_dump / spe;
     int seq = 0;
     for (int c = 0; c < n_cycles && !rc; c += per_window) {
       rc = s->eng.cycles(std::min(per_window, n_cycles - c));
+      /* the window's cycles are compute: only the copy and the file count
+       * as I/O (the reference syncs after every launch, cutil_yeah.h) */
+      if (!rc)
+        rc = s->eng.sync();
       auto t_w = std::chrono::steady_clock::now();
       if (!rc)
         rc = dump_window(s, opt->record_prefix, seq++);
@@ -1307,8 +1314,11 @@ extern "C" int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_confi
     if (rc)
       return rc;
     total += b + (size_t) jobs[i].size.n_rep * 256;
-    const int spe = jobs[i].mcpara->steps_per_exchange;
-    max_cycles = std::max(max_cycles, (jobs[i].mcpara->steps_total + spe - 1) / spe);
+    int run = 0, lpw = 0;
+    rc = gd_schedule(jobs[i].mcpara, &run, &lpw);
+    if (rc)
+      return rc;
+    max_cycles = std::max(max_cycles, run / jobs[i].mcpara->steps_per_exchange);
   }
   gd_engine eng;
   gd_engine_config c;

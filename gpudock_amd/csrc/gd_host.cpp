@@ -50,6 +50,27 @@ extern "C" int gd_device_count(void)
 /* ------------------------------------------------------------------------ */
 /* argument checks                                                           */
 /* ------------------------------------------------------------------------ */
+int gd_schedule(const gd_mcpara *mc, int *steps_run, int *launches_per_window)
+{
+  if (!mc)
+    return gd_fail(GD_ERR_ARG, "null mcpara");
+  const long long total = mc->steps_total, spd = mc->steps_per_dump,
+                  spe = mc->steps_per_exchange;
+  if (total < 0 || spe < 1 || spd < 1)
+    return gd_fail(GD_ERR_ARG,
+                   "steps_total=%lld steps_per_dump=%lld steps_per_exchange=%lld (need "
+                   "steps_total >= 0, steps_per_dump >= 1, steps_per_exchange >= 1)",
+                   total, spd, spe);
+  const long long lpw = (spd + spe - 1) / spe;
+  const long long windows = (total + spd - 1) / spd;
+  const long long run = windows * lpw * spe;
+  if (run > (1LL << 30))
+    return gd_fail(GD_ERR_ARG, "schedule runs %lld MC steps (limit 2^30)", run);
+  *steps_run = (int) run;
+  *launches_per_window = (int) lpw;
+  return GD_OK;
+}
+
 int gd_check_inputs(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
                         const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *ep,
                     const gd_complex_size &sz)

@@ -44,8 +44,14 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   P.n_res = psp->n_res;
   P.term_order = o.term_order;
   P.combine = o.combine;
-  P.steps_total = mc ? mc->steps_total : 0;
+  P.steps_run = 0;
   P.spe = mc ? mc->steps_per_exchange : 1;
+  P.lpw = 1;
+  if (mc) {
+    const int rc = gd_schedule(mc, &P.steps_run, &P.lpw);
+    if (rc)
+      return rc;
+  }
   P.exchange = o.exchange;
   P.away = o.away;
   P.seed = o.seed;
@@ -275,6 +281,31 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     I.rep_prt[r] = p;
     I.rep_lig[r] = l;
     I.idx_tmp[r] = t;
+  }
+  /* The exchange treats slots n_tmp*n_lig*p + n_lig*t + l, t = 0..n_tmp-1, as
+   * the temperature chain of (p, l) (exchangereplicas_d.cu:28-38) and inverts
+   * their temperature indices: every slot of the chain must hold protein
+   * conformer p and ligand conformer l, as SetReplica lays them out
+   * (util.C:537-558), and the chain's temperatures must be a permutation of
+   * 0..n_tmp-1. */
+  if (replica) {
+    std::vector<int> seen(sz.n_tmp);
+    for (int p = 0; p < sz.n_prt; ++p)
+      for (int l = 0; l < sz.n_lig; ++l) {
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int t = 0; t < sz.n_tmp; ++t) {
+          const int r = sz.n_tmp * sz.n_lig * p + sz.n_lig * t + l;
+          if (I.rep_prt[r] != p || I.rep_lig[r] != l)
+            return gd_fail(GD_ERR_ARG,
+                           "replica %d holds conformers (prt %d, lig %d); its slot belongs to "
+                           "the chain of (prt %d, lig %d)",
+                           r, I.rep_prt[r], I.rep_lig[r], p, l);
+          if (seen[I.idx_tmp[r]]++)
+            return gd_fail(GD_ERR_ARG,
+                           "temperature %d appears twice in the chain of (prt %d, lig %d)",
+                           I.idx_tmp[r], p, l);
+        }
+      }
   }
 
   gd_pair_scalars_init(&P.sc, ep->lj0, ep->lj1, ep->el0, ep->el1, ep->a1, ep->b1);

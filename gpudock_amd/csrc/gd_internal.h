@@ -30,6 +30,15 @@ int gd_check_inputs(const gd_ligand *lig, const gd_protein *prt, const gd_psp *p
                     const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *ep,
                     const gd_complex_size &sz);
 
+/* The reference's launch schedule (gpusingle.cu:8-20): windows of
+ * steps_per_dump steps start at s1 = 0, spd, ... while s1 < steps_total; each
+ * window runs whole launches of steps_per_exchange MC steps at s2 = 0, spe,
+ * ... while s2 < spd, each followed by one exchange in mode (s2 / spe) % 2.
+ * *launches_per_window = ceil(spd / spe) and *steps_run = windows x launches
+ * per window x spe, the MC steps that actually run (above steps_total when
+ * spd does not divide steps_total or spe does not divide spd). */
+int gd_schedule(const gd_mcpara *mc, int *steps_run, int *launches_per_window);
+
 /* First-fit sub-allocator over one device allocation (or one pinned host
  * allocation, for staging copies).  Jobs are admitted and retired without
  * hipMalloc/hipFree (which synchronise the device). */

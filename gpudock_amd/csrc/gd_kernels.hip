@@ -409,10 +409,11 @@ __device__ __forceinline__ void gd_load_conformer(const gd_pack &P, int conf, fl
   }
 }
 
-/* Steps of this cycle for a complex: min(spe, steps_total - step). */
+/* Steps of this cycle for a complex: spe while steps are left (steps_run is
+ * a whole number of cycles), else 0. */
 __device__ __forceinline__ int gd_cycle_steps(const gd_pack &P, int s0)
 {
-  const int left = P.steps_total - s0;
+  const int left = P.steps_run - s0;
   return left < P.spe ? left : P.spe;
 }
 
@@ -661,7 +662,9 @@ gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict
     ener[t] = S.e[a * 10 + GD_E_TOTAL];
     orders[temps[t]] = t;
   }
-  const int mode = x & 1;
+  /* cycle x is launch x % lpw of its steps_per_dump window; the mode
+   * alternates from 0 in every window (gpusingle.cu:14-18) */
+  const int mode = (x % P.lpw) & 1;
   const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && mode == 1 ? 1 : 0);
   gd_u32x4 u4;
   for (int j = 0; j < maxpair; ++j) {

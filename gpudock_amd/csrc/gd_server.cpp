@@ -444,7 +444,7 @@ extern "C" int gd_server_submit(gd_server *s, const gd_job *job, long long *tick
     return gd_fail(GD_ERR_NOMEM, "job needs %zu B of HBM; the server arena has %zu B",
                    j->info.footprint, s->eng.arena.size());
   j->info.n_rep = j->img->P.n_rep;
-  j->info.steps_total = j->img->P.steps_total;
+  j->info.steps_total = j->img->P.steps_run;
   j->info.state = GD_JOB_QUEUED;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop)

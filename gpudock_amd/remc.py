@@ -518,7 +518,13 @@ def default_options(seed=1, complex_id=0, exchange=1, away=0, term_order=0, devi
                    mcc, combine)
 
 
-def mcpara(steps_total, steps_per_exchange, move_scale, steps_per_dump=1000):
+def mcpara(steps_total, steps_per_exchange, move_scale, steps_per_dump=None):
+    """McPara.  The run follows the reference's schedule (gpusingle.cu:8-20):
+    windows of steps_per_dump steps, each of whole launches of
+    steps_per_exchange steps with the exchange mode restarting at 0;
+    steps_per_dump None = steps_total (one window)."""
+    if steps_per_dump is None:
+        steps_per_dump = max(int(steps_total), 1)
     return McPara(steps_total, steps_per_dump, steps_per_exchange, (F * 6)(*move_scale))
 
 
@@ -540,8 +546,16 @@ def results_to_numpy(res):
     return out
 
 
-def run(cx, temps, mc, opt=None, n_tmp=None):
-    """gd_run (drop-in for Run()); returns (results dict, McLog)."""
+def replica_table(size):
+    """SetReplica (util.C:537-558) through the library: [n_rep] Replica."""
+    arr = (Replica * size.n_rep)()
+    check(lib().gd_set_replica(arr, size))
+    return arr
+
+
+def run(cx, temps, mc, opt=None, n_tmp=None, replica=None):
+    """gd_run (drop-in for Run()); returns (results dict, McLog).  replica:
+    a Replica table (None = the SetReplica layout)."""
     n_tmp = n_tmp or len(temps)
     size = cx.sized(n_tmp)
     res = (ReplicaResult * size.n_rep)()
@@ -549,7 +563,7 @@ def run(cx, temps, mc, opt=None, n_tmp=None):
     acx = np.zeros(size.n_rep, np.int32)
     log = McLog(0, 0, 0, 0, _ip(acs), 0, _ip(acx))
     opt = opt or default_options()
-    check(lib().gd_run(*cx.args(), temps, None, C.byref(mc), C.byref(log), size,
+    check(lib().gd_run(*cx.args(), temps, replica, C.byref(mc), C.byref(log), size,
                        C.byref(opt), res))
     return results_to_numpy(res), log
 
@@ -596,6 +610,10 @@ class Session:
 
     def sync(self):
         check(lib().gd_session_sync(self.h))
+
+    def steps_done(self):
+        """MC steps queued so far (the schedule's count, gpusingle.cu:8-20)."""
+        return int(lib().gd_session_steps_done(self.h))
 
     def set_timing(self, on):
         check(lib().gd_session_set_timing(self.h, int(on)))
@@ -657,13 +675,15 @@ class JobSpec:
 
     Holds every ctypes object the job's pointers refer to."""
 
-    def __init__(self, cx, temps, mc, opt=None):
+    def __init__(self, cx, temps, mc, opt=None, replica=None):
         self.cx, self.temps, self.mc = cx, temps, mc
         self.opt = opt or default_options()
         self.size = cx.sized(len(temps))
+        self.replica = replica
+        rep = C.cast(replica, C.POINTER(Replica)) if replica is not None else None
         self.c = Job(C.pointer(cx.c_lig), C.pointer(cx.c_prt), C.pointer(cx.c_psp),
                      C.pointer(cx.c_kde), C.pointer(cx.c_mcs), C.pointer(cx.enepara),
-                     C.cast(temps, C.POINTER(Temp)), None, C.pointer(mc), self.size,
+                     C.cast(temps, C.POINTER(Temp)), rep, C.pointer(mc), self.size,
                      C.pointer(self.opt))
 
     @property

@@ -363,17 +363,108 @@ ORC_EXPORT void orc_pose_ref32(const gd_ligand *lig, const gd_protein *prt,
   free(buf);
 }
 
+/* ---------------------------------------------------------------------- */
+/* Decision rules and the launch schedule, shared by the reference-order    */
+/* run and the canonical replay.  canonical = 0 evaluates the exponentials  */
+/* with libm expf as the reference does; 1 with gd_canon.h's gd_exp as the   */
+/* GPU does.                                                                */
+/* ---------------------------------------------------------------------- */
+static float orc_expf(int canonical, float x) { return canonical ? gd_exp(x) : expf(x); }
+
+/* Accept_d (accept_d.cu:22-23): delta_energy = E_new - E_old (the total),
+ * accepted when the draw u in (0, 1] is below expf(delta * minus_beta);
+ * minus_beta = -1 / T is below zero. */
+ORC_EXPORT int orc_accept(int canonical, float e_new, float e_old, float minus_beta, float u)
+{
+  const float delta_energy = e_new - e_old;
+  return u < orc_expf(canonical, delta_energy * minus_beta);
+}
+
+/* ExchangeReplicas_d for one temperature chain (exchangereplicas_d.cu:28-78).
+ * temps[t]: temperature index held by chain slot t (updated in place);
+ * ener[t]: its total energy; minus_beta[i]: of temperature i; mode_t 0 pairs
+ * temperatures (0,1), (2,3), ...; 1 pairs (1,2), (3,4), ...; u[j]: the draw
+ * of pair j; acs[t] (or NULL) counts the swaps of slot t.  Returns the number
+ * of pairs visited. */
+ORC_EXPORT int orc_exchange_chain(int canonical, int n_tmp, const float *minus_beta,
+                                  int *temps, const float *ener, int mode_t, const float *u,
+                                  int *acs)
+{
+  int temp_orders[64];
+  for (int t = 0; t < n_tmp; ++t)
+    temp_orders[temps[t]] = t;
+  const int maxpair = n_tmp / 2 - (n_tmp % 2 == 0) * (mode_t == 1);
+  for (int pair = 0; pair < maxpair; ++pair) {
+    const int i1 = (pair << 1) + mode_t;
+    const int i2 = i1 + 1;
+    const int o1 = temp_orders[i1];
+    const int o2 = temp_orders[i2];
+    const int tt = temps[o1];
+    const float etot1 = ener[o1];
+    const float etot2 = ener[o2];
+    const float minus_beta1 = minus_beta[temps[o1]];
+    const float minus_beta2 = minus_beta[temps[o2]];
+    const float delta = (minus_beta1 - minus_beta2) * (etot2 - etot1);
+    if (orc_expf(canonical, delta) > u[pair]) {
+      temps[o1] = temps[o2];
+      temps[o2] = tt;
+      if (acs) {
+        acs[o1] += 1;
+        acs[o2] += 1;
+      }
+    }
+  }
+  return maxpair;
+}
+
+/* The launch sequence of gpusingle.cu:8-20:
+ *   for (s1 = 0; s1 < steps_total; s1 += steps_per_dump)
+ *     for (s2 = 0; s2 < steps_per_dump; s2 += steps_per_exchange)
+ *       MonteCarlo_d (steps_per_exchange steps), then
+ *       ExchangeReplicas_d in mode_t = (s2 / steps_per_exchange) % 2.
+ * Launch x runs MC steps [x spe, (x + 1) spe) of the run (the Philox step
+ * counter), and is followed by exchange number x.  mode[x] for x < max
+ * (mode may be NULL); returns the number of launches, -1 for a step size
+ * below 1. */
+ORC_EXPORT int orc_schedule(int steps_total, int steps_per_dump, int steps_per_exchange,
+                            int *mode, int max)
+{
+  if (steps_per_dump < 1 || steps_per_exchange < 1)
+    return -1;
+  int x = 0;
+  for (int s1 = 0; s1 < steps_total; s1 += steps_per_dump)
+    for (int s2 = 0; s2 < steps_per_dump; s2 += steps_per_exchange, ++x)
+      if (mode && x < max)
+        mode[x] = (s2 / steps_per_exchange) % 2;
+  return x;
+}
+
+/* The exchange draws of chain c after launch x: Philox counter (c, x, 2,
+ * j >> 2), element j & 3, for pair j. */
+static void exchange_draws(uint32_t c, uint32_t x, int n_pair, uint32_t seed, uint32_t cid,
+                           float *u)
+{
+  gd_u32x4 u4;
+  for (int j = 0; j < n_pair; ++j) {
+    if ((j & 3) == 0)
+      u4 = gd_philox4x32_10(c, x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2), seed, cid);
+    u[j] = gd_u01(u4.v[j & 3]);
+  }
+}
+
 /* Reference-order REMC on the CPU (the CPU baseline): the Metropolis and
  * exchange rules of the reference on the Philox stream, OpenMP over
  * replicas as montecarlo_cpu.C:34-55 does.  Energies come from
  * orc_energy_ref32, so trajectories are statistically, not bitwise, equal
- * to the GPU's.  Runs steps [0, n_steps) from the loaded pose (IS_AWAY 0);
- * exchange after every steps_per_exchange steps when n_tmp > 1 and
- * exchange != 0.  acs: [n_rep] accepted moves; e_final: [n_rep * 10]. */
+ * to the GPU's.  Runs the launches of orc_schedule(n_steps, steps_per_dump,
+ * steps_per_exchange) from the loaded pose (IS_AWAY 0), each followed by an
+ * exchange when n_tmp > 1 and exchange != 0.  acs: [n_rep] accepted moves;
+ * e_final: [n_rep * 10]. */
 static int ref_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
                    const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *ep,
                    const gd_temp *temp, gd_complex_size sz, const float move_scale[6],
-                   int n_steps, int steps_per_exchange, int exchange, uint32_t seed,
+                   int n_steps, int steps_per_dump, int steps_per_exchange, int exchange,
+                   uint32_t seed,
                    uint32_t cid, int term_order, int n_threads, int *acs, int *acs_x,
                    int *idx_tmp_out, float *e_final, float *mv_final, float *e_samp,
                    int *t_samp, int samp_stride, int n_samp)
@@ -400,8 +491,15 @@ static int ref_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *ps
     orc_pose_ref32(lig, prt, psp, kde, mcs, ep, sz, l, p, mv + (size_t) r * 6,
                    term_order, e + (size_t) r * 10);
   }
-  for (int s0 = 0; s0 < n_steps; s0 += steps_per_exchange) {
-    const int s1 = s0 + steps_per_exchange < n_steps ? s0 + steps_per_exchange : n_steps;
+  const int n_launch = orc_schedule(n_steps, steps_per_dump, steps_per_exchange, NULL, 0);
+  if (n_launch < 0) {
+    free(mv); free(e); free(it);
+    return -1;
+  }
+  int *modes = (int *) malloc(sizeof(int) * (n_launch + 1));
+  orc_schedule(n_steps, steps_per_dump, steps_per_exchange, modes, n_launch);
+  for (int x = 0; x < n_launch; ++x) {
+    const int s0 = x * steps_per_exchange, s1 = s0 + steps_per_exchange;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int r = 0; r < n; ++r) {
       const int l = r % sz.n_lig, p = r / (sz.n_tmp * sz.n_lig);
@@ -425,8 +523,7 @@ static int ref_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *ps
                        buf + lna, buf + 2 * lna, cen);
         orc_energy_ref32(lig, prt, psp, kde, mcs, ep, sz, p, buf, buf + lna,
                          buf + 2 * lna, cen, term_order, en);
-        const float de = en[9] - eo[9];
-        if (gd_u01(w[6]) < expf(de * mb)) {
+        if (orc_accept(0, en[9], eo[9], mb, gd_u01(w[6]))) {
           memcpy(mo, mn, sizeof mn);
           memcpy(eo, en, sizeof en);
           acs[r]++;
@@ -435,41 +532,31 @@ static int ref_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *ps
       free(buf);
     }
     if (exchange && sz.n_tmp > 1) {
-      const int x = s0 / steps_per_exchange;
       const int n_chain = sz.n_prt * sz.n_lig;
+      float mbs[64], u[32];
+      for (int t = 0; t < sz.n_tmp; ++t)
+        mbs[t] = temp[t].minus_beta;
       for (int c = 0; c < n_chain; ++c) {
         const int p = c / sz.n_lig, l = c % sz.n_lig;
         const int base = sz.n_tmp * sz.n_lig * p + l;
-        int temps[64], orders[64];
+        int temps[64], acx[64];
         float ener[64];
         for (int t = 0; t < sz.n_tmp; ++t) {
           temps[t] = it[base + sz.n_lig * t];
           ener[t] = e[(size_t) (base + sz.n_lig * t) * 10 + 9];
-          orders[temps[t]] = t;
+          acx[t] = 0;
         }
-        const int mode = x & 1;
-        const int maxpair = sz.n_tmp / 2 - ((sz.n_tmp % 2 == 0) && mode == 1);
-        for (int j = 0; j < maxpair; ++j) {
-          const gd_u32x4 u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x,
-                                               GD_CTR_EXCHANGE, (uint32_t) (j >> 2), seed, cid);
-          const int i1 = 2 * j + mode, i2 = i1 + 1;
-          const int o1 = orders[i1], o2 = orders[i2];
-          const float delta = (temp[i1].minus_beta - temp[i2].minus_beta) * (ener[o2] - ener[o1]);
-          if (expf(delta) > gd_u01(u4.v[j & 3])) {
-            const int tt = temps[o1];
-            temps[o1] = temps[o2];
-            temps[o2] = tt;
-            if (acs_x) {
-              acs_x[base + sz.n_lig * o1]++;
-              acs_x[base + sz.n_lig * o2]++;
-            }
-          }
-        }
-        for (int t = 0; t < sz.n_tmp; ++t)
+        exchange_draws((uint32_t) c, (uint32_t) x, sz.n_tmp / 2, seed, cid, u);
+        orc_exchange_chain(0, sz.n_tmp, mbs, temps, ener, modes[x], u, acx);
+        for (int t = 0; t < sz.n_tmp; ++t) {
           it[base + sz.n_lig * t] = temps[t];
+          if (acs_x)
+            acs_x[base + sz.n_lig * t] += acx[t];
+        }
       }
     }
   }
+  free(modes);
   if (e_final)
     memcpy(e_final, e, sizeof(float) * 10 * n);
   if (mv_final)
@@ -486,12 +573,12 @@ ORC_EXPORT int orc_ref_run(const gd_ligand *lig, const gd_protein *prt,
                            const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
                            const gd_enepara *ep, const gd_temp *temp,
                            gd_complex_size sz, const float move_scale[6], int n_steps,
-                           int steps_per_exchange, int exchange, uint32_t seed,
-                           uint32_t cid, int term_order, int n_threads, int *acs,
-                           int *acs_x, int *idx_tmp_out, float *e_final,
+                           int steps_per_dump, int steps_per_exchange, int exchange,
+                           uint32_t seed, uint32_t cid, int term_order, int n_threads,
+                           int *acs, int *acs_x, int *idx_tmp_out, float *e_final,
                            float *mv_final)
 {
-  return ref_run(lig, prt, psp, kde, mcs, ep, temp, sz, move_scale, n_steps,
+  return ref_run(lig, prt, psp, kde, mcs, ep, temp, sz, move_scale, n_steps, steps_per_dump,
                  steps_per_exchange, exchange, seed, cid, term_order, n_threads, acs, acs_x,
                  idx_tmp_out, e_final, mv_final, NULL, NULL, 1, 0);
 }
@@ -503,15 +590,15 @@ ORC_EXPORT int orc_ref_run_samples(const gd_ligand *lig, const gd_protein *prt,
                                    const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
                                    const gd_enepara *ep, const gd_temp *temp,
                                    gd_complex_size sz, const float move_scale[6],
-                                   int n_steps, int steps_per_exchange, int exchange,
-                                   uint32_t seed, uint32_t cid, int term_order,
-                                   int n_threads, int *acs, float *e_samp, int *t_samp,
-                                   int samp_stride, int n_samp)
+                                   int n_steps, int steps_per_dump, int steps_per_exchange,
+                                   int exchange, uint32_t seed, uint32_t cid, int term_order,
+                                   int n_threads, int *acs, int *acs_x, float *e_samp,
+                                   int *t_samp, int samp_stride, int n_samp)
 {
   if (samp_stride < 1)
     return -1;
-  return ref_run(lig, prt, psp, kde, mcs, ep, temp, sz, move_scale, n_steps,
-                 steps_per_exchange, exchange, seed, cid, term_order, n_threads, acs, NULL,
+  return ref_run(lig, prt, psp, kde, mcs, ep, temp, sz, move_scale, n_steps, steps_per_dump,
+                 steps_per_exchange, exchange, seed, cid, term_order, n_threads, acs, acs_x,
                  NULL, NULL, NULL, e_samp, t_samp, samp_stride, n_samp);
 }
 
@@ -784,8 +871,8 @@ ORC_EXPORT void orc_replay_energy(const gd_ligand *lig, const gd_protein *prt,
 
 /* Canonical replay of gd_run: the same initialisation, MC steps, exchange
  * cadence and counter keys.  Fills results[n_rep] (energy_best_fp64 is
- * left zero).  steps_total need not be a multiple of steps_per_exchange;
- * the last, shorter cycle is followed by an exchange like the others. */
+ * left zero).  The launches are those of orc_schedule (whole steps_per_dump
+ * windows of whole launches, gpusingle.cu:8-20). */
 /* The replay with RecordLigand_d (util_d.cu:72-94): before every MC step s
  * with s % record_stride == 0 the old state goes to
  * rec[r * n_rec + s / record_stride] (record_stride 0: no records). */
@@ -797,6 +884,8 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
                                   gd_replica_result *res, int record_stride,
                                   gd_step_record *rec, int n_rec)
 {
+  if (orc_schedule(mc->steps_total, mc->steps_per_dump, mc->steps_per_exchange, NULL, 0) < 0)
+    return -1;
   rp_ctx c;
   rp_init(&c, lig, prt, psp, kde, mcs, ep, sz, opt->term_order);
   c.combine = opt->combine;
@@ -850,9 +939,12 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
     res[r].step_best = -1;
   }
   const int spe = mc->steps_per_exchange;
-  int x = 0;
-  for (int s0 = 0; s0 < mc->steps_total; s0 += spe, ++x) {
-    const int s1 = s0 + spe < mc->steps_total ? s0 + spe : mc->steps_total;
+  const int n_launch = orc_schedule(mc->steps_total, mc->steps_per_dump, spe, NULL, 0);
+  int *modes = (int *) malloc(sizeof(int) * (n_launch > 0 ? n_launch : 1));
+  if (n_launch > 0)
+    orc_schedule(mc->steps_total, mc->steps_per_dump, spe, modes, n_launch);
+  for (int x = 0; x < n_launch; ++x) {
+    const int s0 = x * spe, s1 = s0 + spe;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int r = 0; r < n; ++r) {
       gd_replica_result *R = &res[r];
@@ -880,8 +972,7 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
         rp_place(&c, lc[r], mn, ax, ay, az, cen);
         float cmn = 0.0f;
         rp_energy(&c, pc[r], ax, ay, az, cen, en, ref, &cmn);
-        const float de = en[GD_E_TOTAL] - R->energy_final[GD_E_TOTAL];
-        if (gd_u01(w[6]) < gd_exp(de * mb)) {
+        if (orc_accept(1, en[GD_E_TOTAL], R->energy_final[GD_E_TOTAL], mb, gd_u01(w[6]))) {
           R->acs_mc++;
           memcpy(R->move_final, mn, sizeof mn);
           memcpy(R->energy_final, en, sizeof en);
@@ -897,39 +988,29 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
     }
     if (opt->exchange && sz.n_tmp > 1) {
       const int n_chain = sz.n_prt * sz.n_lig;
+      float mbs[64], u[32];
+      for (int t = 0; t < sz.n_tmp; ++t)
+        mbs[t] = temp[t].minus_beta;
       for (int ch = 0; ch < n_chain; ++ch) {
         const int p = ch / sz.n_lig, l = ch % sz.n_lig;
         const int base = sz.n_tmp * sz.n_lig * p + l;
-        int temps[64], orders[64];
+        int temps[64], acx[64];
         float ener[64];
         for (int t = 0; t < sz.n_tmp; ++t) {
           temps[t] = it[base + sz.n_lig * t];
           ener[t] = res[base + sz.n_lig * t].energy_final[GD_E_TOTAL];
-          orders[temps[t]] = t;
+          acx[t] = 0;
         }
-        const int mode = x & 1;
-        const int maxpair = sz.n_tmp / 2 - ((sz.n_tmp % 2 == 0) && mode == 1);
-        gd_u32x4 u4;
-        for (int j = 0; j < maxpair; ++j) {
-          if ((j & 3) == 0)
-            u4 = gd_philox4x32_10((uint32_t) ch, (uint32_t) x, GD_CTR_EXCHANGE,
-                                  (uint32_t) (j >> 2), opt->seed, opt->complex_id);
-          const int i1 = 2 * j + mode, i2 = i1 + 1;
-          const int o1 = orders[i1], o2 = orders[i2];
-          const float delta = (temp[i1].minus_beta - temp[i2].minus_beta) * (ener[o2] - ener[o1]);
-          if (gd_exp(delta) > gd_u01(u4.v[j & 3])) {
-            const int tt = temps[o1];
-            temps[o1] = temps[o2];
-            temps[o2] = tt;
-            res[base + sz.n_lig * o1].acs_exchg++;
-            res[base + sz.n_lig * o2].acs_exchg++;
-          }
-        }
-        for (int t = 0; t < sz.n_tmp; ++t)
+        exchange_draws((uint32_t) ch, (uint32_t) x, sz.n_tmp / 2, opt->seed, opt->complex_id, u);
+        orc_exchange_chain(1, sz.n_tmp, mbs, temps, ener, modes[x], u, acx);
+        for (int t = 0; t < sz.n_tmp; ++t) {
           it[base + sz.n_lig * t] = temps[t];
+          res[base + sz.n_lig * t].acs_exchg += acx[t];
+        }
       }
     }
   }
+  free(modes);
   for (int r = 0; r < n; ++r)
     res[r].idx_tmp_final = it[r];
   free(ref);
@@ -973,6 +1054,20 @@ ORC_EXPORT void orc_bayes_pdf(int family, int n, const float *x, float loc, floa
 ORC_EXPORT float orc_bayes_total(const float norm_file[9])
 {
   return gd_bayes_total(norm_file);
+}
+
+/* The canonical move (gd_canon.h gd_rotation + gd_transform, what the GPU
+ * runs) of n points (x, y, z) about centre c by movematrix m. */
+ORC_EXPORT void orc_canon_move(const float m[6], const float *x, const float *y,
+                               const float *z, int n, const float c[3], float *ox, float *oy,
+                               float *oz, float cen[3])
+{
+  float rot[9];
+  gd_rotation(m, rot);
+  for (int a = 0; a < n; ++a)
+    gd_transform(rot, m, c, x[a], y[a], z[a], &ox[a], &oy[a], &oz[a]);
+  for (int i = 0; i < 3; ++i)
+    cen[i] = c[i] + m[i];
 }
 
 ORC_EXPORT void orc_canon_math(int fn, int n, const float *x, float *y, float *y2)

@@ -64,3 +64,26 @@ def load_1a07c1():
         kde_t=r["kde_t"], mcs_xyz=r["mcs_xyz"], mcs_tcc=r["mcs_tcc"],
         enepara=enepara_from_arrays(r))
     return cx, r
+
+
+def load_g2():
+    """Golden vector G2 (gpudocksm-rem-1.2/reference_result.txt:60-80):
+    (steps [10], movematrices [10, 6] float32, raw terms in slot order with
+    the total [10, 10] float64)."""
+    a = np.loadtxt(os.path.join(GOLDEN, "g2_trajectory.txt"), comments="#")
+    return a[:, 0].astype(int), a[:, 1:7].astype(np.float32), a[:, 7:17]
+
+
+def load_1a07c1_raw(lhm="1a07C1.ff"):
+    """The 1a07C1 complex read from its files by the library's loaders, with
+    the rem-1.2 weight file and unit normalisation (a = 1, b = 0), so that
+    the energy terms come out raw: the setting of G2."""
+    params = os.path.join(os.path.dirname(GOLDEN), "..", "gpudock_amd", "params")
+    ep0 = remc.load_enepara0(os.path.join(params, "gpudocksm.ff"),
+                             os.path.join(GOLDEN, "rem12_08ff_opt"))
+    for k in range(10):
+        ep0.a_para[k] = 1.0
+        ep0.b_para[k] = 0.0
+    return remc.load_complex(os.path.join(GOLDEN, "1a07C1.sdf"),
+                             os.path.join(GOLDEN, "1a07C.pdb"),
+                             os.path.join(GOLDEN, lhm), "1a07C1", ep0)

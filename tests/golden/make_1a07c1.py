@@ -73,6 +73,35 @@ def main():
     print("wrote", dst, os.path.getsize(dst), "bytes;", f"n_lig={n_lig} n_prt={n_prt} "
           f"lna={lna} pnp={pnp} pnk={pnk} pos={pos}")
 
+    # SetTemperature (util.C:490-532) ladders of the reference build: the
+    # minus_beta of every temperature for several ladder lengths and ranges
+    ladders = {}
+    for n_tmp in LADDER_SIZES:
+        for lo, hi in LADDER_RANGES:
+            subprocess.run([tool, os.path.join(d, "1a07C1", "1a07C1.sdf"),
+                            os.path.join(d, "1a07C1", "1a07C.pdb"),
+                            os.path.join(d, "1a07C1", "1a07C1-0.8.ff"),
+                            os.path.join(d, "parameters", "paras"),
+                            os.path.join(d, "parameters", "08ff_opt"),
+                            os.path.join(d, "parameters", "08_nor_a"),
+                            os.path.join(d, "parameters", "08_nor_b"),
+                            "1a07C1", str(n_tmp), lo, hi, tmp], check=True,
+                           stdout=subprocess.DEVNULL)
+            ladders[ladder_key(n_tmp, lo, hi)] = read_dump(tmp)["temp_minus_beta"]
+    dst = os.path.join(HERE, "set_temperature_ladders.npz")
+    np.savez_compressed(dst, **ladders)
+    print("wrote", dst, len(ladders), "ladders")
+
+
+LADDER_SIZES = (1, 2, 3, 8, 16, 32, 64)
+# the benchmark range, the reference's own command line (ceiling below floor)
+# and a narrow range
+LADDER_RANGES = (("0.3", "3.0"), ("0.044", "0.036"), ("0.02", "0.036"))
+
+
+def ladder_key(n_tmp, lo, hi):
+    return f"n{n_tmp}_{lo}_{hi}"
+
 
 if __name__ == "__main__":
     main()

@@ -30,6 +30,7 @@ def lib():
         L.orc_energy_ref64.argtypes = cx6 + [remc.ComplexSize, C.c_int, C.c_int, FP, C.c_int,
                                              C.POINTER(C.c_double)]
         L.orc_move_ref32.argtypes = [FP, FP, FP, FP, C.c_int, FP, FP, FP, FP, FP]
+        L.orc_canon_move.argtypes = [FP, FP, FP, FP, C.c_int, FP, FP, FP, FP, FP]
         L.orc_replay_energy.argtypes = cx6 + [remc.ComplexSize, C.c_int, IP, IP, FP, C.c_int, FP]
         L.orc_replay_run.argtypes = cx6 + [C.POINTER(remc.Temp), C.POINTER(remc.Replica),
                                            C.POINTER(remc.McPara), remc.ComplexSize,
@@ -38,12 +39,15 @@ def lib():
         L.orc_replay_run_rec.argtypes = L.orc_replay_run.argtypes + [C.c_int, C.c_void_p,
                                                                      C.c_int]
         L.orc_ref_run.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP, C.c_int,
-                                        C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int,
-                                        C.c_int, IP, IP, IP, FP, FP]
+                                        C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
+                                        C.c_int, C.c_int, IP, IP, IP, FP, FP]
         L.orc_ref_run_samples.argtypes = cx6 + [C.POINTER(remc.Temp), remc.ComplexSize, FP,
-                                                C.c_int, C.c_int, C.c_int, C.c_uint32,
-                                                C.c_uint32, C.c_int, C.c_int, IP, FP, IP,
-                                                C.c_int, C.c_int]
+                                                C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.c_uint32, C.c_uint32, C.c_int, C.c_int, IP,
+                                                IP, FP, IP, C.c_int, C.c_int]
+        L.orc_accept.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.orc_exchange_chain.argtypes = [C.c_int, C.c_int, FP, IP, FP, C.c_int, FP, IP]
+        L.orc_schedule.argtypes = [C.c_int, C.c_int, C.c_int, IP, C.c_int]
         L.orc_bayes_pdf.argtypes = [C.c_int, C.c_int, FP, C.c_float, C.c_float, FP]
         L.orc_bayes_total.argtypes = [FP]
         L.orc_bayes_total.restype = C.c_float
@@ -114,7 +118,8 @@ def replay_run(cx, temps, mc, opt):
 
 
 def ref_run(cx, temps, scale, n_steps, spe, exchange=1, seed=1, cid=0, term_order=0,
-            threads=0):
+            threads=0, spd=None):
+    """Reference-order run; spd (steps per dump window) None = n_steps."""
     size = cx.sized(len(temps))
     n = size.n_rep
     acs = np.zeros(n, np.int32)
@@ -123,27 +128,61 @@ def ref_run(cx, temps, scale, n_steps, spe, exchange=1, seed=1, cid=0, term_orde
     e = np.zeros((n, 10), np.float32)
     mv = np.zeros((n, 6), np.float32)
     sc = np.ascontiguousarray(scale, np.float32)
-    lib().orc_ref_run(*cx.args(), temps, size, _fp(sc), n_steps, spe, exchange, seed, cid,
-                      term_order, threads, _ip(acs), _ip(acx), _ip(it), _fp(e), _fp(mv))
+    spd = spd or max(n_steps, 1)
+    rc = lib().orc_ref_run(*cx.args(), temps, size, _fp(sc), n_steps, spd, spe, exchange,
+                           seed, cid, term_order, threads, _ip(acs), _ip(acx), _ip(it), _fp(e),
+                           _fp(mv))
+    assert rc == 0
     return dict(acs_mc=acs, acs_exchg=acx, idx_tmp_final=it, energy_final=e, move_final=mv)
 
 
 def ref_run_samples(cx, temps, scale, n_steps, spe, stride, exchange=0, seed=1, cid=0,
-                    term_order=0, threads=0):
-    """Reference-order run sampled every `stride` steps:
-    (acs_mc [n_rep], energies [n_rep, n_samp], temperature index [n_rep, n_samp])."""
+                    term_order=0, threads=0, spd=None):
+    """Reference-order run sampled every `stride` steps: (acs_mc [n_rep],
+    energies [n_rep, n_samp], temperature index [n_rep, n_samp], acs_exchg [n_rep])."""
     size = cx.sized(len(temps))
     n = size.n_rep
     n_samp = (n_steps + stride - 1) // stride
     acs = np.zeros(n, np.int32)
+    acx = np.zeros(n, np.int32)
     es = np.zeros((n, n_samp), np.float32)
     ts = np.zeros((n, n_samp), np.int32)
     sc = np.ascontiguousarray(scale, np.float32)
-    rc = lib().orc_ref_run_samples(*cx.args(), temps, size, _fp(sc), n_steps, spe, exchange,
-                                   seed, cid, term_order, threads, _ip(acs), _fp(es), _ip(ts),
-                                   stride, n_samp)
+    spd = spd or max(n_steps, 1)
+    rc = lib().orc_ref_run_samples(*cx.args(), temps, size, _fp(sc), n_steps, spd, spe,
+                                   exchange, seed, cid, term_order, threads, _ip(acs),
+                                   _ip(acx), _fp(es), _ip(ts), stride, n_samp)
     assert rc == 0
-    return acs, es, ts
+    return acs, es, ts, acx
+
+
+def accept(e_new, e_old, minus_beta, u, canonical):
+    """The oracle's Metropolis decision (Accept_d, accept_d.cu:22-23)."""
+    return bool(lib().orc_accept(int(canonical), e_new, e_old, minus_beta, u))
+
+
+def exchange_chain(minus_beta, temps, ener, mode, u, canonical):
+    """The oracle's exchange of one temperature chain with the draws u[pair]:
+    (temperature index held by each slot after it, swaps per slot)."""
+    mb = np.ascontiguousarray(minus_beta, np.float32)
+    t = np.array(temps, np.int32)
+    e = np.ascontiguousarray(ener, np.float32)
+    uu = np.ascontiguousarray(u, np.float32)
+    acs = np.zeros(len(t), np.int32)
+    lib().orc_exchange_chain(int(canonical), len(t), _fp(mb), _ip(t), _fp(e), int(mode),
+                             _fp(uu), _ip(acs))
+    return t, acs
+
+
+def schedule(steps_total, spd, spe):
+    """The launch sequence of the oracle's run loops: exchange mode of each
+    launch (launch x runs MC steps [x spe, (x + 1) spe)); None if invalid."""
+    n = lib().orc_schedule(steps_total, spd, spe, None, 0)
+    if n < 0:
+        return None
+    modes = np.zeros(max(n, 1), np.int32)
+    lib().orc_schedule(steps_total, spd, spe, _ip(modes), n)
+    return modes[:n]
 
 
 def philox(ctr, key):

@@ -177,7 +177,11 @@ def test_server_results_equal_alone_under_continuous_batching(jobs):
     srv.close()
     assert st["completed"] == len(js) + len(extra)
     assert st["failed"] == 0 and st["p99_latency"] >= st["p50_latency"] > 0.0
-    assert st["pose_evals"] == sum(j.n_rep * j.mc.steps_total for j in js + extra)
+    # pose evaluations = the MC steps each job's schedule runs (whole launches,
+    # gpusingle.cu:8-20) x its replicas
+    runs = [len(O.schedule(j.mc.steps_total, j.mc.steps_per_dump, j.mc.steps_per_exchange)) *
+            j.mc.steps_per_exchange for j in js + extra]
+    assert st["pose_evals"] == sum(j.n_rep * r for j, r in zip(js + extra, runs))
 
 
 def test_server_admission_by_footprint_and_cancel(jobs):

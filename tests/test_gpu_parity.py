@@ -166,6 +166,33 @@ def test_run_variants(n_tmp, exchange, away, term_order, steps, spe):
     _compare_runs(g, r)
 
 
+@pytest.mark.parametrize("steps, spd, spe, n_tmp", [
+    (100, 50, 10, 4),    # five launches per window: the mode restarts in each window
+    (60, 25, 10, 5),     # spe does not divide the window: 3 windows x 3 launches, 90 steps
+    (45, 20, 20, 3),     # the last, partial window runs whole: 60 steps
+])
+def test_reference_schedule_windows(steps, spd, spe, n_tmp):
+    """The reference's launch loop (gpusingle.cu:8-20): whole steps_per_dump
+    windows of whole launches, the exchange mode restarting at 0 in each;
+    every decision equals the replay's, and the MC steps run are those of
+    the schedule."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=1, n_tmp=n_tmp, steps=steps,
+                                           spe=spe)
+    mc.steps_per_dump = spd
+    opt = remc.default_options(seed=13, complex_id=2)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    s = remc.Session(cx, temps, mc, opt)
+    n_launch = len(O.schedule(steps, spd, spe))
+    s.cycles(n_launch + 3)          # cycles past the end do nothing
+    s.sync()
+    assert s.steps_done() == n_launch * spe
+    h, _ = s.results()
+    s.close()
+    _compare_runs(h, r)
+
+
 def test_session_equals_run():
     cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=1, steps=60)
     opt = remc.default_options(seed=9)
@@ -288,3 +315,21 @@ def test_class2_points_outside_the_hydrophobic_class():
     g, _ = remc.run(cx, temps, mc, opt)
     r = O.replay_run(cx, temps, mc, opt)
     _compare_runs(g, r)
+
+
+def test_g2_trajectory_vector():
+    """Golden vector G2 (gpudocksm-rem-1.2/reference_result.txt:60-80): the
+    GPU's energies of the ten poses of the reference's deterministic path on
+    1a07C1 equal the canonical replay bit for bit and the reference's printed
+    raw terms and total within 1e-6 of max(|E|, 1) each."""
+    steps, mv, g2 = fixtures.load_g2()
+    cx = fixtures.load_1a07c1_raw()
+    z = np.zeros(len(steps), np.int32)
+    e, e64 = remc.score_poses(cx, z, z, mv)
+    np.testing.assert_array_equal(e, O.replay_energy(cx, z, z, mv))
+    assert np.all(np.abs(e - g2) <= 1e-6 * np.maximum(np.abs(g2), 1.0))
+    floor = 2.0 * 2.0 ** -23 * float(np.max(np.abs(cx.lig_center)))
+    tol = 1e-6 * np.maximum(np.abs(g2), 1.0)
+    tol[:, 8] += floor
+    assert np.all(np.abs(e64 - g2) <= tol)
+

@@ -1,12 +1,33 @@
-"""Long-run statistics: the GPU chains against the reference algorithm.
+"""Long-run statistics of the GPU chains, with temperature exchange on.
 
-The GPU run (canonical FP32) and the reference-order CPU run (libm expf,
-IEEE division and square root, the reference's loop order) use independent
-Philox seeds, so their trajectories are independent samples.  Per
-temperature, the mean energy over the second half of each chain and each
-chain's acceptance rate must agree between the two runs (paired t-test on the
-16 chains of each side, one per protein conformer, paired by conformer).
+1. Against the exact Boltzmann distribution.  With every weight zero except
+   the distance term, the total energy is E = K r + B, r = |translation|
+   (the ligand centre starts on the pocket centre), rotations are free, and
+   the uniform symmetric move of the reference (move_d.cu:79-84) with its
+   Metropolis rule (accept_d.cu:22-23) samples p(r) ~ r^2 exp(-K r / T): r is
+   Gamma(3, T / K) at every temperature, and an exchange between
+   temperatures i and i+1 is accepted with probability
+   E[min(1, exp((1/T_i - 1/T_{i+1})(E_i - E_{i+1})))] over two independent
+   such draws (exchangereplicas_d.cu:57-66).  The GPU run and the
+   reference-order CPU run (libm expf, independent seed) must both match,
+   per temperature: the mean of r (t-test on per-chain means) and its
+   histogram in 8 bins of equal exact probability (Hotelling T^2 on
+   per-chain bin fractions); per adjacent pair: the exchange acceptance
+   (t-test on per-chain rates).  The same GPU run on a ladder 5 % hotter
+   (every beta 5 % off) must fail the mean test -- the power check.
+2. The full energy on 128 identical chains of a small complex: per
+   temperature, the GPU's and the reference-order run's energies agree in
+   distribution (two-sample KS distance, p-value by permuting whole chains,
+   which are independent while their samples are not) and in mean (Welch t on
+   chain means); per adjacent pair, their exchange acceptance agrees.
+
+Samples are the per-step records (state before every 10th step) after a
+burn-in; chains are i.i.d. by construction (identical conformers), so tests
+over per-chain statistics are valid whatever the autocorrelation inside a
+chain.  Seeds are fixed: the outcome is deterministic.
 """
+import math
+
 import numpy as np
 import pytest
 from scipy import stats
@@ -16,43 +37,200 @@ from gpudock_amd import remc, synth, traj
 
 pytestmark = pytest.mark.gpu
 
-N_STEPS = 4000
-STRIDE = 50
+N_TMP = 8
+SPE = 10
+STRIDE = 10
+P_MIN = 1e-4     # per check; the seeds are fixed, so the outcome is too
 
 
-def test_energy_and_acceptance_statistics_match_reference(tmp_path):
-    cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=16, steps=N_STEPS)
-    n_tmp, n_prt = cfg["n_tmp"], 16
-    mc.steps_per_dump = N_STEPS
-    prefix = str(tmp_path / "t")
-    opt = remc.default_options(seed=101, exchange=0, away=0, record_stride=STRIDE,
+def _chains(base, n_chain, enepara=None):
+    """n_chain identical copies of ligand conformer 0 (one protein conformer):
+    chains of replicas that are independent and identically distributed."""
+    return remc.Complex(
+        lig_xyz=np.repeat(base.lig_xyz[:1], n_chain, 0), lig_t=base.lig_t, lig_c=base.lig_c,
+        lig_n=base.lig_n, lig_center=base.lig_center, prt_xyz=base.prt_xyz[:1],
+        prt_t=base.prt_t[:1], prt_c=base.prt_c[:1], prt_ele=base.prt_ele[:1],
+        prt_seq3r=base.prt_seq3r[:1], prt_hpcflag=base.prt_hpcflag[:1],
+        prt_hpp=base.prt_hpp[:1], pocket_center=base.pocket_center, psp=base.psp,
+        kde_xyz=base.kde_xyz, kde_t=base.kde_t, mcs_xyz=base.mcs_xyz, mcs_tcc=base.mcs_tcc,
+        enepara=enepara if enepara is not None else base.enepara)
+
+
+def _gpu_samples(cx, temps, mc, seed, tmp_path, tag):
+    prefix = str(tmp_path / tag)
+    opt = remc.default_options(seed=seed, exchange=1, record_stride=STRIDE,
                                record_prefix=prefix)
-    g, _ = remc.run(cx, temps, mc, opt)
+    remc.run(cx, temps, mc, opt)
     _, rec = traj.read_gdtraj(prefix + "_0000.gdtraj")
-    e_gpu = rec["energy"][:, :, 9]                 # [n_rep, n_samp] total energy
-    t_gpu = rec["idx_tmp"]
+    return rec["energy"][:, :, 9].astype(np.float64), rec["idx_tmp"]
 
-    acs_ref, e_ref, t_ref = O.ref_run_samples(cx, temps, list(mc.move_scale), N_STEPS,
-                                              cfg["spe"], STRIDE, exchange=0, seed=202,
-                                              threads=16)
-    assert np.array_equal(t_gpu[:, 0], t_ref[:, 0])
-    half = e_gpu.shape[1] // 2
-    # replica r = n_tmp * n_lig * p + n_lig * t + l (one ligand conformer)
-    tmp_of = t_gpu[:, 0]
-    for t in range(n_tmp):
-        rows = np.nonzero(tmp_of == t)[0]
-        assert len(rows) == n_prt
-        m_gpu = e_gpu[rows, half:].astype(np.float64).mean(axis=1)
-        m_ref = e_ref[rows, half:].astype(np.float64).mean(axis=1)
-        p_e = stats.ttest_rel(m_gpu, m_ref).pvalue
-        a_gpu = g["acs_mc"][rows] / N_STEPS
-        a_ref = acs_ref[rows] / N_STEPS
-        p_a = stats.ttest_rel(a_gpu, a_ref).pvalue
-        print(f"T{t}: energy {m_gpu.mean():.4f} / {m_ref.mean():.4f} (p {p_e:.3f}), "
-              f"acceptance {a_gpu.mean():.4f} / {a_ref.mean():.4f} (p {p_a:.3f})")
-        assert p_e > 1e-3, (t, m_gpu.mean(), m_ref.mean(), p_e)
-        assert p_a > 1e-3, (t, a_gpu.mean(), a_ref.mean(), p_a)
-    # acceptance rises with temperature on both sides
-    a_gpu_t = [g["acs_mc"][tmp_of == t].mean() for t in range(n_tmp)]
-    a_ref_t = [acs_ref[tmp_of == t].mean() for t in range(n_tmp)]
-    assert a_gpu_t[-1] > a_gpu_t[0] and a_ref_t[-1] > a_ref_t[0]
+
+def _ref_samples(cx, temps, mc, seed):
+    _, es, ts, _ = O.ref_run_samples(cx, temps, list(mc.move_scale), mc.steps_total, SPE,
+                                     STRIDE, exchange=1, seed=seed, threads=16)
+    return es.astype(np.float64), ts
+
+
+def _by_chain(e, t, n_chain, burn):
+    """[chain][temperature] -> energies held at that temperature after burn-in
+    (replica slot n_lig * t + l of chain l, one protein conformer)."""
+    j0 = burn // STRIDE
+    out = []
+    for c in range(n_chain):
+        slots = [n_chain * k + c for k in range(N_TMP)]
+        E, T = e[slots, j0:], t[slots, j0:]
+        out.append([E[T == k] for k in range(N_TMP)])
+    return out
+
+
+def _pair_rates(t, n_chain, burn, modes):
+    """Per chain and adjacent temperature pair: accepted / attempted exchanges,
+    read from the temperature each slot holds before consecutive records (one
+    record per exchange: STRIDE == SPE)."""
+    j0 = burn // STRIDE
+    rates = np.zeros((n_chain, N_TMP - 1))
+    for c in range(n_chain):
+        T = t[[n_chain * k + c for k in range(N_TMP)]]
+        acc = np.zeros(N_TMP - 1)
+        att = np.zeros(N_TMP - 1)
+        for j in range(j0, T.shape[1] - 1):
+            slot_of = np.argsort(T[:, j])      # slot holding each temperature
+            for i in range(modes[j], N_TMP - 1, 2):
+                att[i] += 1
+                acc[i] += T[slot_of[i], j + 1] == i + 1
+        rates[c] = acc / att
+    return rates
+
+
+# --------------------------------------------------- 1. exact distribution
+K_SPRING = 5.0
+N_CHAIN_EXACT = 128
+STEPS_EXACT = 6000
+BURN_EXACT = 1000
+
+
+def _spring_complex():
+    base = synth.make_complex(pnp=40, radius=6.0, lna=6, n_lig=1, n_prt=1, pnk=40, seed=7,
+                              pos=2)
+    ep = base.enepara
+    assert np.all(base.lig_center == 0) and np.all(base.pocket_center == 0)
+    a_dst, b_dst = ep.a_para[5], ep.b_para[5]           # file order: dst is term 5
+    for k in range(10):
+        ep.w[k] = 0.0
+    ep.w[5] = K_SPRING / a_dst
+    return _chains(base, N_CHAIN_EXACT, ep), float(ep.w[5] * b_dst)
+
+
+def _exact_pair_acceptance(T, rng, n=2_000_000):
+    """E[min(1, exp((1/T_i - 1/T_j)(E_i - E_j)))], E = K r, r ~ Gamma(3, T / K)."""
+    out = []
+    for i in range(len(T) - 1):
+        ei = K_SPRING * rng.gamma(3.0, T[i] / K_SPRING, n)
+        ej = K_SPRING * rng.gamma(3.0, T[i + 1] / K_SPRING, n)
+        x = (1.0 / T[i] - 1.0 / T[i + 1]) * (ei - ej)
+        out.append(np.mean(np.exp(np.minimum(x, 0.0))))
+    return np.array(out)
+
+
+def _exact_checks(e, t, T, B, modes, exact_acc):
+    """(z of the mean of r per temperature, Hotelling p of its histogram per
+    temperature, t-test p of the exchange rate per pair)."""
+    ch = _by_chain(e, t, N_CHAIN_EXACT, BURN_EXACT)
+    z, p_hist = [], []
+    for k in range(N_TMP):
+        r = [(c[k] - B) / K_SPRING for c in ch]
+        m = np.array([v.mean() for v in r])
+        z.append((m.mean() - 3.0 * T[k] / K_SPRING) / (m.std(ddof=1) / math.sqrt(len(m))))
+        # per-chain fractions in 8 bins of exact probability 1/8 (7 free)
+        fr = np.array([np.histogram(stats.gamma.cdf(v, 3.0, scale=T[k] / K_SPRING),
+                                    bins=np.linspace(0.0, 1.0, 9))[0] / len(v) for v in r])
+        Y = fr[:, :7] - 0.125
+        mu = Y.mean(0)
+        S = np.cov(Y, rowvar=False)
+        n, p = len(Y), 7
+        t2 = n * mu @ np.linalg.solve(S, mu)
+        p_hist.append(stats.f.sf((n - p) / (p * (n - 1)) * t2, p, n - p))
+    rates = _pair_rates(t, N_CHAIN_EXACT, BURN_EXACT, modes)
+    p_x = [stats.ttest_1samp(rates[:, i], exact_acc[i]).pvalue for i in range(N_TMP - 1)]
+    return np.array(z), np.array(p_hist), np.array(p_x)
+
+
+def test_boltzmann_distribution_and_exchange_rates(tmp_path):
+    cx, B = _spring_complex()
+    temps = remc.temperatures(N_TMP, 0.3, 3.0)
+    T = -1.0 / np.array([x.minus_beta for x in temps], np.float64)
+    mc = remc.mcpara(STEPS_EXACT, SPE, [0.2] * 3 + [0.2] * 3)
+    modes = O.schedule(STEPS_EXACT, STEPS_EXACT, SPE)
+    exact_acc = _exact_pair_acceptance(T, np.random.default_rng(1))
+    z_all = {}
+    for name, (e, t) in (("gpu", _gpu_samples(cx, temps, mc, 101, tmp_path, "g")),
+                         ("reference", _ref_samples(cx, temps, mc, 202))):
+        z, p_hist, p_x = _exact_checks(e, t, T, B, modes, exact_acc)
+        print(f"{name}: mean z {np.round(z, 2)}, histogram p {np.round(p_hist, 3)}, "
+              f"exchange p {np.round(p_x, 3)}")
+        assert np.all(2.0 * stats.norm.sf(np.abs(z)) > P_MIN), (name, z)
+        assert np.all(p_hist > P_MIN), (name, p_hist)
+        assert np.all(p_x > P_MIN), (name, p_x)
+        z_all[name] = z
+    # power: the same run on a ladder 5 % hotter, read against the exact
+    # distribution of the nominal ladder, fails the mean test
+    hot = remc.temperatures(N_TMP, 0.3 * 1.05, 3.0 * 1.05)
+    e, t = _gpu_samples(cx, hot, mc, 303, tmp_path, "h")
+    z, _, _ = _exact_checks(e, t, T, B, modes, exact_acc)
+    print(f"gpu, beta 5 % low: mean z {np.round(z, 2)}")
+    assert stats.chi2.sf(float(np.sum(z ** 2)), N_TMP) < 1e-10
+
+
+# ------------------------------------------------ 2. full energy, GPU vs CPU
+N_CHAIN_FULL = 128
+STEPS_FULL = 2000
+BURN_FULL = 500
+
+
+def _perm_ks(ga, gb, rng, n_perm=400):
+    """KS distance of the pooled samples of two sets of chains, with its
+    p-value from random re-assignments of whole chains to the two sets."""
+    groups = ga + gb
+    lens = np.array([len(g) for g in groups])
+    allv = np.concatenate(groups)
+    owner = np.repeat(np.arange(len(groups)), lens)
+    sel0 = np.arange(len(groups)) < len(ga)
+
+    def ks(sel):
+        m = sel[owner]
+        return stats.ks_2samp(allv[m], allv[~m]).statistic
+
+    obs = ks(sel0)
+    null = np.array([ks(rng.permutation(sel0)) for _ in range(n_perm)])
+    return (1 + np.sum(null >= obs)) / (1 + n_perm)
+
+
+def test_full_energy_statistics_match_reference(tmp_path):
+    base = synth.make_complex(pnp=120, radius=10.0, lna=12, n_lig=1, n_prt=1, pnk=100, seed=7)
+    cx = _chains(base, N_CHAIN_FULL)
+    temps = remc.temperatures(N_TMP, 0.3, 3.0)
+    mc = remc.mcpara(STEPS_FULL, SPE, [0.2] * 3 + [math.radians(3.0)] * 3)
+    modes = O.schedule(STEPS_FULL, STEPS_FULL, SPE)
+    eg, tg = _gpu_samples(cx, temps, mc, 101, tmp_path, "f")
+    er, tr = _ref_samples(cx, temps, mc, 202)
+    assert np.array_equal(tg[:, 0], tr[:, 0])
+    cg = _by_chain(eg, tg, N_CHAIN_FULL, BURN_FULL)
+    cr = _by_chain(er, tr, N_CHAIN_FULL, BURN_FULL)
+    rng = np.random.default_rng(3)
+    for k in range(N_TMP):
+        ga, gb = [c[k] for c in cg], [c[k] for c in cr]
+        p_ks = _perm_ks(ga, gb, rng)
+        ma = np.array([v.mean() for v in ga])
+        mb = np.array([v.mean() for v in gb])
+        p_m = stats.ttest_ind(ma, mb, equal_var=False).pvalue
+        print(f"T{k}: mean {ma.mean():.4f} / {mb.mean():.4f} (p {p_m:.3f}), KS p {p_ks:.3f}")
+        assert p_ks > P_MIN and p_m > P_MIN, (k, p_ks, p_m)
+    ra = _pair_rates(tg, N_CHAIN_FULL, BURN_FULL, modes)
+    rb = _pair_rates(tr, N_CHAIN_FULL, BURN_FULL, modes)
+    for i in range(N_TMP - 1):
+        p = stats.ttest_ind(ra[:, i], rb[:, i], equal_var=False).pvalue
+        print(f"pair {i}-{i + 1}: exchange {ra[:, i].mean():.4f} / {rb[:, i].mean():.4f} "
+              f"(p {p:.3f})")
+        assert p > P_MIN, (i, p)
+    # acceptance of the hottest exchange pair exceeds the coldest (both sides)
+    assert ra[:, -1].mean() > ra[:, 0].mean() and rb[:, -1].mean() > rb[:, 0].mean()

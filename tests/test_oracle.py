@@ -40,6 +40,68 @@ def test_g1_pins_term_order():
     assert abs(e[9] - fixtures.G1_TOTAL) > 1.0
 
 
+# ---------------------------------------------------------------- golden G2
+# gpudocksm-rem-1.2/reference_result.txt:60-80: raw terms of replica 0 of
+# 1a07C1 along ten poses of a deterministic path with all three rotation
+# angles non-zero.  The reference printed FP32 values; they agree with the
+# FP64 oracle to 3e-7 of max(|E|, 1) except the distance term, whose FP32
+# value carries the rounding of centre + translation (one ulp of the ~43 A
+# centre coordinates, 3.8e-6).
+G2_REL = 1e-6
+
+
+def _g2_dst_floor(cx):
+    return 2.0 * 2.0 ** -23 * float(np.max(np.abs(cx.lig_center)))
+
+
+def test_g2_fp64_oracle():
+    """Move (rotation about the ligand centre, ZYX), all nine raw terms, the
+    LHM restraints of the full 1a07C1.ff, and the file-order weighted total,
+    in float64, at every step of G2."""
+    steps, mv, g2 = fixtures.load_g2()
+    cx = fixtures.load_1a07c1_raw()
+    z = np.zeros(len(steps), np.int32)
+    e = O.pose_ref64(cx, z, z, mv)
+    tol = G2_REL * np.maximum(np.abs(g2), 1.0)
+    tol[:, 8] += _g2_dst_floor(cx)
+    assert np.all(np.abs(e - g2) <= tol), np.abs(e - g2) / np.maximum(np.abs(g2), 1.0)
+
+
+def test_g2_canonical_replay():
+    """The canonical FP32 arithmetic the GPU runs (bitwise, tested on the GPU)
+    reproduces G2 within 1e-6 of max(|E|, 1) on every term."""
+    steps, mv, g2 = fixtures.load_g2()
+    cx = fixtures.load_1a07c1_raw()
+    z = np.zeros(len(steps), np.int32)
+    e = O.replay_energy(cx, z, z, mv)
+    assert np.all(np.abs(e - g2) <= G2_REL * np.maximum(np.abs(g2), 1.0))
+
+
+def test_g2_reference_order_fp32():
+    """The reference-order restatement: within 1e-5 of max(|E|, 1) plus its
+    own sequential-sum rounding |E32 - E64| (its PMF sum is the least
+    accurate, 2e-5 relative)."""
+    steps, mv, g2 = fixtures.load_g2()
+    cx = fixtures.load_1a07c1_raw()
+    z = np.zeros(len(steps), np.int32)
+    e32 = O.pose_ref32(cx, z, z, mv)
+    e64 = O.pose_ref64(cx, z, z, mv)
+    tol = 1e-5 * np.maximum(np.abs(g2), 1.0) + np.abs(e32 - e64)
+    tol[:, 8] += _g2_dst_floor(cx)
+    assert np.all(np.abs(e32 - g2) <= tol)
+
+
+def test_g2_pins_lhm_file_and_term_order():
+    """G2's LHM term comes from 1a07C1.ff (1a07C1-0.8.ff misses it by 0.06),
+    and its total weights the terms in file order (slot order misses by 0.5)."""
+    steps, mv, g2 = fixtures.load_g2()
+    z = np.zeros(len(steps), np.int32)
+    e = O.replay_energy(fixtures.load_1a07c1_raw("1a07C1-0.8.ff"), z, z, mv)
+    assert np.abs(e[:, 7] - g2[:, 7]).max() > 0.03
+    e = O.replay_energy(fixtures.load_1a07c1_raw(), z, z, mv, term_order=remc.TERM_ORDER_SLOT)
+    assert np.abs(e[:, 9] - g2[:, 9]).min() > 0.3
+
+
 def test_reference_prepared_start_pose():
     """InitLigCoord (util.C:229-247): coord_new = coord_orig + centre; the
     oracle's move at movematrix 0 must give exactly that pose."""
@@ -331,17 +393,3 @@ def test_temperature_ladder_geometric():
     ratios = T[1:] / T[:-1]
     np.testing.assert_allclose(ratios, ratios[0], rtol=1e-6)
     np.testing.assert_allclose([x.minus_beta for x in t], -1.0 / T, rtol=1e-6)
-
-
-def test_exchange_pairs_cover_ladder():
-    """Mode 0 pairs (0,1),(2,3)..; mode 1 pairs (1,2),(3,4)..
-    (exchangereplicas_d.cu:43-46): together every adjacent pair once."""
-    for n in range(1, 12):
-        pairs = set()
-        for mode in (0, 1):
-            maxpair = n // 2 - (1 if (n % 2 == 0 and mode == 1) else 0)
-            for j in range(maxpair):
-                i1 = 2 * j + mode
-                assert i1 + 1 < n
-                pairs.add(i1)
-        assert pairs == set(range(n - 1))

@@ -60,6 +60,20 @@ def test_set_temperature_matches_reference():
     assert np.float32(t[0].minus_beta) == r["temp_minus_beta"][0]
 
 
+@pytest.mark.parametrize("n_tmp", [1, 2, 3, 8, 16, 32, 64])
+@pytest.mark.parametrize("lo, hi", [("0.3", "3.0"), ("0.044", "0.036"), ("0.02", "0.036")])
+def test_set_temperature_ladders_match_reference(n_tmp, lo, hi):
+    """Every minus_beta of the geometric ladder equals the reference build's
+    SetTemperature (util.C:490-532) bit for bit, for the ladder lengths the
+    configurations use and beyond (tests/golden/make_1a07c1.py writes them)."""
+    lad = np.load(os.path.join(fixtures.GOLDEN, "set_temperature_ladders.npz"))
+    want = lad[f"n{n_tmp}_{lo}_{hi}"]
+    t = remc.temperatures(n_tmp, float(lo), float(hi))
+    got = np.array([x.minus_beta for x in t], np.float32)
+    np.testing.assert_array_equal(got, want)
+    assert [x.order for x in t] == list(range(n_tmp))
+
+
 def test_set_replica_layout():
     size = remc.ComplexSize(3, 2, 4, 24, 5, 10, 0, 0)
     arr = (remc.Replica * 24)()
@@ -69,6 +83,34 @@ def test_set_replica_layout():
             for l in range(3):
                 f = 4 * 3 * p + 3 * t + l
                 assert (arr[f].idx_rep, arr[f].idx_prt, arr[f].idx_tmp, arr[f].idx_lig) == (f, p, t, l)
+
+
+def test_replica_table_must_hold_whole_chains():
+    """The exchange inverts each chain's temperature indices
+    (exchangereplicas_d.cu:34-38): a caller's replica table is accepted when
+    every chain slot holds its own conformers and the chain's temperatures
+    are a permutation, and refused otherwise (before anything reaches a GPU)."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=2, n_tmp=3, steps=10)
+    size = cx.sized(3)
+
+    def slot(p, t, l):
+        return 3 * 2 * p + 2 * t + l
+
+    rep = remc.replica_table(size)
+    assert remc.JobSpec(cx, temps, mc, replica=rep).footprint() > 0
+    # a permutation of the temperatures of chain (p 1, l 0) is a valid state
+    rep[slot(1, 0, 0)].idx_tmp, rep[slot(1, 2, 0)].idx_tmp = 2, 0
+    assert remc.JobSpec(cx, temps, mc, replica=rep).footprint() > 0
+    # the same temperature twice in chain (p 0, l 1)
+    rep = remc.replica_table(size)
+    rep[slot(0, 1, 1)].idx_tmp = 0
+    with pytest.raises(RuntimeError, match="appears twice"):
+        remc.JobSpec(cx, temps, mc, replica=rep).footprint()
+    # a slot of chain (p 0, l 1) holding protein conformer 1
+    rep = remc.replica_table(size)
+    rep[slot(0, 2, 1)].idx_prt = 1
+    with pytest.raises(RuntimeError, match="belongs to the chain"):
+        remc.JobSpec(cx, temps, mc, replica=rep).footprint()
 
 
 def test_bad_inputs_are_rejected_without_a_gpu():
