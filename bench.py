@@ -10,11 +10,14 @@ conformers x 16 temperatures = 16,384 replicas, 2,000 KDE points, exchange
 every 10 steps; 500 cycles are the full 5,000-step run.
 
 Prints one JSON line: pose energy evals/s (= MC replica-steps/s), the
-roofline of the MC kernel (FP32 VALU), and the reference-order CPU oracle
-timed on this host on a bounded sample of the same complex.
---workload batch (256 uneven complexes in one engine) and --workload serve
-(Poisson job arrivals into the serving loop, p50/p99 latency) print their
-own lines.
+roofline of the MC kernel (FP32 VALU), and the reference-order CPU
+restatement built for and timed on this host on a bounded sample of the
+same complex.  The same line carries two more legs (--legs): "batched", the
+256 uneven complexes in one engine with fewer steps and their speed-up over
+the plain one-complex-per-call path, and "serving", Poisson job arrivals at
+200 jobs/s into the serving loop for 30 s with p50/p99 job latency.
+--workload batch and --workload serve run those in full and print their own
+lines (serve adds the sustainable completion rate and p50/p99 there).
 """
 import argparse
 import json
@@ -43,32 +46,119 @@ def flops_per_pose(cx):
     return 60 * lna * pnp + 12 * m_kde + 10 * pos * lna + 8 * lna + 40, m_kde
 
 
-def cpu_baseline(cx, temps_all, scale, cfg, seed):
-    """Reference-order FP32 oracle (the cpu_mic loop order, libm math) on
-    this host: 1 thread and all cores, on a bounded sample of replicas x
-    steps of the same complex.  Returns pose evals/s."""
+def host_cpu():
+    """The host's CPU as the CPU baseline reports it: model, logical CPUs in
+    the affinity mask, the thread share used (OMP_NUM_THREADS when set: the
+    CPU share that goes with the GPU), the clock and the FP32 FLOPs per cycle
+    per core assumed for the host's peak."""
+    model, mhz, flags = "unknown", None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model == "unknown":
+                    model = v.strip()
+                elif k == "cpu MHz" and mhz is None:
+                    mhz = float(v)
+                elif k == "flags" and not flags:
+                    flags = set(v.split())
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cpufreq/cpuinfo_max_freq") as f:
+            mhz = int(f.read()) / 1000.0
+        clock_src = "cpuinfo_max_freq"
+    except (OSError, ValueError):
+        clock_src = "/proc/cpuinfo cpu MHz"
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = max(1, min(visible, int(share))) if share and share.isdigit() else visible
+    # FP32 FLOPs per cycle per core: two FMA pipes of the widest vector the
+    # core executes at full rate (AVX-512 on Zen 5 and on Xeon with two
+    # 512-bit FMA units, 256-bit otherwise)
+    import re
+    epyc = re.search(r"EPYC (\d{4})", model)
+    zen4 = bool(epyc) and epyc.group(1)[-1] == "4"    # 512-bit ops on 256-bit pipes
+    wide = "avx512f" in flags and not zen4
+    fpc = 64 if wide else (32 if "fma" in flags else 16)
+    return {"model": model, "cpus_visible": visible, "threads": threads,
+            "thread_limit": "OMP_NUM_THREADS" if share else "affinity mask",
+            "clock_ghz": (mhz or 0.0) / 1000.0, "clock_source": clock_src,
+            "fp32_flops_per_cycle_per_core": fpc,
+            "peak_note": "threads x clock x FLOPs/cycle (2 FMA pipes x vector width); "
+                         "SMT threads counted as cores"}
+
+
+def native_oracle_lib():
+    """The reference-order CPU restatement (oracle/gd_oracle.c) compiled for
+    this host: gcc -O3 -march=native, contraction off, no fast-math, OpenMP.
+    Built into a temporary directory at run time, because the shipped build
+    is portable; returns (path, flags) or (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    gcc = shutil.which("gcc")
+    if not gcc:
+        return None, "no gcc on this host: portable -O2 build used"
+    d = tempfile.mkdtemp(prefix="gd_cpu_")
+    out = os.path.join(d, "liboracle_native.so")
+    flags = ["-O3", "-march=native", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
+             "-fopenmp", "-fPIC", "-shared"]
+    cmd = [gcc] + flags + ["-I", os.path.join(ROOT, "gpudock_amd", "csrc"),
+                           "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "oracle", "gd_oracle.c"), "-o", out, "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return None, "native build failed: portable -O2 build used"
+    return out, " ".join(flags)
+
+
+def cpu_baseline(cx, temps_all, scale, cfg, seed, f_pose):
+    """Reference-order FP32 restatement (the cpu_mic loop order, libm math)
+    on this host, built for it: 1 thread and the host's thread share, each on
+    a bounded sample of replicas x steps of the same complex (about 10 s of
+    CPU work each).  Returns the cpu_baseline record."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from gpudock_amd import remc, synth
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    out = {}
-    # about 10 s of CPU work each: 1 thread ~2.6e3 and 16 threads ~3.6e4
-    # pose evals/s on this complex
-    for threads, n_lig, steps in ((1, 8, 800), (cores, 64, 1500 * cores // 16)):
+    lib, build = native_oracle_lib()
+    if lib:
+        O.ORACLE_SO = lib
+    info = host_cpu()
+    threads = info["threads"]
+    runs = {}
+    # 1 thread ~2.6e3 and 16 threads ~3.6e4 pose evals/s on this complex
+    for n_thr, n_lig, steps in ((1, 8, 800), (threads, 64, max(100, 1500 * threads // 16))):
         sub = synth.subset(cx, n_lig, 1)
         t_sub = remc.temperatures(4, cfg["t_lo"], cfg["t_hi"])
         n_rep = n_lig * 4
         t0 = time.perf_counter()
-        O.ref_run(sub, t_sub, scale, steps, cfg["spe"], exchange=1, seed=seed,
-                  threads=threads)
+        O.ref_run(sub, t_sub, scale, steps, cfg["spe"], exchange=1, seed=seed, threads=n_thr)
         dt = time.perf_counter() - t0
-        out[threads] = (n_rep * steps / dt, n_rep, steps, dt)
-    return out, cores
+        runs[n_thr] = (n_rep * steps / dt, n_rep, steps, dt)
+    vn, v1 = runs[threads], runs[1]
+    pairs = cfg["lna"] * cfg["pnp"]
+    peak = threads * info["clock_ghz"] * 1e9 * info["fp32_flops_per_cycle_per_core"]
+    return {
+        "value": vn[0], "unit": "pose evals/s", "cores": threads, "kind": "port",
+        "sample": f"reference-order FP32 restatement, same complex (protein conformer 0), "
+                  f"{vn[1]} replicas (4 T) x {vn[2]} MC steps on {threads} threads, "
+                  f"{vn[3]:.1f} s",
+        "single_thread_value": v1[0],
+        "single_thread_sample": f"{v1[1]} replicas x {v1[2]} steps, {v1[3]:.1f} s",
+        "pair_interactions_per_s": vn[0] * pairs,
+        "single_thread_pair_interactions_per_s": v1[0] * pairs,
+        "algorithmic_tflops": vn[0] * f_pose / 1e12,
+        "host_peak_tflops": peak / 1e12 if peak else None,
+        "host_peak_frac": vn[0] * f_pose / peak if peak else None,
+        "build": build if lib else build,
+        "host": info,
+    }
 
 
 def copy_calibration(mib):
@@ -90,69 +180,77 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n}))
 
 
-def bench_batch(args):
-    """Batched uneven workload: 256 complexes (ligand atoms U[12, 96], pocket
-    points U[200, 1500], ligand conformers U[4, 64], 8|16 temperatures, 2,000
-    steps) resident together, launch shape autotuned, cycles replayed from
-    one captured hipGraph.  Timed region: the cycles, inputs resident in
-    HBM.  Beside it, the plain path (gd_run, one complex per call) on every
-    16th complex."""
+def batch_leg(n, steps, seed, tune=True, plain_every=16):
+    """Batched uneven workload: n complexes (ligand atoms U[12, 96], pocket
+    points U[200, 1500], ligand conformers U[4, 64], 8|16 temperatures,
+    `steps` MC steps each) resident together, launch shape autotuned, cycles
+    replayed from one captured hipGraph.  Timed: the cycles, inputs resident
+    in HBM.  Beside it, the plain path (gd_run, one complex per call, wall
+    time from host buffers) on every plain_every-th complex."""
     from gpudock_amd import remc, synth
 
     t0 = time.perf_counter()
-    jobs = synth.make_batch_jobs(n=args.batch_n, seed=args.seed, steps=args.batch_steps)
+    jobs = synth.make_batch_jobs(n=n, seed=seed, steps=steps)
     gen_s = time.perf_counter() - t0
     poses = sum(j.n_rep * j.mc.steps_total for j in jobs)
     flops = sum(flops_per_pose(j.cx)[0] * j.n_rep * j.mc.steps_total for j in jobs)
     pairs = sum(j.n_rep * j.mc.steps_total * j.size.lna * j.size.pnp for j in jobs)
     print(f"batch: {len(jobs)} complexes, {sum(j.n_rep for j in jobs)} replicas, "
           f"{poses:.3e} pose evals (generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
-    res, rep = remc.batch_run(jobs, autotune=0 if args.no_tune else 1)
+    res, rep = remc.batch_run(jobs, autotune=1 if tune else 0)
     value = poses / rep["run_s"]
     achieved = flops / rep["run_s"] / 1e12
-    print(f"batch: run {rep['run_s']:.2f} s", file=sys.stderr, flush=True)
+    print(f"batch: run {rep['run_s']:.2f} s, {value:.3e} pose evals/s", file=sys.stderr,
+          flush=True)
     plain_poses, plain_s = 0, 0.0
-    for j in jobs[::16]:
+    for j in jobs[::plain_every]:
         t1 = time.perf_counter()
         remc.run(j.cx, j.temps, j.mc, j.opt)
         plain_s += time.perf_counter() - t1
         plain_poses += j.n_rep * j.mc.steps_total
     plain = plain_poses / plain_s
-    acc = float(sum(r["acs_mc"].sum() for r in res)) / poses
+    print(f"batch: plain path {plain:.3e} pose evals/s", file=sys.stderr, flush=True)
+    return {
+        "value": value, "unit": "pose evals/s",
+        "workload": f"batched uneven: {len(jobs)} complexes, ligand atoms U[12,96], pocket "
+                    f"points U[200,1500], ligand conformers U[4,64], 8|16 temperatures, "
+                    f"1 protein conformer, {steps} steps, exchange every 10",
+        "complexes": len(jobs), "replicas": sum(j.n_rep for j in jobs), "mc_steps": steps,
+        "cycles": rep["cycles"], "run_s": rep["run_s"],
+        "pair_interactions_per_s": pairs / rep["run_s"],
+        "end_to_end_s": rep["admit_s"] + rep["tune_s"] + rep["run_s"] + rep["results_s"],
+        "phases_s": {k: rep[k] for k in ("admit_s", "tune_s", "run_s", "results_s")},
+        "plain_path": {"value": plain, "unit": "pose evals/s",
+                       "sample": f"gd_run on {len(jobs[::plain_every])} of the complexes "
+                                 f"(every {plain_every}th), one call each, wall time"},
+        "speedup_vs_plain": value / plain,
+        "mc_acceptance": float(sum(r["acs_mc"].sum() for r in res)) / poses,
+        "autotune": rep["tune"], "layout": rep["layout"],
+        "achieved_tflops": achieved, "frac_of_fp32_peak": achieved / PEAK_FP32_TFLOPS,
+    }
+
+
+def bench_batch(args):
+    leg = batch_leg(args.batch_n, args.batch_steps, args.seed, tune=not args.no_tune)
     out = {
         "metric": "pose energy evals/s (MC replica-steps/s)",
-        "value": value,
+        "value": leg["value"],
         "unit": "pose evals/s",
         "n_gpus": 1,
-        "steps": rep["cycles"],
+        "steps": leg["cycles"],
         "warmup": 0,
-        "ms_per_step": rep["run_s"] * 1e3 / max(1, rep["cycles"]),
+        "ms_per_step": leg["run_s"] * 1e3 / max(1, leg["cycles"]),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, reference parameter files)",
-        "config": {
-            "workload": f"batched uneven: {len(jobs)} complexes, ligand atoms U[12,96], "
-                        f"pocket points U[200,1500], ligand conformers U[4,64], 8|16 "
-                        f"temperatures, 1 protein conformer, {args.batch_steps} steps, "
-                        "exchange every 10",
-            "replicas": sum(j.n_rep for j in jobs), "complexes": len(jobs),
-            "parallelism": "single GPU, one engine",
-        },
-        "pair_interactions_per_s": pairs / rep["run_s"],
-        "end_to_end_s": rep["admit_s"] + rep["tune_s"] + rep["run_s"] + rep["results_s"],
-        "phases_s": {k: rep[k] for k in ("admit_s", "tune_s", "run_s", "results_s")},
-        "plain_path": {"value": plain, "unit": "pose evals/s",
-                       "sample": f"gd_run on {len(jobs[::16])} of the complexes (every 16th), "
-                                 "one call each, wall time"},
-        "speedup_vs_plain": value / plain,
-        "mc_acceptance": acc,
-        "autotune": rep["tune"],
-        "layout": rep["layout"],
+        "config": {"workload": leg["workload"], "replicas": leg["replicas"],
+                   "complexes": leg["complexes"], "parallelism": "single GPU, one engine"},
+        "batched": leg,
         "roofline": {
-            "bound": "valu", "kernel": "gd_mc_kernel", "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "bound": "valu", "kernel": "gd_mc_kernel", "achieved": leg["achieved_tflops"],
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": leg["frac_of_fp32_peak"],
             "traffic": None,
             "note": "achieved = algorithmic FLOPs of all complexes / wall time of the "
                     "graph-replayed cycles (MC + exchange + advance launches)",
@@ -162,36 +260,35 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
-def bench_serve(args):
-    """Serving loop: Poisson arrivals at --rate jobs/s for --duration s, each
+def serve_leg(rate, duration, steps, seed, wpg=8, persistent=1, occ=6, kde_policy=1,
+              arena_gib=16, tag="serve"):
+    """Serving loop: Poisson arrivals at `rate` jobs/s for `duration` s, each
     job a complex drawn from a pool of 256 (ligand atoms lognormal median 32
     in [10, 120], pocket points lognormal median 600 in [150, 2000]) with 16
-    ligand conformers x 8 temperatures x 1,000 steps and its own Philox key.
-    Then a burst of --burst jobs submitted at once measures the sustainable
-    completion rate."""
+    ligand conformers x 8 temperatures x `steps` steps and its own Philox
+    key; the server drains after the last arrival."""
     import numpy as np
     from gpudock_amd import remc, synth
 
-    pool = synth.make_serving_pool(n=256, seed=args.seed + 1, steps=args.serve_steps)
-    rng = np.random.default_rng(args.seed)
-    gaps = rng.exponential(1.0 / args.rate, size=int(args.rate * args.duration * 1.5) + 16)
+    pool = serve_leg.pools.get((seed, steps))
+    if pool is None:
+        pool = synth.make_serving_pool(n=256, seed=seed + 1, steps=steps)
+        serve_leg.pools[(seed, steps)] = pool
+    rng = np.random.default_rng(seed + int(rate))
+    gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 1.5) + 16)
     arrivals = np.cumsum(gaps)
-    arrivals = arrivals[arrivals < args.duration]
-    picks = rng.integers(0, len(pool), size=len(arrivals) + args.burst)
-    def job(i):
-        cx, temps, mc = pool[picks[i]]
-        return remc.JobSpec(cx, temps, mc, remc.default_options(seed=args.seed, complex_id=i))
-    specs = [job(i) for i in range(len(arrivals))]
-    fp = max(s.footprint() for s in specs[:64])
-    srv = remc.Server(arena_bytes=args.arena_gib << 30, wpg=args.wpg,
-                      persistent=args.persistent, keep_results=0, occ=args.occ,
-                      kde_policy=args.kde_policy)
-    print(f"serve: {len(specs)} jobs at {args.rate}/s over {args.duration} s", file=sys.stderr,
+    arrivals = arrivals[arrivals < duration]
+    picks = rng.integers(0, len(pool), size=len(arrivals))
+    specs = [remc.JobSpec(*pool[picks[i]], remc.default_options(seed=seed, complex_id=i))
+             for i in range(len(arrivals))]
+    srv = remc.Server(arena_bytes=arena_gib << 30, wpg=wpg, persistent=persistent,
+                      keep_results=0, occ=occ, kde_policy=kde_policy)
+    print(f"{tag}: {len(specs)} jobs at {rate:.1f}/s over {duration} s", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
     behind = 0.0
     next_report = 10.0
-    for i, (t, sp) in enumerate(zip(arrivals, specs)):
+    for t, sp in zip(arrivals, specs):
         now = time.perf_counter() - t0
         if t > now:
             time.sleep(t - now)
@@ -200,38 +297,83 @@ def bench_serve(args):
         srv.submit(sp)
         if t >= next_report:
             st = srv.stats()
-            print(f"serve: t={t:.0f}s submitted {st['submitted']} completed {st['completed']} "
-                  f"running {st['running']} queued {st['queued']}", file=sys.stderr, flush=True)
+            print(f"{tag}: t={t:.0f}s submitted {st['submitted']} completed "
+                  f"{st['completed']} running {st['running']} queued {st['queued']}",
+                  file=sys.stderr, flush=True)
             next_report += 10.0
     srv.drain()
     st = srv.stats()
     wall = time.perf_counter() - t0
     srv.close()
-    # burst: sustainable completion rate
-    burst = None
-    if args.burst:
-        bs = [job(len(arrivals) + k) for k in range(args.burst)]
-        srv = remc.Server(arena_bytes=args.arena_gib << 30, wpg=args.wpg,
-                          persistent=args.persistent, keep_results=0, occ=args.occ,
-                          kde_policy=args.kde_policy)
-        tb = time.perf_counter()
-        for sp in bs:
-            srv.submit(sp)
-        srv.drain()
-        sb = srv.stats()
-        srv.close()
-        span = sb["last_done"] - sb["first_submit"]
-        burst = {"jobs": args.burst, "jobs_per_s": args.burst / span,
-                 "pose_evals_per_s": sb["pose_evals_per_s"], "wall_s": time.perf_counter() - tb}
-        print(f"serve: burst {burst}", file=sys.stderr, flush=True)
     pairs_per_job = float(np.mean([pool[k][0].lig_xyz.shape[1] * pool[k][0].prt_xyz.shape[1]
-                                   for k in picks[:len(arrivals)]])) * 16 * 8 * args.serve_steps
+                                   for k in picks])) * 16 * 8 * steps
+    print(f"{tag}: p50 {st['p50_latency']:.3f} s, p99 {st['p99_latency']:.3f} s, "
+          f"{st['pose_evals_per_s']:.3e} pose evals/s", file=sys.stderr, flush=True)
+    return {
+        "value": st["pose_evals_per_s"], "unit": "pose evals/s",
+        "workload": f"serving: Poisson arrivals {rate:.1f} jobs/s for {duration} s; ligand "
+                    "atoms lognormal(median 32, sigma 0.5) in [10,120], pocket points "
+                    "lognormal(median 600, sigma 0.5) in [150,2000]; 16 ligand conformers x "
+                    f"8 temperatures x {steps} steps per job",
+        "rate_jobs_per_s": rate, "duration_s": duration, "jobs": len(specs),
+        "latency_s": {"p50": st["p50_latency"], "p99": st["p99_latency"],
+                      "mean": st["mean_latency"], "max": st["max_latency"],
+                      "queue_wait_p50": st["p50_wait"], "queue_wait_p99": st["p99_wait"]},
+        "completed": st["completed"], "failed": st["failed"],
+        "offered_pairs_per_s": pairs_per_job * rate,
+        "running_peak": st["running_peak"], "arena_peak_bytes": st["arena_peak"],
+        "cycles": st["cycles"], "rebuilds": st["rebuilds"],
+        "submit_lag_max_s": behind, "wall_s": wall,
+        "server": {"wpg": wpg, "persistent": persistent, "occ": occ, "kde_policy": kde_policy},
+    }
+
+
+serve_leg.pools = {}
+
+
+def burst_rate(n, steps, seed, wpg=8, persistent=1, occ=6, kde_policy=1, arena_gib=16):
+    """n jobs submitted at once: the completion rate the server sustains."""
+    from gpudock_amd import remc, synth
+    import numpy as np
+
+    pool = serve_leg.pools.get((seed, steps)) or synth.make_serving_pool(n=256, seed=seed + 1,
+                                                                       steps=steps)
+    picks = np.random.default_rng(seed + 7).integers(0, len(pool), size=n)
+    specs = [remc.JobSpec(*pool[picks[i]], remc.default_options(seed=seed, complex_id=i))
+             for i in range(n)]
+    srv = remc.Server(arena_bytes=arena_gib << 30, wpg=wpg, persistent=persistent,
+                      keep_results=0, occ=occ, kde_policy=kde_policy)
+    tb = time.perf_counter()
+    for sp in specs:
+        srv.submit(sp)
+    srv.drain()
+    sb = srv.stats()
+    srv.close()
+    span = sb["last_done"] - sb["first_submit"]
+    out = {"jobs": n, "jobs_per_s": n / span, "pose_evals_per_s": sb["pose_evals_per_s"],
+           "wall_s": time.perf_counter() - tb}
+    print(f"serve: burst {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def bench_serve(args):
+    """The serving workload at --rate for --duration s; then a burst of
+    --burst jobs gives the sustainable completion rate, and a Poisson stream
+    at 95 % of it for --max-rate-duration s gives p50/p99 at that rate."""
+    kw = dict(wpg=args.wpg, persistent=args.persistent, occ=args.occ,
+              kde_policy=args.kde_policy, arena_gib=args.arena_gib)
+    leg = serve_leg(args.rate, args.duration, args.serve_steps, args.seed, **kw)
+    burst = burst_rate(args.burst, args.serve_steps, args.seed, **kw) if args.burst else None
+    at_max = None
+    if burst and args.max_rate_duration > 0:
+        at_max = serve_leg(0.95 * burst["jobs_per_s"], args.max_rate_duration,
+                           args.serve_steps, args.seed, tag="serve@max", **kw)
     out = {
         "metric": "pose energy evals/s (MC replica-steps/s); p50/p99 docking-job latency",
-        "value": st["pose_evals_per_s"],
+        "value": leg["value"],
         "unit": "pose evals/s",
         "n_gpus": 1,
-        "steps": int(st["completed"]),
+        "steps": int(leg["completed"]),
         "warmup": 0,
         "ms_per_step": None,
         "higher_is_better": True,
@@ -239,25 +381,13 @@ def bench_serve(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, reference parameter files)",
-        "config": {
-            "workload": f"serving: Poisson arrivals {args.rate} jobs/s for {args.duration} s; "
-                        "ligand atoms lognormal(median 32, sigma 0.5) in [10,120], pocket "
-                        "points lognormal(median 600, sigma 0.5) in [150,2000]; 16 ligand "
-                        f"conformers x 8 temperatures x {args.serve_steps} steps per job",
-            "jobs": len(specs), "rate_jobs_per_s": args.rate, "duration_s": args.duration,
-            "parallelism": "single GPU, continuous batching",
-        },
-        "latency_s": {"p50": st["p50_latency"], "p99": st["p99_latency"],
-                      "mean": st["mean_latency"], "max": st["max_latency"],
-                      "queue_wait_p50": st["p50_wait"], "queue_wait_p99": st["p99_wait"]},
-        "completed": st["completed"], "failed": st["failed"],
-        "offered_pairs_per_s": pairs_per_job * args.rate,
-        "running_peak": st["running_peak"], "arena_peak_bytes": st["arena_peak"],
-        "cycles": st["cycles"], "rebuilds": st["rebuilds"],
-        "submit_lag_max_s": behind, "wall_s": wall, "job_footprint_max_bytes": fp,
+        "config": {"workload": leg["workload"], "jobs": leg["jobs"],
+                   "rate_jobs_per_s": args.rate, "duration_s": args.duration,
+                   "parallelism": "single GPU, continuous batching"},
+        "serving": leg,
+        "latency_s": leg["latency_s"],
         "burst": burst,
-        "server": {"wpg": args.wpg, "persistent": args.persistent, "occ": args.occ,
-                   "kde_policy": args.kde_policy},
+        "at_max_sustainable_rate": at_max,
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -286,6 +416,15 @@ def main():
     ap.add_argument("--duration", type=float, default=120.0)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--burst", type=int, default=1000)
+    ap.add_argument("--max-rate-duration", type=float, default=60.0,
+                    help="serve: seconds of arrivals at 95 %% of the burst completion rate")
+    ap.add_argument("--legs", default="batch,serve",
+                    help="single: extra legs in the same line (batch: the 256-complex "
+                         "batch at --leg-batch-steps with its speed-up over the plain "
+                         "path; serve: --leg-serve-duration s of arrivals at --rate); "
+                         "'' for none")
+    ap.add_argument("--leg-batch-steps", type=int, default=500)
+    ap.add_argument("--leg-serve-duration", type=float, default=30.0)
     ap.add_argument("--wpg", type=int, default=8)
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--arena-gib", type=int, default=16)
@@ -345,7 +484,7 @@ def main():
     traffic = None
     valu_per_launch = None
     tfile = os.path.join(ROOT, "profiles", "gd_mc_kernel_16384rep_traffic.json")
-    if args.config == "c1" and os.path.exists(tfile):
+    if args.config == "c1" and os.path.exists(tfile) and not args.mcc:
         # HBM bytes per cycle launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
         # passes (calibrated against a device copy), and the VALU instructions
         # per launch (SQ_INSTS_VALU), kept with the profile
@@ -428,18 +567,19 @@ def main():
             "note": "t0 = upload + cycles (PCIe-inclusive inputs); t1 = t0 + FP64 re-score "
                     "and read-back of results; not the headline value",
         }
+    legs = [x for x in args.legs.split(",") if x]
+    if "batch" in legs:
+        out["batched"] = batch_leg(args.batch_n, args.leg_batch_steps, args.seed,
+                                   tune=not args.no_tune)
+    if "serve" in legs:
+        out["serving"] = serve_leg(args.rate, args.leg_serve_duration, args.serve_steps,
+                                   args.seed, wpg=args.wpg, persistent=args.persistent,
+                                   occ=args.occ, kde_policy=args.kde_policy,
+                                   arena_gib=args.arena_gib)
+        out["latency_s"] = out["serving"]["latency_s"]
     if not args.no_cpu:
-        scale = list(mc.move_scale)
-        base, cores = cpu_baseline(cx, temps, scale, cfg, args.seed)
-        v1 = base[1]
-        vn = base[cores]
-        out["cpu_baseline"] = {
-            "value": vn[0], "unit": "pose evals/s", "cores": cores, "kind": "port",
-            "sample": f"reference-order FP32 oracle, same complex (protein conformer 0), {vn[1]} replicas "
-                      f"(4 T) x {vn[2]} MC steps on {cores} threads, {vn[3]:.1f} s",
-            "single_thread_value": v1[0],
-            "single_thread_sample": f"{v1[1]} replicas x {v1[2]} steps, {v1[3]:.1f} s",
-        }
+        out["cpu_baseline"] = cpu_baseline(cx, temps, list(mc.move_scale), cfg, args.seed,
+                                           f_pose)
     print(json.dumps(out))
 
 

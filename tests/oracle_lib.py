@@ -98,12 +98,19 @@ def replay_energy(cx, lig_conf, prt_conf, moves, term_order=0):
     return out
 
 
+def steps_run(mc):
+    """MC steps the schedule of McPara mc runs (whole windows of whole
+    launches, gpusingle.cu:8-20)."""
+    return len(schedule(mc.steps_total, mc.steps_per_dump, mc.steps_per_exchange)) * \
+        mc.steps_per_exchange
+
+
 def replay_run_records(cx, temps, mc, opt, stride):
-    """Replay plus the pre-move state of every stride-th step:
-    (results, records [n_rep, ceil(steps / stride)])."""
+    """Replay plus the pre-move state of every stride-th step of the run:
+    (results, records [n_rep, ceil(steps run / stride)])."""
     size = cx.sized(len(temps))
     res = (remc.ReplicaResult * size.n_rep)()
-    n_rec = (mc.steps_total + stride - 1) // stride
+    n_rec = (steps_run(mc) + stride - 1) // stride
     rec = np.zeros((size.n_rep, n_rec), remc.STEP_RECORD_DTYPE)
     lib().orc_replay_run_rec(*cx.args(), temps, None, C.byref(mc), size, C.byref(opt), res,
                              stride, rec.ctypes.data_as(C.c_void_p), n_rec)

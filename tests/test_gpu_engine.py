@@ -179,8 +179,7 @@ def test_server_results_equal_alone_under_continuous_batching(jobs):
     assert st["failed"] == 0 and st["p99_latency"] >= st["p50_latency"] > 0.0
     # pose evaluations = the MC steps each job's schedule runs (whole launches,
     # gpusingle.cu:8-20) x its replicas
-    runs = [len(O.schedule(j.mc.steps_total, j.mc.steps_per_dump, j.mc.steps_per_exchange)) *
-            j.mc.steps_per_exchange for j in js + extra]
+    runs = [O.steps_run(j.mc) for j in js + extra]
     assert st["pose_evals"] == sum(j.n_rep * r for j, r in zip(js + extra, runs))
 
 

@@ -41,8 +41,7 @@ def _check_properties(spec, g):
     size = spec.size
     for k in ("energy_final", "energy_best", "move_final", "move_best", "energy_best_fp64"):
         assert np.all(np.isfinite(g[k])), k
-    run = len(O.schedule(spec.mc.steps_total, spec.mc.steps_per_dump,
-                         spec.mc.steps_per_exchange)) * spec.mc.steps_per_exchange
+    run = O.steps_run(spec.mc)
     acc = g["acs_mc"].sum() / (size.n_rep * run)
     assert 0.0 < acc < 1.0
     assert np.all(g["energy_best"][:, 9] <= g["energy_final"][:, 9])
