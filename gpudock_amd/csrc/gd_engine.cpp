@@ -1384,12 +1384,45 @@ extern "C" int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_confi
 /* ------------------------------------------------------------------------ */
 /* gd_score_poses                                                            */
 /* ------------------------------------------------------------------------ */
-extern "C" int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
-                              const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
-                              const gd_enepara *enepara, gd_complex_size size,
-                              int n_pose, const int *lig_conf, const int *prt_conf,
-                              const float *move, int term_order, int device,
-                              float *energy, double *energy_fp64)
+namespace {
+
+/* Device layout of one pose-scoring call inside one region: the complex
+ * image, the task list, the pose order and arrays, the FP32 and FP64
+ * energies, each at a 256-byte aligned offset. */
+struct ScoreLayout {
+  size_t img, tasks, order, lc, pc, mv, e, e64, total;
+};
+
+ScoreLayout score_layout(size_t img_bytes, size_t n_tasks, int n_pose)
+{
+  ScoreLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += GdArena::round(bytes);
+    return o;
+  };
+  L.img = take(img_bytes);
+  L.tasks = take(n_tasks * sizeof(gd_task));
+  L.order = take((size_t) n_pose * 4);
+  L.lc = take((size_t) n_pose * 4);
+  L.pc = take((size_t) n_pose * 4);
+  L.mv = take((size_t) n_pose * 24);
+  L.e = take((size_t) n_pose * 40);
+  L.e64 = take((size_t) n_pose * 80);
+  L.total = off;
+  return L;
+}
+
+/* Argument checks, the host image of the complex (the pose arrays ride in it
+ * as a one-temperature replica set) and the workgroup tasks: runs of up to 8
+ * poses on one protein conformer, poses ordered by conformer. */
+int score_prepare(const gd_ligand *lig, const gd_protein *prt, const gd_psp *psp,
+                  const gd_kde *kde, const gd_mcs *mcs, const gd_enepara *enepara,
+                  gd_complex_size size, int n_pose, const int *lig_conf, const int *prt_conf,
+                  const float *move, int term_order, int device, float *energy,
+                  double *energy_fp64, GdImage &I, std::vector<int> &order,
+                  std::vector<gd_task> &tasks)
 {
   int rc = gd_check_inputs(lig, prt, psp, kde, mcs, enepara, size);
   if (rc)
@@ -1403,33 +1436,28 @@ extern "C" int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
         prt_conf[i] >= size.n_prt)
       return gd_fail(GD_ERR_ARG, "pose %d: conformer index out of range", i);
   const int ndev = gd_device_count();
-  if (ndev < 1)
-    return gd_fail(GD_ERR_NODEV, "no HIP device visible");
-  if (device < 0 || device >= ndev)
+  if (ndev > 0 && (device < 0 || device >= ndev))
     return gd_fail(GD_ERR_ARG, "device %d of %d", device, ndev);
-  if (n_pose == 0)
-    return GD_OK;
-  GD_HIP_TRY(hipSetDevice(device));
-  int max_lds = 0;
-  GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  int max_lds = 65536;
+  if (ndev > 0)
+    GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                     device));
   gd_options o = default_opts(nullptr);
   o.term_order = term_order;
   o.device = device;
-  /* the pose arrays ride in the image as a 1-temperature replica set */
   gd_complex_size sz1 = size;
   sz1.n_tmp = 1;
   sz1.n_rep = size.n_lig * size.n_prt;
-  GdImage I;
   rc = gd_build_image(I, lig, prt, psp, kde, mcs, enepara, nullptr, nullptr, nullptr, sz1, o,
                       max_lds, 0);
   if (rc)
     return rc;
-  std::vector<int> order(n_pose);
+  order.resize(n_pose);
   for (int i = 0; i < n_pose; ++i)
     order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return prt_conf[a] < prt_conf[b]; });
-  std::vector<gd_task> tasks;
+  tasks.clear();
   for (int i = 0; i < n_pose;) {
     int j = i;
     while (j < n_pose && prt_conf[order[j]] == prt_conf[order[i]] && j - i < 8)
@@ -1437,66 +1465,120 @@ extern "C" int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
     tasks.push_back(gd_task{0, prt_conf[order[i]], i, j - i});
     i = j;
   }
-  GdArena arena;
+  return GD_OK;
+}
+
+/* Uploads, launches and read-back, all ordered on st, into the device
+ * region at B laid out as L; returns after st has drained. */
+int score_run(char *B, const ScoreLayout &L, GdImage &I, const std::vector<gd_task> &tasks,
+              const std::vector<int> &order, int n_pose, const int *lig_conf,
+              const int *prt_conf, const float *move, float *energy, double *energy_fp64,
+              hipStream_t st)
+{
   const size_t img = gd_image_footprint(I);
-  const size_t extra = GdArena::round(vbytes(tasks)) + GdArena::round(n_pose * 4) * 3 +
-                       GdArena::round((size_t) n_pose * 24) +
-                       GdArena::round((size_t) n_pose * 40) +
-                       GdArena::round((size_t) n_pose * 80) + 4096;
-  GD_HIP_TRY(arena.init(img + extra));
+  std::vector<char> host(img);
+  GdBlock b;
+  gd_pack_image(I, host.data(), B + L.img, b);
+  const gd_pack *pack_dev = (const gd_pack *) (B + L.img + b.pack);
+  /* the host buffers outlive the final synchronisation */
+  hipError_t he = hipMemcpyAsync(B + L.img, host.data(), img, hipMemcpyHostToDevice, st);
+  auto put = [&](const void *src, size_t bytes, size_t off) {
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(B + off, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  put(tasks.data(), vbytes(tasks), L.tasks);
+  put(order.data(), (size_t) n_pose * 4, L.order);
+  put(lig_conf, (size_t) n_pose * 4, L.lc);
+  put(prt_conf, (size_t) n_pose * 4, L.pc);
+  put(move, (size_t) n_pose * 24, L.mv);
+  if (he == hipSuccess && energy) {
+    he = gd_set_kernel_attrs(I.P.lds_bytes);
+    if (he == hipSuccess)
+      he = gd_launch_score(pack_dev, (const gd_task *) (B + L.tasks), (int) tasks.size(),
+                           (const int *) (B + L.order), (const int *) (B + L.lc),
+                           (const float *) (B + L.mv), I.P.lds_bytes, (float *) (B + L.e), st);
+  }
+  if (he == hipSuccess && energy_fp64)
+    he = gd_launch_rescore64(pack_dev, 0, (const int *) (B + L.lc), (const int *) (B + L.pc),
+                             (const float *) (B + L.mv), 6, n_pose, (double *) (B + L.e64), st);
+  if (he == hipSuccess && energy)
+    he = hipMemcpyAsync(energy, B + L.e, (size_t) n_pose * 40, hipMemcpyDeviceToHost, st);
+  if (he == hipSuccess && energy_fp64)
+    he = hipMemcpyAsync(energy_fp64, B + L.e64, (size_t) n_pose * 80, hipMemcpyDeviceToHost,
+                        st);
+  const hipError_t hs = hipStreamSynchronize(st);
+  if (he == hipSuccess)
+    he = hs;
+  if (he != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "score: %s", hipGetErrorString(he));
+  return GD_OK;
+}
+
+}  // namespace
+
+extern "C" int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
+                              const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                              const gd_enepara *enepara, gd_complex_size size,
+                              int n_pose, const int *lig_conf, const int *prt_conf,
+                              const float *move, int term_order, int device,
+                              float *energy, double *energy_fp64)
+{
+  GdImage I;
+  std::vector<int> order;
+  std::vector<gd_task> tasks;
+  int rc = score_prepare(lig, prt, psp, kde, mcs, enepara, size, n_pose, lig_conf, prt_conf,
+                         move, term_order, device, energy, energy_fp64, I, order, tasks);
+  if (rc)
+    return rc;
+  if (gd_device_count() < 1)
+    return gd_fail(GD_ERR_NODEV, "no HIP device visible");
+  if (n_pose == 0)
+    return GD_OK;
+  GD_HIP_TRY(hipSetDevice(device));
+  const ScoreLayout L = score_layout(gd_image_footprint(I), tasks.size(), n_pose);
+  GdArena arena;
+  GD_HIP_TRY(arena.init(L.total));
   struct StreamGuard {
     hipStream_t s = nullptr;
     ~StreamGuard() { if (s) (void) hipStreamDestroy(s); }
   } sg;
   GD_HIP_TRY(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
-  hipStream_t st = sg.s;
-  char *B = arena.base();
-  /* every copy and launch is ordered on st; the host buffers outlive the
-   * final synchronisation */
-  const size_t o_img = arena.alloc(img);
-  std::vector<char> host(img);
-  GdBlock b;
-  gd_pack_image(I, host.data(), B + o_img, b);
-  const gd_pack *pack_dev = (const gd_pack *) (B + o_img + b.pack);
-  hipError_t he = hipMemcpyAsync(B + o_img, host.data(), img, hipMemcpyHostToDevice, st);
-  auto put = [&](const void *src, size_t bytes, size_t *off) {
-    *off = arena.alloc(bytes);
-    if (he == hipSuccess)
-      he = hipMemcpyAsync(B + *off, src, bytes, hipMemcpyHostToDevice, st);
-  };
-  size_t o_tasks, o_order, o_lc, o_pc, o_mv;
-  put(tasks.data(), vbytes(tasks), &o_tasks);
-  put(order.data(), (size_t) n_pose * 4, &o_order);
-  put(lig_conf, (size_t) n_pose * 4, &o_lc);
-  put(prt_conf, (size_t) n_pose * 4, &o_pc);
-  put(move, (size_t) n_pose * 24, &o_mv);
-  const size_t o_e = arena.alloc((size_t) n_pose * 40);
-  const size_t o_e64 = arena.alloc((size_t) n_pose * 80);
-  if (o_e64 == (size_t) -1)
-    return gd_fail(GD_ERR_NOMEM, "score: workspace sizing");
-  if (he == hipSuccess && energy) {
-    he = gd_set_kernel_attrs(I.P.lds_bytes);
-    if (he == hipSuccess)
-      he = gd_launch_score(pack_dev, (const gd_task *) (B + o_tasks), (int) tasks.size(),
-                           (const int *) (B + o_order), (const int *) (B + o_lc),
-                           (const float *) (B + o_mv), I.P.lds_bytes, (float *) (B + o_e), st);
-  }
-  if (he == hipSuccess && energy_fp64)
-    he = gd_launch_rescore64(pack_dev, 0, (const int *) (B + o_lc), (const int *) (B + o_pc),
-                             (const float *) (B + o_mv), 6, n_pose, (double *) (B + o_e64), st);
-  if (he == hipSuccess && energy)
-    he = hipMemcpyAsync(energy, B + o_e, (size_t) n_pose * 40, hipMemcpyDeviceToHost, st);
-  if (he == hipSuccess && energy_fp64)
-    he = hipMemcpyAsync(energy_fp64, B + o_e64, (size_t) n_pose * 80, hipMemcpyDeviceToHost,
-                        st);
-  {
-    const hipError_t hs = hipStreamSynchronize(st);
-    if (he == hipSuccess)
-      he = hs;
-  }
-  if (he != hipSuccess)
-    return gd_fail(GD_ERR_HIP, "score: %s", hipGetErrorString(he));
-  return GD_OK;
+  return score_run(arena.base(), L, I, tasks, order, n_pose, lig_conf, prt_conf, move, energy,
+                   energy_fp64, sg.s);
+}
+
+/* The same scoring with no device allocation: the region is the static
+ * per-GPU workspace (GD_SCRATCH_BYTES); work on the default stream. */
+extern "C" int gd_score_poses_noalloc(const gd_ligand *lig, const gd_protein *prt,
+                                      const gd_psp *psp, const gd_kde *kde,
+                                      const gd_mcs *mcs, const gd_enepara *enepara,
+                                      gd_complex_size size, int n_pose, const int *lig_conf,
+                                      const int *prt_conf, const float *move, int term_order,
+                                      int device, float *energy, double *energy_fp64)
+{
+  GdImage I;
+  std::vector<int> order;
+  std::vector<gd_task> tasks;
+  int rc = score_prepare(lig, prt, psp, kde, mcs, enepara, size, n_pose, lig_conf, prt_conf,
+                         move, term_order, device, energy, energy_fp64, I, order, tasks);
+  if (rc)
+    return rc;
+  const ScoreLayout L = score_layout(gd_image_footprint(I), tasks.size(), n_pose);
+  if (L.total > GD_SCRATCH_BYTES)
+    return gd_fail(GD_ERR_ARG,
+                   "gd_score_poses_noalloc: %d poses of this complex need %zu bytes of "
+                   "workspace; the static workspace holds %zu (use gd_score_poses, or fewer "
+                   "poses per call)",
+                   n_pose, L.total, (size_t) GD_SCRATCH_BYTES);
+  if (gd_device_count() < 1)
+    return gd_fail(GD_ERR_NODEV, "no HIP device visible");
+  if (n_pose == 0)
+    return GD_OK;
+  GD_HIP_TRY(hipSetDevice(device));
+  char *base = nullptr;
+  GD_HIP_TRY(gd_scratch_region(device, &base));
+  return score_run(base, L, I, tasks, order, n_pose, lig_conf, prt_conf, move, energy,
+                   energy_fp64, nullptr);
 }
 
 /* ------------------------------------------------------------------------ */

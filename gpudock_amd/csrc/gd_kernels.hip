@@ -36,6 +36,7 @@
  * unchanged from a captured hipGraph.
  */
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include "gd_device.h"
 #include "gd_launch.h"
 
@@ -1106,6 +1107,42 @@ hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_
   case 16: return occ_wpg<16>(C.occ, stage_bytes, blocks_per_cu);
   default: return occ_wpg<8>(C.occ, stage_bytes, blocks_per_cu);
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Static device workspace of the allocation-free pose scoring entry point:  */
+/* one array per GPU, for the whole process.  Host code never uses the      */
+/* array's name as an address: gd_scratch_region resolves each device's     */
+/* own copy once, with that device current, and hands out that address.     */
+/* ------------------------------------------------------------------------ */
+static __device__ __align__(256) char scratch[GD_SCRATCH_BYTES];
+
+#define GD_SCRATCH_MAX_DEVICES 64
+
+hipError_t gd_scratch_region(int device, char **ptr)
+{
+  static std::mutex mu;
+  static char *dev_ptr[GD_SCRATCH_MAX_DEVICES] = {};
+  if (device < 0 || device >= GD_SCRATCH_MAX_DEVICES)
+    return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!dev_ptr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+      return e;
+    if (cur != device)
+      return hipErrorInvalidDevice;   /* the caller makes `device` current */
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return e;
+    if (!p)
+      return hipErrorInvalidSymbol;
+    dev_ptr[device] = (char *) p;
+  }
+  *ptr = dev_ptr[device];
+  return hipSuccess;
 }
 
 /* ------------------------------------------------------------------------ */

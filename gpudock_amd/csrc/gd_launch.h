@@ -35,6 +35,12 @@ hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *li
 hipError_t gd_launch_rescore64_best(const gd_pack *packs, const int2 *items, int n,
                                     double *energy, hipStream_t st);
 hipError_t gd_set_kernel_attrs(int stage_bytes);
+
+/* Bytes of the static per-GPU workspace of gd_score_poses_noalloc. */
+#define GD_SCRATCH_BYTES ((size_t) 64 << 20)
+/* Device address of that workspace on `device`, which must be the current
+ * device; resolved once per device (hipGetSymbolAddress) and kept. */
+hipError_t gd_scratch_region(int device, char **ptr);
 hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_per_cu);
 
 #endif /* GD_LAUNCH_H */

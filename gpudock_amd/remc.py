@@ -227,6 +227,10 @@ SYMBOLS = {
                                  C.POINTER(Kde), C.POINTER(Mcs), C.POINTER(EnePara),
                                  ComplexSize, C.c_int, IP, IP, FP, C.c_int, C.c_int, FP,
                                  C.POINTER(C.c_double)]),
+    "gd_score_poses_noalloc": (C.c_int, [C.POINTER(Ligand), C.POINTER(Protein),
+                                         C.POINTER(Psp), C.POINTER(Kde), C.POINTER(Mcs),
+                                         C.POINTER(EnePara), ComplexSize, C.c_int, IP, IP, FP,
+                                         C.c_int, C.c_int, FP, C.POINTER(C.c_double)]),
     "gd_session_create": (C.c_int, [C.POINTER(Ligand), C.POINTER(Protein),
                                     C.POINTER(Psp), C.POINTER(Kde), C.POINTER(Mcs),
                                     C.POINTER(EnePara), C.POINTER(Temp),
@@ -576,17 +580,19 @@ def valu_probe(waves_per_simd=8, device=0):
     return ns.value, tf.value, n.value
 
 
-def score_poses(cx, lig_conf, prt_conf, moves, term_order=0, device=0, fp64=True):
-    """gd_score_poses: FP32 canonical energies (and FP64) of poses, slot order."""
+def score_poses(cx, lig_conf, prt_conf, moves, term_order=0, device=0, fp64=True,
+                noalloc=False):
+    """gd_score_poses: FP32 canonical energies (and FP64) of poses, slot order.
+    noalloc: through gd_score_poses_noalloc (static device workspace)."""
     lig_conf = _i32(lig_conf)
     prt_conf = _i32(prt_conf)
     moves = _f32(moves).reshape(-1, 6)
     n = len(lig_conf)
     e = np.zeros((n, 10), np.float32)
     e64 = np.zeros((n, 10), np.float64) if fp64 else None
-    check(lib().gd_score_poses(
-        *cx.args(), cx.size, n, _ip(lig_conf), _ip(prt_conf), _fp(moves), term_order,
-        device, _fp(e), e64.ctypes.data_as(C.POINTER(C.c_double)) if fp64 else None))
+    fn = lib().gd_score_poses_noalloc if noalloc else lib().gd_score_poses
+    check(fn(*cx.args(), cx.size, n, _ip(lig_conf), _ip(prt_conf), _fp(moves), term_order,
+             device, _fp(e), e64.ctypes.data_as(C.POINTER(C.c_double)) if fp64 else None))
     return (e, e64) if fp64 else e
 
 

@@ -276,6 +276,24 @@ int gd_score_poses(const gd_ligand *lig, const gd_protein *prt,
                    const float *move, int term_order, int device,
                    float *energy, double *energy_fp64);
 
+/* gd_score_poses without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  The complex's device image, the pose
+ * arrays and the energies are staged in a static workspace of 64 MiB that
+ * each GPU the library runs on holds until the process ends; a call whose
+ * complex and poses need more returns GD_ERR_ARG (split the poses, or use
+ * gd_score_poses).  The workspace keeps the last call's contents; every
+ * byte a call reads it has written first.  Calls on one GPU share that
+ * workspace: calls that overlap on one GPU (from several threads) race.
+ * Work is ordered on the default stream; the call returns when it is done.
+ * Results are those of gd_score_poses, bit for bit. */
+int gd_score_poses_noalloc(const gd_ligand *lig, const gd_protein *prt,
+                           const gd_psp *psp, const gd_kde *kde, const gd_mcs *mcs,
+                           const gd_enepara *enepara, gd_complex_size size,
+                           int n_pose, const int *lig_conf, const int *prt_conf,
+                           const float *move, int term_order, int device,
+                           float *energy, double *energy_fp64);
+
 /* ---------------------------------------------------------------------- */
 /* Session: the same run split into steps, with the complex and the replica */
 /* state resident on the GPU between calls (gd_run is create + init +       */
