@@ -424,6 +424,9 @@ def main():
                          "path; serve: --leg-serve-duration s of arrivals at --rate); "
                          "'' for none")
     ap.add_argument("--leg-batch-steps", type=int, default=500)
+    ap.add_argument("--split", type=int, default=0,
+                    help="single: wavefronts per replica (0 = chosen from the replica count "
+                         "and tuned, 1, 4, 8, 16)")
     ap.add_argument("--leg-serve-duration", type=float, default=30.0)
     ap.add_argument("--wpg", type=int, default=8)
     ap.add_argument("--persistent", type=int, default=1)
@@ -461,7 +464,7 @@ def main():
     f_pose, m_kde = flops_per_pose(cx)
     opt = remc.default_options(seed=args.seed, mcc=int(args.mcc))
     job = remc.JobSpec(cx, temps, mc, opt)
-    eng = remc.Engine(arena_bytes=job.footprint() + (64 << 20), use_graph=1)
+    eng = remc.Engine(arena_bytes=job.footprint() + (64 << 20), use_graph=1, split=args.split)
     h = eng.add(job)
     tune = eng.autotune() if not args.no_tune else None
     if args.warmup:
@@ -546,6 +549,7 @@ def main():
             "lds_bytes": layout["stage_bytes"],
             "kde_in_lds": layout["kde_in_lds"],
             "waves_per_group": layout["wpg"],
+            "waves_per_replica": layout["split"],
             "persistent": layout["persistent"],
             "grid": layout["grid"],
             "valu_issue": valu_issue,

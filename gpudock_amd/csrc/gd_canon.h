@@ -475,6 +475,28 @@ GD_HD int gd_point_sort_key(int type, int is_c2, int hpc, uint32_t hdb_mask)
          (is_c2 ? 2 : 0) + (hpc ? 1 : 0);
 }
 
+/* Summation blocks.  The ligand atoms are cut into at most GD_MAX_BLOCKS
+ * blocks of gd_block_size(lna) consecutive atoms, and the 64-pair rounds of
+ * the KDE pair list into at most GD_MAX_BLOCKS blocks of
+ * gd_block_size(rounds) consecutive rounds.  Each lane sums each term within
+ * a block from +0 in pair order; a lane's total is its block partials added
+ * in block order from +0; the per-atom hydrophobic-restraint terms sum the
+ * same way.  The blocks depend on the complex only, so a pose's energy has
+ * the same bits whether one wavefront evaluates every block in order or
+ * several wavefronts evaluate blocks in parallel and one adds the partials. */
+#define GD_MAX_BLOCKS 16
+
+GD_HD int gd_block_size(int n)
+{
+  return n > GD_MAX_BLOCKS ? (n + GD_MAX_BLOCKS - 1) / GD_MAX_BLOCKS : 1;
+}
+
+GD_HD int gd_block_count(int n)
+{
+  const int bs = gd_block_size(n);
+  return (n + bs - 1) / bs;
+}
+
 /* Raw (averaged, not normalised) terms in the parameter-file order used by
  * the weight and normalisation files: vdw ele pmf hpc hdb dst psp kde lhm. */
 enum {

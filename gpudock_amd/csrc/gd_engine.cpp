@@ -58,6 +58,7 @@ int gd_engine::init(const gd_engine_config *cfg)
   lc.wpg = (c.wpg == 4 || c.wpg == 16) ? c.wpg : 8;
   lc.persistent = c.persistent ? 1 : 0;
   lc.occ = (c.occ == 6 || c.occ == 8) ? c.occ : 0;
+  lc.split = (c.split == 1 || c.split == 4 || c.split == 8 || c.split == 16) ? c.split : 0;
   lc.stage_bytes = 16;
   GD_HIP_TRY(hipSetDevice(c.device));
   GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
@@ -96,6 +97,9 @@ gd_engine::~gd_engine()
 }
 
 int *gd_engine::queue_ptr() { return (int *) (arena.base() + ctl_off); }
+/* the exchange launch's finished-workgroup counter: the last int of the
+ * control block (the queue counters, one per class, come first) */
+int *gd_engine::done_ptr() { return (int *) (arena.base() + ctl_off) + 63; }
 
 void gd_engine::defer(size_t off, size_t bytes, hipGraph_t g, hipGraphExec_t x, bool host)
 {
@@ -202,12 +206,37 @@ int gd_engine::upload_image(GdImage &I, size_t off, GdBlock &b)
   return GD_OK;
 }
 
+/* Wavefronts per replica.  One, unless the resident replicas are too few to
+ * give every SIMD of the device two wavefronts: then 16, 8 or 4 per replica
+ * (the most that keeps the total within that), one replica per workgroup.
+ * Complexes with more position restraints than the split area holds keep
+ * one wavefront per replica. */
+int gd_engine::choose_split() const
+{
+  long long reps = 0;
+  for (auto *s : slots) {
+    if (s->img->P.pos > GD_SPLIT_MAX_POS)
+      return 1;
+    reps += s->img->P.n_rep;
+  }
+  if (lc.split == 1 || lc.split == 4 || lc.split == 8 || lc.split == 16)
+    return lc.split;
+  const long long waves = 2LL * 4 * n_cu;
+  for (int s : {16, 8, 4})
+    if (reps > 0 && reps * s <= waves)
+      return s;
+  return 1;
+}
+
 /* New device copies of the pack, task and chain arrays for the current
  * members (fresh blocks: queued launches keep reading the old ones, which
  * are released behind them). */
 int gd_engine::rebuild()
 {
   drop_graph();
+  split_eff = choose_split();
+  gd_launch_cfg lce = lc;
+  lce.split = split_eff;
   std::vector<gd_pack> packs;
   std::vector<gd_chain> chains;
   /* occupancy class of every complex: workgroups per CU at its LDS need */
@@ -217,7 +246,7 @@ int gd_engine::rebuild()
     if (it != occ_cache.end())
       return it->second;
     int k = 0;
-    if (gd_mc_occupancy(lc, stage, &k) != hipSuccess || k < 1)
+    if (gd_mc_occupancy(lce, stage, &k) != hipSuccess || k < 1)
       k = 1;
     occ_cache[stage] = k;
     return k;
@@ -242,7 +271,7 @@ int gd_engine::rebuild()
     Pending &cl = by_class[per_cu_of(stage)];
     cl.stage = std::max(cl.stage, stage);
     t1.clear();
-    gd_complex_tasks(*s->img, cx, lc.wpg, t1);
+    gd_complex_tasks(*s->img, cx, split_eff > 1 ? 1 : lc.wpg, t1);
     for (const gd_task &t : t1) {
       cl.tasks.push_back(t);
       cl.cost.push_back((double) t.r_count * p.lna * p.pnp_pad);
@@ -280,6 +309,8 @@ int gd_engine::rebuild()
     for (int i : order)
       sorted.push_back(cl.tasks[i]);
     classes.push_back(c);
+    if (classes.size() > 62)
+      return gd_fail(GD_ERR_ARG, "more than 62 occupancy classes");
     max_stage = std::max(max_stage, c.stage);
   }
 
@@ -329,6 +360,7 @@ int gd_engine::rebuild()
     rc = upload(arena.base() + toff, init_tasks.data(), vbytes(init_tasks));
     gd_launch_cfg ic = lc;
     ic.persistent = 0;
+    ic.split = 1;
     ic.stage_bytes = init_stage;
     hipError_t he = rc ? hipSuccess
                        : gd_launch_mc(ic, (const gd_pack *) (arena.base() + packs_off),
@@ -377,6 +409,7 @@ static hipError_t launch_classes(gd_engine &E, int *queue_base)
   for (size_t i = 0; i < E.classes.size(); ++i) {
     const gd_engine::Class &c = E.classes[i];
     gd_launch_cfg C = E.lc;
+    C.split = E.split_eff;
     C.stage_bytes = c.stage;
     hipError_t e = gd_launch_mc(C, packs, tasks + c.off, c.n,
                                 E.lc.persistent ? queue_base + i : nullptr, c.grid, 0,
@@ -502,8 +535,8 @@ int gd_engine::enqueue_cycle()
   const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
   const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
   GD_HIP_TRY(launch_classes(*this, queue_ptr()));
-  GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, stream));
-  GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), (int) classes.size(), stream));
+  GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, n_cx, queue_ptr(),
+                                (int) classes.size(), done_ptr(), stream));
   return GD_OK;
 }
 
@@ -530,9 +563,9 @@ int gd_engine::cycles(int n)
       GD_HIP_TRY(launch_classes(*this, queue_ptr()));
       GD_HIP_TRY(hipEventRecord(b, stream));
       GD_HIP_TRY(hipEventRecord(x0, stream));
-      GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, stream));
+      GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, n_cx, queue_ptr(),
+                                    (int) classes.size(), done_ptr(), stream));
       GD_HIP_TRY(hipEventRecord(x1, stream));
-      GD_HIP_TRY(gd_launch_advance(packs, n_cx, queue_ptr(), (int) classes.size(), stream));
       ev_mc.push_back(a);
       ev_mc.push_back(b);
       ev_x.push_back(x0);
@@ -629,18 +662,28 @@ int gd_engine::autotune(gd_tune_report *rep)
     he = hipEventCreate(&b);
   struct Cand {
     float ms;
-    int wpg, pers, kg, occ;
+    int wpg, pers, kg, occ, split;
   };
   std::vector<Cand> cands;
+  /* the split the resident set would get by itself: when it is above one,
+   * the replicas are few and the split launches are candidates too */
+  const int split_auto = [&] {
+    const int keep = lc.split;
+    lc.split = 0;
+    const int v = choose_split();
+    lc.split = keep;
+    return v;
+  }();
   /* time `reps` cycles of a configuration after a warm-up cycle; returns
    * the mean cycle time in ms, or a negative value on error */
-  auto time_cfg = [&](int wpg, int pers, int kg, int occ, int reps) -> float {
+  auto time_cfg = [&](int wpg, int pers, int kg, int occ, int split, int reps) -> float {
     if (rc || he != hipSuccess)
       return -1.0f;
     kde_global = kg;
     lc.wpg = wpg;
     lc.persistent = pers;
     lc.occ = occ;
+    lc.split = split;
     rc = rebuild();
     if (!rc)
       rc = trial_cycle();
@@ -661,8 +704,8 @@ int gd_engine::autotune(gd_tune_report *rep)
     return ms / (float) reps;
   };
   /* one trial of the staged search */
-  auto trial = [&](int wpg, int pers, int kg, int occ) {
-    const float ms = time_cfg(wpg, pers, kg, occ, 2);
+  auto trial = [&](int wpg, int pers, int kg, int occ, int split) {
+    const float ms = time_cfg(wpg, pers, kg, occ, split, 2);
     if (ms < 0.0f)
       return;
     if (rep && n < GD_TUNE_MAX) {
@@ -670,38 +713,45 @@ int gd_engine::autotune(gd_tune_report *rep)
       rep->persistent[n] = pers;
       rep->kde_global[n] = kg;
       rep->occ[n] = occ;
+      rep->split[n] = split_eff;
       rep->ms[n] = ms;
     }
     ++n;
-    cands.push_back({ms, wpg, pers, kg, occ});
+    cands.push_back({ms, wpg, pers, kg, occ, split});
     if (ms < best) {
       best = ms;
       best_c = lc;
       best_kg = kg;
     }
   };
-  /* staged search: launch shape; then KDE placement x register budget;
-   * then the launch shape again at the chosen placement and budget */
+  /* staged search: launch shape (and, for few replicas, wavefronts per
+   * replica); then KDE placement x register budget; then the launch shape
+   * again at the chosen placement and budget */
   const int wpgs[3] = {4, 8, 16};
   const int kg0 = kde_global, occ0 = lc.occ;
   for (int wi = 0; wi < 3; ++wi)
     for (int pers = 0; pers < 2; ++pers)
-      trial(wpgs[wi], pers, kg0, occ0);
+      trial(wpgs[wi], pers, kg0, occ0, 1);
+  if (split_auto > 1)
+    for (int sp : {4, 8, 16})
+      for (int pers = 0; pers < 2; ++pers)
+        trial(lc.wpg, pers, kg0, occ0, sp);
   {
-    /* KDE placement x register budget at the best launch shape */
+    /* KDE placement x register budget (one wavefront per replica) at the
+     * best launch shape */
     const gd_launch_cfg c = best_c;
     for (int kg = 0; kg < (any_kde_lds ? 2 : 1); ++kg)
       for (int occ : {0, 6, 8})
-        if (kg != kg0 || occ != occ0)
-          trial(c.wpg, c.persistent, any_kde_lds ? kg : kg0, occ);
+        if ((kg != kg0 || occ != occ0) && (c.split <= 1 || occ == occ0))
+          trial(c.wpg, c.persistent, any_kde_lds ? kg : kg0, occ, c.split);
   }
-  {
+  if (best_c.split <= 1) {
     const gd_launch_cfg c = best_c;
     const int kg = best_kg;
     for (int wi = 0; wi < 3; ++wi)
       for (int pers = 0; pers < 2; ++pers)
         if ((wpgs[wi] != c.wpg || pers != c.persistent) && (c.occ != occ0 || kg != kg0))
-          trial(wpgs[wi], pers, kg, c.occ);
+          trial(wpgs[wi], pers, kg, c.occ, 1);
   }
   /* the staged trials differ by a few per cent, about what one short trial
    * can resolve: the three fastest distinct configurations are timed again
@@ -713,7 +763,8 @@ int gd_engine::autotune(gd_tune_report *rep)
     for (const Cand &c : cands) {
       bool dup = false;
       for (const Cand &t : top)
-        dup |= t.wpg == c.wpg && t.pers == c.pers && t.kg == c.kg && t.occ == c.occ;
+        dup |= t.wpg == c.wpg && t.pers == c.pers && t.kg == c.kg && t.occ == c.occ &&
+               t.split == c.split;
       if (!dup)
         top.push_back(c);
       if (top.size() == 3)
@@ -721,7 +772,7 @@ int gd_engine::autotune(gd_tune_report *rep)
     }
     float conf_best = 3.0e38f;
     for (const Cand &c : top) {
-      const float ms = time_cfg(c.wpg, c.pers, c.kg, c.occ, 4);
+      const float ms = time_cfg(c.wpg, c.pers, c.kg, c.occ, c.split, 4);
       if (ms >= 0.0f && ms < conf_best) {
         conf_best = ms;
         best_c = lc;
@@ -747,6 +798,7 @@ int gd_engine::autotune(gd_tune_report *rep)
     rep->chosen_persistent = lc.persistent;
     rep->chosen_kde_global = kde_global;
     rep->chosen_occ = lc.occ;
+    rep->chosen_split = lc.split;
   }
   return rebuild();
 }
@@ -1009,6 +1061,7 @@ extern "C" int gd_engine_layout(gd_engine *e, gd_engine_layout_info *info)
   info->n_classes = (int) e->classes.size();
   info->kde_global = e->kde_global;
   info->occ = e->lc.occ;
+  info->split = e->split_eff;
   info->arena_used = e->arena.used();
   info->arena_size = e->arena.size();
   int kde_lds = 1;
@@ -1330,6 +1383,7 @@ extern "C" int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_confi
   c.use_graph = bc.no_graph ? 0 : 1;
   c.kde_policy = bc.kde_policy;
   c.occ = bc.occ;
+  c.split = bc.split;
   int rc = eng.init(&c);
   if (rc)
     return rc;

@@ -114,6 +114,8 @@ struct gd_engine {
   };
   std::vector<Class> classes;
   int kde_global = 0;               /* 1: every complex reads KDE points from HBM */
+  int split_eff = 1;                /* wavefronts per replica of the current launches
+                                       (lc.split 0 = chosen at each rebuild) */
   int timing = 0;
   std::vector<hipEvent_t> ev_mc, ev_x;
   double mc_ms = 0.0, x_ms = 0.0;
@@ -167,6 +169,7 @@ struct gd_engine {
   int grid() const;                 /* workgroups per cycle, all classes */
   int resident() const;             /* workgroups resident at once, all classes */
   int *queue_ptr();
+  int *done_ptr();
   size_t free_bytes() const { return arena.size() - arena.used(); }
 
 private:
@@ -180,6 +183,8 @@ private:
   int upload_image(GdImage &I, size_t off, GdBlock &b);
   int enqueue_cycle();
   int trial_cycle();
+  /* wavefronts per replica for the resident complexes under lc.split */
+  int choose_split() const;
 };
 typedef gd_engine gd_engine_t;
 

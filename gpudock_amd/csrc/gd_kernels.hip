@@ -137,22 +137,28 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Energy of one pose, one wavefront (calcenergy_d.cu:22-374).               */
-/* Lane j holds ligand atoms j and j + 64 in (ax0..az0) and (ax1..az1).      */
-/* On return out[] (slot order) and *cmcc (MCC against the reference       */
-/* contacts, CalcMcc_d calcmcc_d.cu:148-213, or 0) are identical in every   */
-/* lane.  prt = the protein conformer staged in smem.                       */
+/* Energy of one pose (calcenergy_d.cu:22-374), in summation blocks          */
+/* (gd_block_size in gd_canon.h): atom blocks of the pair loop, blocks of    */
+/* KDE rounds, one item per position restraint.  Lane j holds ligand atoms   */
+/* j and j + 64 in (ax0..az0) and (ax1..az1).  prt = the protein conformer   */
+/* staged in smem.                                                           */
 /* ------------------------------------------------------------------------ */
-__device__ __forceinline__ void gd_energy_wave(
+/* Lane partials of the directly summed terms over one atom block. */
+struct gd_blk {
+  float vdw, pmf, psp, hdb, ele;
+};
+
+/* Pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) over ligand
+ * atoms [l0, l1): lane partials from +0 into blk, the block's restraint sum
+ * (uniform) into *ehpc_b, MCC confusion counts (uniform) added to tp/fp/fn. */
+__device__ __forceinline__ void gd_atoms_block(
     const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
-    float ay1, float az1, const float cen[3], float out[GD_E_N], float *cmcc)
+    float ay1, float az1, int l0, int l1, gd_blk &blk, float *ehpc_b, int &tp, int &fp,
+    int &fn)
 {
   const float4 *__restrict__ s_pos = (const float4 *) smem;
   const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
   const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
-  const int kde_lds = P.kde_in_lds;
-  const __amdgpu_buffer_rsrc_t kde_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void *) P.kde, 0, P.pnk * 16, GD_BUFFER_DWORD3);
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
@@ -172,7 +178,6 @@ __device__ __forceinline__ void gd_energy_wave(
   typedef const __attribute__((address_space(4))) unsigned long long *mask_ptr;
   const mask_ptr mref =
       P.mcc_ref ? (mask_ptr) (P.mcc_ref + (size_t) prt * lna * nchunk) : nullptr;
-  int tp = 0, fp = 0, fn = 0;
   typedef const __attribute__((address_space(1))) float *gfloat_ptr;
   const gfloat_ptr psp_prt = (gfloat_ptr) (P.psp_pt + (size_t) prt * n_rows * P.pnp_pad);
   const int kh = gd_uload(P.chunk_bounds + 2 * prt);
@@ -184,8 +189,7 @@ __device__ __forceinline__ void gd_energy_wave(
   const __amdgpu_buffer_rsrc_t eh_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void *) (P.pt_eh + (size_t) prt * P.pnp_pad), 0, P.pnp_pad * 8, GD_BUFFER_DWORD3);
 
-  /* pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) */
-  for (int l = 0; l < lna; ++l) {
+  for (int l = l0; l < l1; ++l) {
     const int src = l & (GD_WAVE - 1);
     const bool hi = l >= GD_WAVE;
     const float lx = gd_bcast(hi ? ax1 : ax0, src);
@@ -269,109 +273,175 @@ __device__ __forceinline__ void gd_energy_wave(
     const float z = (hs - ar.hpl0) * ar.inv_hpl1;
     ehpc += 0.5f * z * z - ar.hpl2;
   }
+  blk.vdw = acc.vdw;
+  blk.pmf = acc.pmf;
+  blk.psp = acc.psp;
+  blk.hdb = acc.hdb;
+  blk.ele = eele;
+  *ehpc_b = ehpc;
+}
 
-  /* KDE (calcenergy_d.cu:175-213): sum over atoms of the mean of
-   * e^(kde2 d^2) over the KDE points of the atom's type, taken as one list of
-   * (atom, point) pairs, 64 to a round, so that no lane idles at the end of
-   * an atom's short segment.  Lane j of a round takes pair 64 t + j: its
-   * atom's coordinates come from the lane that holds the atom (atoms < 64
-   * from (ax0, ay0, az0), then atoms >= 64 from (ax1, ay1, az1)), the point
-   * carries the pair's weight 1 / (segment length) in .w, and each lane
-   * keeps its weighted sum across the whole list. */
+/* KDE rounds (calcenergy_d.cu:175-213): the pose's (atom, KDE point) pairs as
+ * one list, 64 to a round -- atoms < 64 first, then atoms >= 64, each group
+ * padded to whole rounds -- so that no lane idles at the end of an atom's
+ * short segment.  Lane j of round r takes pair 64 r + j: its atom's
+ * coordinates come from the lane that holds the atom, the point carries the
+ * pair's weight 1 / (segment length) in .w.  Returns this lane's sum over
+ * rounds [r0, r1) from +0. */
+__device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *smem, float ax0,
+                                               float ay0, float az0, float ax1, float ay1,
+                                               float az1, int r0, int r1)
+{
+  const unsigned ulane = threadIdx.x & (GD_WAVE - 1);
   const float kde2 = P.kde2;
-  float ekde = 0.0f, ekde_lane = 0.0f;
-  {
-    const int n_lo = P.kde_n_lo, n_hi = P.kde_n_hi;
-    const int hi0 = (n_lo + GD_WAVE - 1) & ~(GD_WAVE - 1);
-    const int n_all = hi0 + ((n_hi + GD_WAVE - 1) & ~(GD_WAVE - 1));
-    const __amdgpu_buffer_rsrc_t pairs_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *) P.kde_pairs, 0, n_all * 4, GD_BUFFER_DWORD3);
-    auto group = [&](auto read, float gx, float gy, float gz, int p0, int n) {
-      for (int p = 0; p < n; p += GD_WAVE) {
-        /* every lane runs the permutes (padding entries name lane 0 and
-         * point 0); only the lanes with a pair keep the sum */
-        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
-                                                                (p0 + p) * 4, 0);
-        const int src = (int) (e & 0xffu);
-        const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
-        const float ly = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gy)));
-        const float lz = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gz)));
-        const gd_f4 q = read(e >> 12);   /* byte offset of the point */
-        const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-        float es;
-        const float ep = gd_exp_hot_split(kde2 * d2, &es);
-        const float sum = gd_fma(ep * es, q.w, ekde_lane);   /* ep * 2^n is exact */
-        ekde_lane = (int) ulane < n - p ? sum : ekde_lane;
-      }
-    };
-    auto both = [&](auto read) {
-      group(read, ax0, ay0, az0, 0, n_lo);
-      group(read, ax1, ay1, az1, hi0, n_hi);
-    };
-    if (kde_lds) {
-      const __attribute__((address_space(3))) char *pts =
-          (const __attribute__((address_space(3))) char *) (smem + P.off_kde);
-      both([&](uint32_t off) {
-        return *(const __attribute__((address_space(3))) gd_f4 *) (pts + off);
-      });
-    } else {
-      /* from HBM as a raw buffer */
-      both([&](uint32_t off) {
-        return __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(
-                                             kde_rsrc, off, 0, 0));
-      });
+  const int n_lo = P.kde_n_lo, n_hi = P.kde_n_hi;
+  const int rounds_lo = (n_lo + GD_WAVE - 1) / GD_WAVE;
+  const int n_all = (rounds_lo + (n_hi + GD_WAVE - 1) / GD_WAVE) * GD_WAVE;
+  const __amdgpu_buffer_rsrc_t pairs_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *) P.kde_pairs, 0, n_all * 4, GD_BUFFER_DWORD3);
+  float lane_sum = 0.0f;
+  auto rounds = [&](auto read) {
+    for (int r = r0; r < r1; ++r) {
+      const bool hi = r >= rounds_lo;
+      /* pairs of this round's group left from its first lane on */
+      const int left = hi ? n_hi - (r - rounds_lo) * GD_WAVE : n_lo - r * GD_WAVE;
+      /* every lane runs the permutes (padding entries name lane 0 and
+       * point 0); only the lanes with a pair keep the sum */
+      const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
+                                                              r * GD_WAVE * 4, 0);
+      const int src = (int) (e & 0xffu);
+      const float gx = hi ? ax1 : ax0, gy = hi ? ay1 : ay0, gz = hi ? az1 : az0;
+      const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
+      const float ly = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gy)));
+      const float lz = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gz)));
+      const gd_f4 q = read(e >> 12);   /* byte offset of the point */
+      const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+      float es;
+      const float ep = gd_exp_hot_split(kde2 * d2, &es);
+      const float sum = gd_fma(ep * es, q.w, lane_sum);   /* ep * 2^n is exact */
+      lane_sum = (int) ulane < left ? sum : lane_sum;
+    }
+  };
+  if (P.kde_in_lds) {
+    const __attribute__((address_space(3))) char *pts =
+        (const __attribute__((address_space(3))) char *) (smem + P.off_kde);
+    rounds([&](uint32_t off) {
+      return *(const __attribute__((address_space(3))) gd_f4 *) (pts + off);
+    });
+  } else {
+    /* from HBM as a raw buffer */
+    const __amdgpu_buffer_rsrc_t kde_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.kde, 0, P.pnk * 16, GD_BUFFER_DWORD3);
+    rounds([&](uint32_t off) {
+      return __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(kde_rsrc, off, 0,
+                                                                             0));
+    });
+  }
+  return lane_sum;
+}
+
+/* KDE rounds of the pose: (rounds_lo + rounds_hi). */
+__device__ __forceinline__ int gd_kde_round_count(const gd_pack &P)
+{
+  return (P.kde_n_lo + GD_WAVE - 1) / GD_WAVE + (P.kde_n_hi + GD_WAVE - 1) / GD_WAVE;
+}
+
+/* Position restraint m (calcenergy_d.cu:240-258): tcc_m sqrt(mean squared
+ * distance of the atoms it names), 0 when it names none (uniform). */
+__device__ __forceinline__ float gd_lhm_one(const gd_pack &P, float ax0, float ay0, float az0,
+                                            float ax1, float ay1, float az1, int m)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int lna = P.lna;
+  float s = 0.0f, c = 0.0f;
+  const float4 *tm = P.mcs + (size_t) m * lna;
+  if (lane < lna) {
+    const float4 t = tm[lane];
+    if (t.w != 0.0f) {
+      s += gd_dist2(ax0 - t.x, ay0 - t.y, az0 - t.z);
+      c += 1.0f;
     }
   }
-  ekde = gd_wave_sum(ekde_lane) / P.kde3;
+  if (lane + GD_WAVE < lna) {
+    const float4 t = tm[lane + GD_WAVE];
+    if (t.w != 0.0f) {
+      s += gd_dist2(ax1 - t.x, ay1 - t.y, az1 - t.z);
+      c += 1.0f;
+    }
+  }
+  s = gd_wave_sum(s);
+  c = gd_wave_sum(c);
+  return c > 0.0f ? gd_uload(P.mcs_tcc + m) * gd_sqrt(s / c) : 0.0f;
+}
 
-  /* position restraints (calcenergy_d.cu:229-270) */
-  float elhm = 0.0f;
+/* From the lane totals to the energies: wave sums, KDE normalisation, the
+ * restraint log, the distance term, averaging, normalisation and total
+ * (calcenergy_d.cu:276-371); out[] (slot order) and *cmcc uniform. */
+__device__ __forceinline__ void gd_energy_tail(const gd_pack &P, const gd_blk &tot,
+                                               float ehpc, float kde_lane, float elhm_sum,
+                                               int tp, int fp, int fn, const float cen[3],
+                                               float out[GD_E_N], float *cmcc)
+{
   const int pos = P.pos;
-  for (int m = 0; m < pos; ++m) {
-    float s = 0.0f, c = 0.0f;
-    const float4 *tm = P.mcs + (size_t) m * lna;
-    if (lane < lna) {
-      const float4 t = tm[lane];
-      if (t.w != 0.0f) {
-        s += gd_dist2(ax0 - t.x, ay0 - t.y, az0 - t.z);
-        c += 1.0f;
-      }
-    }
-    if (lane + GD_WAVE < lna) {
-      const float4 t = tm[lane + GD_WAVE];
-      if (t.w != 0.0f) {
-        s += gd_dist2(ax1 - t.x, ay1 - t.y, az1 - t.z);
-        c += 1.0f;
-      }
-    }
-    s = gd_wave_sum(s);
-    c = gd_wave_sum(c);
-    if (c > 0.0f)
-      elhm += gd_uload(P.mcs_tcc + m) * gd_sqrt(s / c);
-  }
-  elhm = (pos > 0) ? gd_log(elhm / (float) pos) : 0.0f;
-
+  const float ekde = gd_wave_sum(kde_lane) / P.kde3;
+  const float elhm = (pos > 0) ? gd_log(elhm_sum / (float) pos) : 0.0f;
   /* distance to the pocket centre (calcenergy_d.cu:276-283) */
   const float edst = gd_sqrt(gd_dist2(cen[0] - P.pocket_center[0],
                                       cen[1] - P.pocket_center[1],
                                       cen[2] - P.pocket_center[2]));
-
   float sums[GD_F_NTERM];
-  sums[GD_F_VDW] = gd_wave_sum(acc.vdw);
-  sums[GD_F_ELE] = gd_wave_sum(eele);
-  sums[GD_F_PMF] = gd_wave_sum(acc.pmf);
+  sums[GD_F_VDW] = gd_wave_sum(tot.vdw);
+  sums[GD_F_ELE] = gd_wave_sum(tot.ele);
+  sums[GD_F_PMF] = gd_wave_sum(tot.pmf);
   sums[GD_F_HPC] = ehpc;
-  sums[GD_F_HDB] = gd_wave_sum(acc.hdb);
+  sums[GD_F_HDB] = gd_wave_sum(tot.hdb);
   sums[GD_F_DST] = edst;
-  sums[GD_F_PSP] = gd_wave_sum(acc.psp);
+  sums[GD_F_PSP] = gd_wave_sum(tot.psp);
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;
-  gd_finalize(sums, (float) lna, P.a_para, P.b_para, P.w, P.term_order, P.combine, out);
+  gd_finalize(sums, (float) P.lna, P.a_para, P.b_para, P.w, P.term_order, P.combine, out);
   *cmcc = 0.0f;
-  if (mref) {
-    const int tn = lna * P.pnp - tp - fp - fn;   /* padding points are never contacts */
+  if (P.mcc_ref) {
+    const int tn = P.lna * P.pnp - tp - fp - fn;   /* padding points are never contacts */
     *cmcc = gd_mcc((float) tp, (float) fn, (float) fp, (float) tn);
   }
+}
+
+/* One wavefront evaluates every block in order.  On return out[] (slot
+ * order) and *cmcc (MCC against the reference contacts, CalcMcc_d
+ * calcmcc_d.cu:148-213, or 0) are identical in every lane. */
+__device__ __forceinline__ void gd_energy_wave(
+    const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
+    float ay1, float az1, const float cen[3], float out[GD_E_N], float *cmcc)
+{
+  gd_blk tot;
+  tot.vdw = 0.0f; tot.pmf = 0.0f; tot.psp = 0.0f; tot.hdb = 0.0f; tot.ele = 0.0f;
+  float ehpc = 0.0f;
+  int tp = 0, fp = 0, fn = 0;
+  const int lna = P.lna;
+  const int bs = gd_block_size(lna);
+  for (int l0 = 0; l0 < lna; l0 += bs) {
+    gd_blk blk;
+    float eb;
+    gd_atoms_block(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, l0, min(l0 + bs, lna), blk, &eb,
+                   tp, fp, fn);
+    tot.vdw += blk.vdw;
+    tot.pmf += blk.pmf;
+    tot.psp += blk.psp;
+    tot.hdb += blk.hdb;
+    tot.ele += blk.ele;
+    ehpc += eb;
+  }
+  const int n_rounds = gd_kde_round_count(P);
+  const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
+  float kde_lane = 0.0f;
+  for (int r0 = 0; r0 < n_rounds; r0 += kbs)
+    kde_lane += gd_kde_rounds(P, smem, ax0, ay0, az0, ax1, ay1, az1, r0,
+                              min(r0 + kbs, n_rounds));
+  float elhm = 0.0f;
+  for (int m = 0; m < P.pos; ++m)
+    elhm += gd_lhm_one(P, ax0, ay0, az0, ax1, ay1, az1, m);
+  gd_energy_tail(P, tot, ehpc, kde_lane, elhm, tp, fp, fn, cen, out, cmcc);
 }
 
 /* Move the conformer held by this lane (move_d.cu:86-132). */
@@ -456,22 +526,134 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
   }
 }
 
-/* n_steps MC steps from step s0.  The movematrices and energies live in
- * the wave's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in registers. */
+/* ------------------------------------------------------------------------ */
+/* One replica on SPLIT wavefronts.  Its area in LDS (floats): lane partials */
+/* of every atom block (5 terms x 64 lanes) and of every block of KDE rounds */
+/* (64 lanes); per atom block the restraint sum and the MCC counts; per      */
+/* position restraint its value.  Wavefront `role` evaluates items role,     */
+/* role + SPLIT, ... of the list (atom blocks, KDE blocks, restraints); after */
+/* a barrier wavefront 0 adds the partials in block order -- the order one   */
+/* wavefront keeps alone -- so the energies have the same bits.              */
+/* ------------------------------------------------------------------------ */
+#define GD_SA_PART 0
+#define GD_SA_KPART (GD_MAX_BLOCKS * 5 * GD_WAVE)
+#define GD_SA_EHPC (GD_SA_KPART + GD_MAX_BLOCKS * GD_WAVE)
+#define GD_SA_MCC (GD_SA_EHPC + GD_MAX_BLOCKS)
+#define GD_SA_LHM (GD_SA_MCC + GD_MAX_BLOCKS * 3)
+#define GD_SA_XYZ (GD_SA_LHM + GD_SPLIT_MAX_POS)   /* the placed atoms, by lane */
+#define GD_SA_CEN (GD_SA_XYZ + 6 * GD_WAVE)       /* the placed centre */
+#define GD_SA_FLOATS (GD_SA_CEN + 4)
+static_assert(GD_SA_FLOATS * 4 == GD_SPLIT_AREA_BYTES, "split area size");
+
+__device__ __forceinline__ void gd_energy_items(
+    const gd_pack &P, const char *smem, gd_lds_f32 *A, int role, int split, int prt,
+    float ax0, float ay0, float az0, float ax1, float ay1, float az1)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int lna = P.lna;
+  const int bs = gd_block_size(lna);
+  const int nb = (lna + bs - 1) / bs;
+  const int n_rounds = gd_kde_round_count(P);
+  const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
+  const int nbk = (n_rounds + kbs - 1) / kbs;
+  const int n_items = nb + nbk + P.pos;
+  for (int i = role; i < n_items; i += split) {
+    if (i < nb) {
+      gd_blk blk;
+      float eb;
+      int tp = 0, fp = 0, fn = 0;
+      gd_atoms_block(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, i * bs, min(i * bs + bs, lna),
+                     blk, &eb, tp, fp, fn);
+      gd_lds_f32 *pb = A + GD_SA_PART + i * 5 * GD_WAVE + lane;
+      pb[0 * GD_WAVE] = blk.vdw;
+      pb[1 * GD_WAVE] = blk.pmf;
+      pb[2 * GD_WAVE] = blk.psp;
+      pb[3 * GD_WAVE] = blk.hdb;
+      pb[4 * GD_WAVE] = blk.ele;
+      if (lane == 0) {
+        A[GD_SA_EHPC + i] = eb;
+        A[GD_SA_MCC + 3 * i + 0] = __int_as_float(tp);
+        A[GD_SA_MCC + 3 * i + 1] = __int_as_float(fp);
+        A[GD_SA_MCC + 3 * i + 2] = __int_as_float(fn);
+      }
+    } else if (i < nb + nbk) {
+      const int kb = i - nb;
+      A[GD_SA_KPART + kb * GD_WAVE + lane] =
+          gd_kde_rounds(P, smem, ax0, ay0, az0, ax1, ay1, az1, kb * kbs,
+                        min(kb * kbs + kbs, n_rounds));
+    } else {
+      const int m = i - nb - nbk;
+      const float v = gd_lhm_one(P, ax0, ay0, az0, ax1, ay1, az1, m);
+      if (lane == 0)
+        A[GD_SA_LHM + m] = v;
+    }
+  }
+}
+
+/* Wavefront 0, after the barrier: the partials added in block order. */
+__device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds_f32 *A,
+                                                  const float cen[3], float out[GD_E_N],
+                                                  float *cmcc)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int lna = P.lna;
+  const int bs = gd_block_size(lna);
+  const int nb = (lna + bs - 1) / bs;
+  const int n_rounds = gd_kde_round_count(P);
+  const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
+  const int nbk = (n_rounds + kbs - 1) / kbs;
+  gd_blk tot;
+  tot.vdw = 0.0f; tot.pmf = 0.0f; tot.psp = 0.0f; tot.hdb = 0.0f; tot.ele = 0.0f;
+  float ehpc = 0.0f;
+  int tp = 0, fp = 0, fn = 0;
+  for (int b = 0; b < nb; ++b) {
+    const gd_lds_f32 *pb = A + GD_SA_PART + b * 5 * GD_WAVE + lane;
+    tot.vdw += pb[0 * GD_WAVE];
+    tot.pmf += pb[1 * GD_WAVE];
+    tot.psp += pb[2 * GD_WAVE];
+    tot.hdb += pb[3 * GD_WAVE];
+    tot.ele += pb[4 * GD_WAVE];
+    ehpc += A[GD_SA_EHPC + b];
+    tp += __float_as_int(A[GD_SA_MCC + 3 * b + 0]);
+    fp += __float_as_int(A[GD_SA_MCC + 3 * b + 1]);
+    fn += __float_as_int(A[GD_SA_MCC + 3 * b + 2]);
+  }
+  float kde_lane = 0.0f;
+  for (int kb = 0; kb < nbk; ++kb)
+    kde_lane += A[GD_SA_KPART + kb * GD_WAVE + lane];
+  float elhm = 0.0f;
+  for (int m = 0; m < P.pos; ++m)
+    elhm += A[GD_SA_LHM + m];
+  gd_energy_tail(P, tot, ehpc, kde_lane, elhm, tp, fp, fn, cen, out, cmcc);
+}
+
+/* n_steps MC steps from step s0 of replica r on SPLIT wavefronts (SPLIT = 1:
+ * one wavefront, no area, no barrier).  The movematrices and energies live
+ * in the replica's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in
+ * registers; wavefront 0 keeps the decisions, records and state, and places
+ * the moved ligand in the area for the others, which only evaluate items:
+ * per step, wavefront 0 moves, barrier, all evaluate, barrier, wavefront 0
+ * adds and decides. */
+template <int SPLIT>
 __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *smem,
-                                                 gd_lds_f32 *ws, int r, int prt, int s0,
-                                                 int n_steps)
+                                                 gd_lds_f32 *ws, gd_lds_f32 *A, int role,
+                                                 int r, int prt, int s0, int n_steps)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const gd_state S = P.S;
   const int lconf = gd_uload(P.rep_lig + r);
-  if (lane < 6)
-    ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
-  if (lane < GD_E_N)
-    ws[GD_WS_EOLD + lane] = S.e[r * 10 + lane];
-  if (lane == 0)
-    ws[GD_WS_CMCC] = S.cmcc[r];
-  __builtin_amdgcn_wave_barrier();
+  if (role == 0) {
+    if (lane < 6)
+      ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
+    if (lane < GD_E_N)
+      ws[GD_WS_EOLD + lane] = S.e[r * 10 + lane];
+    if (lane == 0)
+      ws[GD_WS_CMCC] = S.cmcc[r];
+  }
+  if (SPLIT > 1)
+    __syncthreads();
+  else
+    __builtin_amdgcn_wave_barrier();
   float e_old_total = S.e[r * 10 + GD_E_TOTAL];
   float best_total = S.e_best[r * 10 + GD_E_TOTAL];
   const int itmp = S.idx_tmp[r];
@@ -480,6 +662,18 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
   int acs = 0;
 
   for (int s = s0; s < s0 + n_steps; ++s) {
+    if (SPLIT > 1 && role != 0) {
+      /* evaluate the items of the pose wavefront 0 placed */
+      __syncthreads();
+      float xyz[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        xyz[k] = A[GD_SA_XYZ + k * GD_WAVE + lane];
+      gd_energy_items(P, smem, A, role, SPLIT, prt, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4],
+                      xyz[5]);
+      __syncthreads();
+      continue;
+    }
     if (P.rec != nullptr && s % P.rec_stride == 0) {
       /* RecordLigand_d (util_d.cu:72-94): the state before this step, one
        * word per lane */
@@ -519,50 +713,76 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
     float ax0, ay0, az0, ax1, ay1, az1, cen[3], e_new[GD_E_N], cm_new;
     gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
              &az1, cen);
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-        ws[GD_WS_MNEW + k] = m_new[k];
-    }
-    gd_energy_wave(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, cen, e_new, &cm_new);
-
-    const float de = e_new[GD_E_TOTAL] - e_old_total;
-    if (u_acc < gd_exp(de * mb)) {
-      ++acs;
-      e_old_total = e_new[GD_E_TOTAL];
+    if (SPLIT == 1) {
       if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < GD_E_N; ++i)
-          ws[GD_WS_EOLD + i] = e_new[i];
+        for (int k = 0; k < 6; ++k)
+          ws[GD_WS_MNEW + k] = m_new[k];
+      }
+      gd_energy_wave(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, cen, e_new, &cm_new);
+    } else {
+      if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 6; ++k)
-          ws[GD_WS_MOLD + k] = ws[GD_WS_MNEW + k];
-        ws[GD_WS_CMCC] = cm_new;
+          ws[GD_WS_MNEW + k] = m_new[k];
       }
-      if (e_new[GD_E_TOTAL] < best_total) {
-        best_total = e_new[GD_E_TOTAL];
+      A[GD_SA_XYZ + 0 * GD_WAVE + lane] = ax0;
+      A[GD_SA_XYZ + 1 * GD_WAVE + lane] = ay0;
+      A[GD_SA_XYZ + 2 * GD_WAVE + lane] = az0;
+      A[GD_SA_XYZ + 3 * GD_WAVE + lane] = ax1;
+      A[GD_SA_XYZ + 4 * GD_WAVE + lane] = ay1;
+      A[GD_SA_XYZ + 5 * GD_WAVE + lane] = az1;
+      __syncthreads();
+      gd_energy_items(P, smem, A, role, SPLIT, prt, ax0, ay0, az0, ax1, ay1, az1);
+      __syncthreads();
+      gd_energy_combine(P, A, cen, e_new, &cm_new);
+    }
+
+    {
+      const float de = e_new[GD_E_TOTAL] - e_old_total;
+      if (u_acc < gd_exp(de * mb)) {
+        ++acs;
+        e_old_total = e_new[GD_E_TOTAL];
         if (lane == 0) {
 #pragma unroll
-          for (int k = 0; k < 6; ++k)
-            S.mv_best[r * 8 + k] = ws[GD_WS_MNEW + k];
-#pragma unroll
           for (int i = 0; i < GD_E_N; ++i)
-            S.e_best[r * 10 + i] = e_new[i];
-          S.cmcc_best[r] = cm_new;
-          S.step_best[r] = s;
+            ws[GD_WS_EOLD + i] = e_new[i];
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            ws[GD_WS_MOLD + k] = ws[GD_WS_MNEW + k];
+          ws[GD_WS_CMCC] = cm_new;
+        }
+        if (e_new[GD_E_TOTAL] < best_total) {
+          best_total = e_new[GD_E_TOTAL];
+          if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+              S.mv_best[r * 8 + k] = ws[GD_WS_MNEW + k];
+#pragma unroll
+            for (int i = 0; i < GD_E_N; ++i)
+              S.e_best[r * 10 + i] = e_new[i];
+            S.cmcc_best[r] = cm_new;
+            S.step_best[r] = s;
+          }
         }
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    /* (split: the next step's first barrier orders the new state and the
+     * next placement before the other wavefronts read them, and this step's
+     * partials are read before any wavefront writes the next ones) */
+    if (SPLIT == 1)
+      __builtin_amdgcn_wave_barrier();
   }
 
-  if (lane < 6)
-    S.mv[r * 8 + lane] = ws[GD_WS_MOLD + lane];
-  if (lane < GD_E_N)
-    S.e[r * 10 + lane] = ws[GD_WS_EOLD + lane];
-  if (lane == 0) {
-    S.acs_mc[r] += acs;
-    S.cmcc[r] = ws[GD_WS_CMCC];
+  if (role == 0) {
+    if (lane < 6)
+      S.mv[r * 8 + lane] = ws[GD_WS_MOLD + lane];
+    if (lane < GD_E_N)
+      S.e[r * 10 + lane] = ws[GD_WS_EOLD + lane];
+    if (lane == 0) {
+      S.acs_mc[r] += acs;
+      S.cmcc[r] = ws[GD_WS_CMCC];
+    }
   }
 }
 
@@ -621,7 +841,7 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
       if (w < task.r_count) {
         const int r = task.r_begin + w;
         if (mode == 0)
-          gd_replica_cycle(P, smem, ws, r, task.prt, s0, n_steps);
+          gd_replica_cycle<1>(P, smem, ws, nullptr, 0, r, task.prt, s0, n_steps);
         else
           gd_replica_init(P, smem, r);
       }
@@ -631,81 +851,173 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
   }
 }
 
-/* ------------------------------------------------------------------------ */
-/* Temperature exchange (exchangereplicas_d.cu:28-78), one lane per chain.  */
-/* ------------------------------------------------------------------------ */
-#define GD_MAXTMP_DEV 64
-
-__global__ void __launch_bounds__(256)
-gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict__ chains,
-                   int n_chain)
+/* MC cycle with SPLIT wavefronts per replica and one replica per workgroup
+ * (tasks of one replica), grid form or persistent as gd_mc_kernel: for
+ * launches with fewer replicas than the device has SIMDs, where one
+ * wavefront per replica leaves most of the device idle and every step
+ * waits on its own latency. */
+template <int SPLIT>
+__global__ void __launch_bounds__(GD_WAVE *SPLIT)
+gd_mc_split_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ tasks,
+                   int n_tasks, int *queue, int wslot_off)
 {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_chain)
-    return;
-  const gd_chain ch = chains[g];
-  const gd_pack &P = packs[ch.cx];
-  const int n_tmp = P.n_tmp;
-  if (!P.exchange || n_tmp < 2 || gd_cycle_steps(P, P.step[0]) <= 0)
-    return;
-  const int x = P.step[1];
-  const int c = ch.c;
-  const int p = c / P.n_lig;
-  const int l = c - p * P.n_lig;
-  const int base = n_tmp * P.n_lig * p + l;
-  const gd_state S = P.S;
-  int temps[GD_MAXTMP_DEV];
-  int orders[GD_MAXTMP_DEV];
-  float ener[GD_MAXTMP_DEV];
-  for (int t = 0; t < n_tmp; ++t) {
-    const int a = base + P.n_lig * t;
-    temps[t] = S.idx_tmp[a];
-    ener[t] = S.e[a * 10 + GD_E_TOTAL];
-    orders[temps[t]] = t;
-  }
-  /* cycle x is launch x % lpw of its steps_per_dump window; the mode
-   * alternates from 0 in every window (gpusingle.cu:14-18) */
-  const int mode = (x % P.lpw) & 1;
-  const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && mode == 1 ? 1 : 0);
-  gd_u32x4 u4;
-  for (int j = 0; j < maxpair; ++j) {
-    if ((j & 3) == 0)
-      u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2),
-                            P.seed, P.cid);
-    const int i1 = 2 * j + mode;
-    const int i2 = i1 + 1;
-    const int o1 = orders[i1];
-    const int o2 = orders[i2];
-    const float delta = (P.minus_beta[i1] - P.minus_beta[i2]) * (ener[o2] - ener[o1]);
-    if (gd_exp(delta) > gd_u01(u4.v[j & 3])) {
-      const int tt = temps[o1];
-      temps[o1] = temps[o2];
-      temps[o2] = tt;
-      S.acs_x[base + P.n_lig * o1] += 1;
-      S.acs_x[base + P.n_lig * o2] += 1;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ int s_task;
+  const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / GD_WAVE);
+  gd_lds_f32 *ws = (gd_lds_f32 *) (smem + wslot_off);
+  gd_lds_f32 *A = ws + GD_WSLOT_FLOATS;
+  int staged_cx = -1, staged_prt = -1;
+  int t = queue ? -1 : (int) blockIdx.x;
+
+  for (;;) {
+    if (queue) {
+      if (threadIdx.x == 0)
+        s_task = atomicAdd(queue, 1);
+      __syncthreads();
+      t = __builtin_amdgcn_readfirstlane(s_task);
+      __syncthreads();
+    } else if (t != (int) blockIdx.x) {
+      break;
     }
+    if (t >= n_tasks)
+      break;
+    const gd_task task = tasks[t];
+    const gd_pack &P = packs[task.cx];
+    const int s0 = P.step[0];
+    const int n_steps = gd_cycle_steps(P, s0);
+    if (n_steps > 0) {
+      if (task.cx != staged_cx || task.prt != staged_prt) {
+        __syncthreads();
+        gd_stage(P, task.prt, smem);
+        __syncthreads();
+        staged_cx = task.cx;
+        staged_prt = task.prt;
+      }
+      gd_replica_cycle<SPLIT>(P, smem, ws, A, role, task.r_begin, task.prt, s0, n_steps);
+    }
+    if (!queue)
+      t = -2;
   }
-  for (int t = 0; t < n_tmp; ++t)
-    S.idx_tmp[base + P.n_lig * t] = temps[t];
 }
 
-/* After MC and exchange: step[0] += steps of this cycle, step[1] += 1;
- * threads 0 .. n_queue-1 re-arm the persistent work queues. */
+/* ------------------------------------------------------------------------ */
+/* Temperature exchange (exchangereplicas_d.cu:28-78), one wavefront per     */
+/* chain: its lanes gather the chain's temperature indices and energies into */
+/* LDS in parallel, lane 0 walks the temperature pairs, the lanes write the  */
+/* indices back.  The last workgroup to finish then advances every complex's */
+/* step counters and re-arms the work queues (what gd_advance_kernel does    */
+/* alone), so a cycle is two launches.                                       */
+/* ------------------------------------------------------------------------ */
+#define GD_MAXTMP_DEV 64
+#define GD_XWAVES 4
+
+__device__ __forceinline__ void gd_advance_all(const gd_pack *__restrict__ packs, int n_cx,
+                                               int *queue, int n_queue)
+{
+  const int n = n_cx > n_queue ? n_cx : n_queue;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (i < n_queue)
+      queue[i] = 0;
+    if (i < n_cx) {
+      const gd_pack &P = packs[i];
+      const int s0 = P.step[0];
+      const int k = gd_cycle_steps(P, s0);
+      if (k > 0) {
+        P.step[0] = s0 + k;
+        P.step[1] = P.step[1] + 1;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(GD_WAVE * GD_XWAVES)
+gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict__ chains,
+                   int n_chain, int n_cx, int *queue, int n_queue, int *done)
+{
+  __shared__ int s_temps[GD_XWAVES][GD_MAXTMP_DEV];
+  __shared__ int s_orders[GD_XWAVES][GD_MAXTMP_DEV];
+  __shared__ float s_ener[GD_XWAVES][GD_MAXTMP_DEV];
+  __shared__ int s_last;
+  const int wv = threadIdx.x / GD_WAVE;
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int g = blockIdx.x * GD_XWAVES + wv;
+  int active = 0, n_tmp = 0, n_lig = 0, base = 0, x = 0, c = 0;
+  const gd_pack *P = nullptr;
+  if (g < n_chain) {
+    const gd_chain ch = chains[g];
+    P = &packs[ch.cx];
+    n_tmp = P->n_tmp;
+    active = P->exchange && n_tmp >= 2 && gd_cycle_steps(*P, P->step[0]) > 0;
+    if (active) {
+      x = P->step[1];
+      c = ch.c;
+      n_lig = P->n_lig;
+      const int p = c / n_lig;
+      base = n_tmp * n_lig * p + (c - p * n_lig);
+      if (lane < n_tmp) {
+        const int a = base + n_lig * lane;
+        const int t = P->S.idx_tmp[a];
+        s_temps[wv][lane] = t;
+        s_ener[wv][lane] = P->S.e[a * 10 + GD_E_TOTAL];
+        s_orders[wv][t] = lane;
+      }
+    }
+  }
+  __syncthreads();
+  if (active && lane == 0) {
+    const gd_state S = P->S;
+    int *temps = s_temps[wv];
+    const int *orders = s_orders[wv];
+    const float *ener = s_ener[wv];
+    /* cycle x is launch x % lpw of its steps_per_dump window; the mode
+     * alternates from 0 in every window (gpusingle.cu:14-18) */
+    const int mode = (x % P->lpw) & 1;
+    const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && mode == 1 ? 1 : 0);
+    gd_u32x4 u4;
+    for (int j = 0; j < maxpair; ++j) {
+      if ((j & 3) == 0)
+        u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2),
+                              P->seed, P->cid);
+      const int i1 = 2 * j + mode;
+      const int i2 = i1 + 1;
+      const int o1 = orders[i1];
+      const int o2 = orders[i2];
+      const float delta = (P->minus_beta[i1] - P->minus_beta[i2]) * (ener[o2] - ener[o1]);
+      if (gd_exp(delta) > gd_u01(u4.v[j & 3])) {
+        const int tt = temps[o1];
+        temps[o1] = temps[o2];
+        temps[o2] = tt;
+        S.acs_x[base + n_lig * o1] += 1;
+        S.acs_x[base + n_lig * o2] += 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (active && lane < n_tmp)
+    P->S.idx_tmp[base + n_lig * lane] = s_temps[wv][lane];
+  /* the last workgroup advances the counters every workgroup has read */
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(done, 1) == (int) gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    gd_advance_all(packs, n_cx, queue, n_queue);
+    if (threadIdx.x == 0)
+      *done = 0;
+  }
+}
+
+/* After MC (no temperature chains to exchange): step[0] += steps of this
+ * cycle, step[1] += 1; threads 0 .. n_queue-1 re-arm the persistent work
+ * queues. */
 __global__ void __launch_bounds__(256)
 gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue, int n_queue)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_queue)
-    queue[i] = 0;
-  if (i >= n_cx)
-    return;
-  const gd_pack &P = packs[i];
-  const int s0 = P.step[0];
-  const int n = gd_cycle_steps(P, s0);
-  if (n > 0) {
-    P.step[0] = s0 + n;
-    P.step[1] = P.step[1] + 1;
-  }
+  if (blockIdx.x == 0)
+    gd_advance_all(packs, n_cx, queue, n_queue);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -992,11 +1304,29 @@ static hipError_t launch_mc_wpg(const gd_launch_cfg &C, const gd_pack *packs,
   }
 }
 
+template <int SPLIT>
+static hipError_t launch_split(const gd_pack *packs, const gd_task *tasks, int n_tasks,
+                               int *queue, int grid, int stage_bytes, hipStream_t st)
+{
+  const int lds = stage_bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES;
+  hipLaunchKernelGGL(gd_mc_split_kernel<SPLIT>, dim3(grid), dim3(GD_WAVE * SPLIT), lds, st,
+                     packs, tasks, n_tasks, queue, stage_bytes);
+  return hipGetLastError();
+}
+
 hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
                         int n_tasks, int *queue, int grid, int mode, hipStream_t st)
 {
   if (n_tasks <= 0 || grid <= 0)
     return hipSuccess;
+  if (mode == 0 && C.split > 1) {
+    switch (C.split) {
+    case 4: return launch_split<4>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, st);
+    case 8: return launch_split<8>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, st);
+    case 16: return launch_split<16>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, st);
+    default: return hipErrorInvalidValue;
+    }
+  }
   switch (C.wpg) {
   case 4: return launch_mc_wpg<4>(C, packs, tasks, n_tasks, queue, grid, mode, st);
   case 16: return launch_mc_wpg<16>(C, packs, tasks, n_tasks, queue, grid, mode, st);
@@ -1005,21 +1335,21 @@ hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_t
 }
 
 hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int n_chain,
-                              hipStream_t st)
+                              int n_cx, int *queue, int n_queue, int *done, hipStream_t st)
 {
   if (n_chain <= 0)
-    return hipSuccess;
-  hipLaunchKernelGGL(gd_exchange_kernel, dim3((n_chain + 255) / 256), dim3(256), 0, st,
-                     packs, chains, n_chain);
+    return gd_launch_advance(packs, n_cx, queue, n_queue, st);
+  hipLaunchKernelGGL(gd_exchange_kernel, dim3((n_chain + GD_XWAVES - 1) / GD_XWAVES),
+                     dim3(GD_WAVE * GD_XWAVES), 0, st, packs, chains, n_chain, n_cx, queue,
+                     n_queue, done);
   return hipGetLastError();
 }
 
 hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, int n_queue,
                              hipStream_t st)
 {
-  const int n = n_cx > n_queue ? n_cx : (n_queue > 1 ? n_queue : 1);
-  hipLaunchKernelGGL(gd_advance_kernel, dim3((n + 255) / 256), dim3(256), 0, st, packs, n_cx,
-                     queue, n_queue);
+  hipLaunchKernelGGL(gd_advance_kernel, dim3(1), dim3(256), 0, st, packs, n_cx, queue,
+                     n_queue);
   return hipGetLastError();
 }
 
@@ -1072,9 +1402,20 @@ static hipError_t set_attr_wpg(int bytes)
   return e;
 }
 
+template <int SPLIT>
+static hipError_t set_attr_split(int bytes)
+{
+  return hipFuncSetAttribute((const void *) gd_mc_split_kernel<SPLIT>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES);
+}
+
 hipError_t gd_set_kernel_attrs(int stage_bytes)
 {
-  hipError_t e = set_attr_wpg<4>(stage_bytes);
+  hipError_t e = set_attr_split<4>(stage_bytes);
+  if (e == hipSuccess) e = set_attr_split<8>(stage_bytes);
+  if (e == hipSuccess) e = set_attr_split<16>(stage_bytes);
+  if (e == hipSuccess) e = set_attr_wpg<4>(stage_bytes);
   if (e == hipSuccess) e = set_attr_wpg<8>(stage_bytes);
   if (e == hipSuccess) e = set_attr_wpg<16>(stage_bytes);
   if (e == hipSuccess)
@@ -1100,8 +1441,22 @@ static hipError_t occ_wpg(int occ, int stage_bytes, int *blocks)
   }
 }
 
+template <int SPLIT>
+static hipError_t occ_split(int stage_bytes, int *blocks)
+{
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks, gd_mc_split_kernel<SPLIT>, GD_WAVE * SPLIT,
+      stage_bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES);
+}
+
 hipError_t gd_mc_occupancy(const gd_launch_cfg &C, int stage_bytes, int *blocks_per_cu)
 {
+  switch (C.split) {
+  case 4: return occ_split<4>(stage_bytes, blocks_per_cu);
+  case 8: return occ_split<8>(stage_bytes, blocks_per_cu);
+  case 16: return occ_split<16>(stage_bytes, blocks_per_cu);
+  default: break;
+  }
   switch (C.wpg) {
   case 4: return occ_wpg<4>(C.occ, stage_bytes, blocks_per_cu);
   case 16: return occ_wpg<16>(C.occ, stage_bytes, blocks_per_cu);

@@ -13,12 +13,17 @@ typedef struct gd_launch_cfg {
   int stage_bytes;  /* dynamic LDS for staging: max over resident complexes */
   int persistent;   /* 1: grid of resident workgroups pulling from a queue */
   int occ;          /* register budget for waves per SIMD: 0 (compiler), 6 or 8 */
+  int split;        /* wavefronts per replica in MC cycles: 1, or 4, 8, 16 with one
+                       replica per workgroup (tasks of one replica) */
 } gd_launch_cfg;
 
 hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
                         int n_tasks, int *queue, int grid, int mode, hipStream_t st);
+/* exchange over chains[0 .. n_chain), then (the last workgroup) the advance
+ * of gd_launch_advance; *done is a zeroed counter the launch leaves zeroed.
+ * With no chains, the advance alone. */
 hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int n_chain,
-                              hipStream_t st);
+                              int n_cx, int *queue, int n_queue, int *done, hipStream_t st);
 /* advances the step counters of packs[0 .. n_cx) and zeroes queue[0 .. n_queue) */
 hipError_t gd_launch_advance(const gd_pack *packs, int n_cx, int *queue, int n_queue,
                              hipStream_t st);

@@ -418,6 +418,7 @@ extern "C" int gd_server_create(const gd_server_config *cfg, gd_server **out)
   ec.use_graph = s->cfg.use_graph;
   ec.kde_policy = s->cfg.kde_policy;
   ec.occ = s->cfg.occ;
+  ec.split = s->cfg.split;
   int rc = s->eng.init(&ec);
   if (rc)
     return rc;

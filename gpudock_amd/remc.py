@@ -136,7 +136,7 @@ class Job(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("arena_bytes", C.c_size_t), ("wpg", C.c_int),
                 ("persistent", C.c_int), ("use_graph", C.c_int), ("kde_policy", C.c_int),
-                ("staging_bytes", C.c_size_t), ("occ", C.c_int)]
+                ("staging_bytes", C.c_size_t), ("occ", C.c_int), ("split", C.c_int)]
 
 
 TUNE_MAX = 16
@@ -146,8 +146,10 @@ class TuneReport(C.Structure):
     _fields_ = [("n", C.c_int), ("wpg", C.c_int * TUNE_MAX),
                 ("persistent", C.c_int * TUNE_MAX), ("kde_global", C.c_int * TUNE_MAX),
                 ("occ", C.c_int * TUNE_MAX), ("ms", C.c_double * TUNE_MAX),
+                ("split", C.c_int * TUNE_MAX),
                 ("chosen_wpg", C.c_int), ("chosen_persistent", C.c_int),
-                ("chosen_kde_global", C.c_int), ("chosen_occ", C.c_int)]
+                ("chosen_kde_global", C.c_int), ("chosen_occ", C.c_int),
+                ("chosen_split", C.c_int)]
 
 
 class EngineLayout(C.Structure):
@@ -155,13 +157,14 @@ class EngineLayout(C.Structure):
                 ("n_tasks", C.c_int), ("n_chains", C.c_int), ("n_complexes", C.c_int),
                 ("grid", C.c_int), ("resident_groups", C.c_int), ("kde_in_lds", C.c_int),
                 ("arena_used", C.c_size_t), ("arena_size", C.c_size_t),
-                ("n_classes", C.c_int), ("kde_global", C.c_int), ("occ", C.c_int)]
+                ("n_classes", C.c_int), ("kde_global", C.c_int), ("occ", C.c_int),
+                ("split", C.c_int)]
 
 
 class BatchConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("wpg", C.c_int), ("persistent", C.c_int),
                 ("no_graph", C.c_int), ("autotune", C.c_int), ("kde_policy", C.c_int),
-                ("occ", C.c_int)]
+                ("occ", C.c_int), ("split", C.c_int)]
 
 
 class BatchReport(C.Structure):
@@ -175,7 +178,7 @@ class ServerConfig(C.Structure):
                 ("persistent", C.c_int), ("use_graph", C.c_int), ("kde_policy", C.c_int),
                 ("max_resident_replicas", C.c_int), ("cycles_per_tick", C.c_int),
                 ("max_ticks_in_flight", C.c_int), ("keep_results", C.c_int),
-                ("readback_bytes", C.c_size_t), ("occ", C.c_int)]
+                ("readback_bytes", C.c_size_t), ("occ", C.c_int), ("split", C.c_int)]
 
 
 class ServerJobInfo(C.Structure):
@@ -706,8 +709,9 @@ class Engine:
     """gd_engine_*: many complexes resident on one GPU, advanced together."""
 
     def __init__(self, device=0, arena_bytes=1 << 30, wpg=8, persistent=0, use_graph=1,
-                 kde_policy=0, occ=0):
-        cfg = EngineConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy, 0, occ)
+                 kde_policy=0, occ=0, split=0):
+        cfg = EngineConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy, 0, occ,
+                           split)
         h = C.c_void_p()
         check(lib().gd_engine_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -774,7 +778,7 @@ class Engine:
 
 
 def batch_run(jobs, device=0, wpg=0, persistent=0, no_graph=0, autotune=0, kde_policy=0,
-              occ=0):
+              occ=0, split=0):
     """gd_batch_run over JobSpecs; returns ([results dict per job], report dict)."""
     n = len(jobs)
     arr = (Job * n)(*[j.c for j in jobs])
@@ -782,7 +786,7 @@ def batch_run(jobs, device=0, wpg=0, persistent=0, no_graph=0, autotune=0, kde_p
     ptrs = (C.POINTER(ReplicaResult) * n)(*[C.cast(r, C.POINTER(ReplicaResult)) for r in res])
     logs = (McLog * n)()
     rep = BatchReport()
-    cfg = BatchConfig(device, wpg, persistent, no_graph, autotune, kde_policy, occ)
+    cfg = BatchConfig(device, wpg, persistent, no_graph, autotune, kde_policy, occ, split)
     check(lib().gd_batch_run(n, arr, C.byref(cfg), ptrs, logs, C.byref(rep)))
     return [results_to_numpy(r) for r in res], _struct_dict(rep)
 
@@ -792,10 +796,10 @@ class Server:
 
     def __init__(self, device=0, arena_bytes=0, wpg=8, persistent=0, use_graph=0,
                  kde_policy=0, max_resident_replicas=0, cycles_per_tick=1,
-                 max_ticks_in_flight=2, keep_results=1, readback_bytes=0, occ=0):
+                 max_ticks_in_flight=2, keep_results=1, readback_bytes=0, occ=0, split=0):
         cfg = ServerConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy,
                            max_resident_replicas, cycles_per_tick, max_ticks_in_flight,
-                           keep_results, readback_bytes, occ)
+                           keep_results, readback_bytes, occ, split)
         h = C.c_void_p()
         check(lib().gd_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -378,6 +378,11 @@ typedef struct gd_engine_config {
   size_t staging_bytes;/* pinned host memory for uploads; 0 = 64 MiB */
   int occ;             /* MC kernel register budget: 0 (compiler's choice),
                           6 or 8 waves per SIMD */
+  int split;           /* wavefronts per replica in the MC cycles: 0 = chosen
+                          from the resident replica count (several when they
+                          are fewer than the device's SIMDs), 1 = one, or 4, 8,
+                          16 (one replica per workgroup); the energies are the
+                          same bits for every choice */
 } gd_engine_config;
 
 #define GD_TUNE_MAX 16
@@ -388,7 +393,8 @@ typedef struct gd_tune_report {
   int kde_global[GD_TUNE_MAX];
   int occ[GD_TUNE_MAX];
   double ms[GD_TUNE_MAX];   /* the MC launches of one cycle */
-  int chosen_wpg, chosen_persistent, chosen_kde_global, chosen_occ;
+  int split[GD_TUNE_MAX];   /* wavefronts per replica */
+  int chosen_wpg, chosen_persistent, chosen_kde_global, chosen_occ, chosen_split;
 } gd_tune_report;
 
 typedef struct gd_engine_layout_info {
@@ -402,6 +408,7 @@ typedef struct gd_engine_layout_info {
   int n_classes;       /* MC launches per cycle (occupancy classes) */
   int kde_global;      /* 1: KDE points read from HBM for every complex */
   int occ;             /* MC kernel register budget (waves per SIMD, 0 = default) */
+  int split;           /* wavefronts per replica in the MC cycles */
 } gd_engine_layout_info;
 
 int gd_engine_create(const gd_engine_config *cfg, gd_engine **out);
@@ -438,6 +445,7 @@ typedef struct gd_batch_config {
   int autotune;
   int kde_policy;
   int occ;             /* MC kernel register budget: 0, 6 or 8 waves per SIMD */
+  int split;           /* wavefronts per replica: as gd_engine_config.split */
 } gd_batch_config;
 
 typedef struct gd_batch_report {
@@ -476,6 +484,7 @@ typedef struct gd_server_config {
                                   0: only latency statistics are kept */
   size_t readback_bytes;       /* pinned host memory for read-back; 0 = 64 MiB */
   int occ;                     /* MC kernel register budget: 0, 6 or 8 */
+  int split;                   /* wavefronts per replica: as gd_engine_config.split */
 } gd_server_config;
 
 enum gd_job_state {

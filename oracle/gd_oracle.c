@@ -723,13 +723,23 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
   const size_t po = (size_t) prt_conf * pnp;
   const gd_enepara *ep = c->ep;
   const gd_protein *prt = c->prt;
+  /* lane totals: block partials added in block order (gd_block_size) */
+  gd_pair_acc tot[W];
+  memset(tot, 0, sizeof tot);
+  float eele_tot[W];
+  for (int j = 0; j < W; ++j)
+    eele_tot[j] = 0.0f;
+  float ehpc = 0.0f;
+  const int bs = gd_block_size(lna);
+  for (int b0 = 0; b0 < lna; b0 += bs) {
+  const int b1 = b0 + bs < lna ? b0 + bs : lna;
   gd_pair_acc acc[W];
   memset(acc, 0, sizeof acc);
   float eele[W];
   for (int j = 0; j < W; ++j)
     eele[j] = 0.0f;
-  float ehpc = 0.0f;
-  for (int l = 0; l < lna; ++l) {
+  float ehpc_b = 0.0f;
+  for (int l = b0; l < b1; ++l) {
     const float lx = ax[l], ly = ay[l], lz = az[l];
     const int lt = c->lig->t[l];
     const float lq = c->lig->c[l];
@@ -765,15 +775,33 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
       eele[j] = gd_fma(lq, acc[j].ele, eele[j]);
     const float hs = bfly(hacc);
     const float z = (hs - ep->hpl0[lt]) * (1.0f / ep->hpl1[lt]);
-    ehpc += 0.5f * z * z - ep->hpl2[lt];
+    ehpc_b += 0.5f * z * z - ep->hpl2[lt];
+  }
+  for (int j = 0; j < W; ++j) {
+    tot[j].vdw += acc[j].vdw;
+    tot[j].pmf += acc[j].pmf;
+    tot[j].psp += acc[j].psp;
+    tot[j].hdb += acc[j].hdb;
+    eele_tot[j] += eele[j];
+  }
+  ehpc += ehpc_b;
   }
 
   /* KDE as one list of (atom, point) pairs: atoms < W, each atom's points
-   * in segment order, then atoms >= W the same way; pair q of a group goes
-   * to lane q % W, weighted by 1 / (segment length) */
-  float ekde = 0.0f, ekde_lane[W];
-  for (int j = 0; j < W; ++j)
-    ekde_lane[j] = 0.0f;
+   * in segment order, then atoms >= W the same way, each group padded to
+   * whole rounds of W pairs; pair q of a group goes to lane q % W, weighted
+   * by 1 / (segment length); lane partials per block of rounds */
+  int n_pair[2] = {0, 0};
+  for (int g = 0; g < 2; ++g) {
+    const int la = g ? W : 0, lb = g ? lna : (lna < W ? lna : W);
+    for (int l = la; l < lb; ++l)
+      n_pair[g] += c->kseg[c->lig->t[l] + 1] - c->kseg[c->lig->t[l]];
+  }
+  const int rounds_lo = (n_pair[0] + W - 1) / W;
+  const int n_rounds = rounds_lo + (n_pair[1] + W - 1) / W;
+  const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
+  float ekde = 0.0f, ekde_lane[W], ekde_blk[GD_MAX_BLOCKS][W];
+  memset(ekde_blk, 0, sizeof ekde_blk);
   for (int g = 0; g < 2; ++g) {
     const int la = g ? W : 0, lb = g ? lna : (lna < W ? lna : W);
     int q = 0;
@@ -786,9 +814,15 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
         float es;
         const float e = gd_exp_hot_split(ep->kde2 * gd_dist2(ax[l] - c->kx[i], ay[l] - c->ky[i],
                                                              az[l] - c->kz[i]), &es);
-        ekde_lane[q % W] = gd_fma(e * es, w, ekde_lane[q % W]);
+        const int kb = ((g ? rounds_lo : 0) + q / W) / kbs;
+        ekde_blk[kb][q % W] = gd_fma(e * es, w, ekde_blk[kb][q % W]);
       }
     }
+  }
+  for (int j = 0; j < W; ++j) {
+    ekde_lane[j] = 0.0f;
+    for (int kb = 0; kb * kbs < n_rounds; ++kb)
+      ekde_lane[j] += ekde_blk[kb][j];
   }
   ekde = bfly(ekde_lane) / ep->kde3;
 
@@ -817,16 +851,16 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
                                       cen[2] - prt->pocket_center[2]));
   float t[W];
   float sums[GD_F_NTERM];
-  for (int j = 0; j < W; ++j) t[j] = acc[j].vdw;
+  for (int j = 0; j < W; ++j) t[j] = tot[j].vdw;
   sums[GD_F_VDW] = bfly(t);
-  sums[GD_F_ELE] = bfly(eele);
-  for (int j = 0; j < W; ++j) t[j] = acc[j].pmf;
+  sums[GD_F_ELE] = bfly(eele_tot);
+  for (int j = 0; j < W; ++j) t[j] = tot[j].pmf;
   sums[GD_F_PMF] = bfly(t);
   sums[GD_F_HPC] = ehpc;
-  for (int j = 0; j < W; ++j) t[j] = acc[j].hdb;
+  for (int j = 0; j < W; ++j) t[j] = tot[j].hdb;
   sums[GD_F_HDB] = bfly(t);
   sums[GD_F_DST] = edst;
-  for (int j = 0; j < W; ++j) t[j] = acc[j].psp;
+  for (int j = 0; j < W; ++j) t[j] = tot[j].psp;
   sums[GD_F_PSP] = bfly(t);
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;

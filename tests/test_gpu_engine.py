@@ -76,6 +76,37 @@ def test_launch_shapes_identical(jobs, wpg, persistent, no_graph, kde_policy, oc
         _same(g, r, f"wpg={wpg} persistent={persistent} job {i}")
 
 
+@pytest.mark.parametrize("split,persistent,no_graph", [(4, 0, 0), (8, 1, 0), (16, 0, 1),
+                                                       (16, 1, 0), (1, 0, 0)])
+def test_split_replicas_identical(jobs, split, persistent, no_graph):
+    """Each replica's pair loop spread over 4, 8 or 16 wavefronts (atom and
+    KDE blocks evaluated in parallel, partials added in block order): the same
+    bits as one wavefront per replica and as the replay."""
+    js, replays = jobs
+    res, rep = remc.batch_run(js, persistent=persistent, no_graph=no_graph, split=split)
+    assert rep["layout"]["split"] == split
+    for i, (g, r) in enumerate(zip(res, replays)):
+        _same(g, r, f"split={split} job {i}")
+
+
+def test_split_chosen_for_few_replicas(jobs):
+    """With fewer resident replicas than SIMDs the engine spreads each replica
+    over several wavefronts by itself, and the autotuner times those shapes
+    too; the results stay the replay's."""
+    js, replays = jobs
+    eng = remc.Engine(arena_bytes=256 << 20)
+    h = [eng.add(j) for j in js[:2]]
+    assert eng.layout()["split"] > 1
+    tune = eng.autotune()
+    assert any(s > 1 for s in tune["split"][:tune["n"]])
+    while any(eng.cycles_left(x) > 0 for x in h):
+        eng.cycles(1)
+    for i, x in enumerate(h):
+        g, _ = eng.results(x)
+        _same(g, replays[i], f"auto split job {i}")
+    eng.close()
+
+
 def test_alone_equals_mix(jobs):
     """gd_run of one job alone == the same job inside the mix."""
     js, replays = jobs
