@@ -69,6 +69,13 @@ typedef struct gd_pack {
   int off_eh, off_tab, off_kde, lds_bytes;
   int kde_in_lds;
   int pocket_in_lds;     /* 0: pocket points read through the caches */
+  /* launches that split a replica over several wavefronts (one replica per
+   * workgroup) also stage, after the KDE points, the (atom, KDE point) pair
+   * list, the position-restraint targets and the PSP rows of the staged
+   * protein conformer, when split_extra = 1 (they fit); split_lds_bytes is
+   * then the staging size of such launches (else lds_bytes) */
+  int split_extra;
+  int off_pairs, off_mcs, off_psp, split_lds_bytes;
 
   /* pocket, per protein conformer: [n_prt][pnp_pad]; points pnp..pnp_pad-1
    * are padding that contributes exactly zero to every term */
@@ -167,6 +174,6 @@ typedef struct gd_chain {
  * GD_SPLIT_MAX_POS run one wavefront per replica. */
 #define GD_SPLIT_MAX_POS 64
 #define GD_SPLIT_AREA_BYTES \
-  ((GD_MAX_BLOCKS * 6 * 64 + GD_MAX_BLOCKS * 4 + GD_SPLIT_MAX_POS + 6 * 64 + 4) * 4)
+  ((GD_MAX_BLOCKS * 6 * 64 + GD_MAX_BLOCKS * 4 + GD_SPLIT_MAX_POS + 6 * 64 + 4 + 16) * 4)
 
 #endif /* GD_DEVICE_H */

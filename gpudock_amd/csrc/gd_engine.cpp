@@ -267,7 +267,8 @@ int gd_engine::rebuild()
       p.lds_bytes = p.off_kde;
     }
     packs.push_back(p);
-    const int stage = std::max(16, p.lds_bytes);
+    const int stage = std::max(16, split_eff > 1 ? std::max(p.lds_bytes, p.split_lds_bytes)
+                                                 : p.lds_bytes);
     Pending &cl = by_class[per_cu_of(stage)];
     cl.stage = std::max(cl.stage, stage);
     t1.clear();
@@ -420,6 +421,15 @@ static hipError_t launch_classes(gd_engine &E, int *queue_base)
   return hipSuccess;
 }
 
+/* the exchange and advance of a cycle */
+static hipError_t launch_exchange(gd_engine &E)
+{
+  const gd_pack *packs = (const gd_pack *) (E.arena.base() + E.packs_off);
+  const gd_chain *chains = (const gd_chain *) (E.arena.base() + E.chains_off);
+  return gd_launch_exchange(packs, chains, E.n_chain, E.n_cx, E.queue_ptr(),
+                            (int) E.classes.size(), E.done_ptr(), E.stream);
+}
+
 int gd_engine::build_image(GdImage &I, const gd_job &job) const
 {
   gd_options o;
@@ -532,11 +542,8 @@ int gd_engine::remove(int handle)
 
 int gd_engine::enqueue_cycle()
 {
-  const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
-  const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
   GD_HIP_TRY(launch_classes(*this, queue_ptr()));
-  GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, n_cx, queue_ptr(),
-                                (int) classes.size(), done_ptr(), stream));
+  GD_HIP_TRY(launch_exchange(*this));
   return GD_OK;
 }
 
@@ -552,8 +559,6 @@ int gd_engine::cycles(int n)
   GD_HIP_TRY(hipSetDevice(conf.device));
   for (int c = 0; c < n; ++c) {
     if (timing) {
-      const gd_pack *packs = (const gd_pack *) (arena.base() + packs_off);
-      const gd_chain *chains = (const gd_chain *) (arena.base() + chains_off);
       hipEvent_t a, b, x0, x1;
       GD_HIP_TRY(hipEventCreate(&a));
       GD_HIP_TRY(hipEventCreate(&b));
@@ -563,8 +568,7 @@ int gd_engine::cycles(int n)
       GD_HIP_TRY(launch_classes(*this, queue_ptr()));
       GD_HIP_TRY(hipEventRecord(b, stream));
       GD_HIP_TRY(hipEventRecord(x0, stream));
-      GD_HIP_TRY(gd_launch_exchange(packs, chains, n_chain, n_cx, queue_ptr(),
-                                    (int) classes.size(), done_ptr(), stream));
+      GD_HIP_TRY(launch_exchange(*this));
       GD_HIP_TRY(hipEventRecord(x1, stream));
       ev_mc.push_back(a);
       ev_mc.push_back(b);

@@ -383,6 +383,15 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   if (P.lds_bytes + reserve > max_lds)
     return gd_fail(GD_ERR_ARG, "%d ligand atom types need %d B of LDS (device has %d)",
                    P.n_rows, P.lds_bytes + reserve, max_lds);
+  /* split launches: pair list, restraint targets and PSP rows after the KDE
+   * points, beside the wave slot and the partial-sum area of one replica */
+  const int split_reserve = GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES + 64;
+  P.off_pairs = with_kde;
+  P.off_mcs = al16(P.off_pairs + (int) I.kde_pairs.size() * 4);
+  P.off_psp = al16(P.off_mcs + P.pos * P.lna * 16);
+  const int split_bytes = al16(P.off_psp + P.n_rows * pnp_pad * 4);
+  P.split_extra = P.pocket_in_lds && P.kde_in_lds && split_bytes + split_reserve <= max_lds;
+  P.split_lds_bytes = P.split_extra ? split_bytes : P.lds_bytes;
   return GD_OK;
 }
 

@@ -6,8 +6,11 @@
  *                      (montecarlo_d.cu:59-106 with move_d.cu:73-134,
  *                      calcenergy_d.cu:22-374, accept_d.cu:12-52), or the
  *                      initial energies (MonteCarlo_Init_d, :12-55).
- *   gd_exchange_kernel temperature swaps, one lane per chain
- *                      (exchangereplicas_d.cu:7-136).
+ *   gd_mc_split_kernel the same cycle with one replica split over 4, 8 or
+ *                      16 wavefronts (launches with few replicas).
+ *   gd_exchange1_kernel / gd_exchange_kernel temperature swaps, one
+ *                      wavefront per chain, one lane per temperature pair
+ *                      (exchangereplicas_d.cu:7-136), then the advance.
  *   gd_advance_kernel  advances each complex's device step counter after a
  *                      cycle and re-arms the work queue.
  *   gd_score_kernel    energies of caller-given poses (FP32).
@@ -136,6 +139,27 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
   }
 }
 
+/* Split launches (one replica per workgroup) with split_extra: the
+ * (atom, KDE point) pair list, the restraint targets and the PSP rows of
+ * protein conformer prt, read every step by every item, staged once. */
+__device__ __forceinline__ void gd_stage_split_extra(const gd_pack &P, int prt, char *smem)
+{
+  const int n_words = ((P.kde_n_lo + GD_WAVE - 1) / GD_WAVE +
+                       (P.kde_n_hi + GD_WAVE - 1) / GD_WAVE) * GD_WAVE;
+  uint32_t *s_pairs = (uint32_t *) (smem + P.off_pairs);
+  for (int i = threadIdx.x; i < n_words; i += blockDim.x)
+    s_pairs[i] = P.kde_pairs[i];
+  float4 *s_mcs = (float4 *) (smem + P.off_mcs);
+  const int n_mcs = P.pos * P.lna;
+  for (int i = threadIdx.x; i < n_mcs; i += blockDim.x)
+    s_mcs[i] = P.mcs[i];
+  float *s_psp = (float *) (smem + P.off_psp);
+  const float *g_psp = P.psp_pt + (size_t) prt * P.n_rows * P.pnp_pad;
+  const int n_psp = P.n_rows * P.pnp_pad;
+  for (int i = threadIdx.x; i < n_psp; i += blockDim.x)
+    s_psp[i] = g_psp[i];
+}
+
 /* ------------------------------------------------------------------------ */
 /* Energy of one pose (calcenergy_d.cu:22-374), in summation blocks          */
 /* (gd_block_size in gd_canon.h): atom blocks of the pair loop, blocks of    */
@@ -150,7 +174,9 @@ struct gd_blk {
 
 /* Pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) over ligand
  * atoms [l0, l1): lane partials from +0 into blk, the block's restraint sum
- * (uniform) into *ehpc_b, MCC confusion counts (uniform) added to tp/fp/fn. */
+ * (uniform) into *ehpc_b, MCC confusion counts (uniform) added to tp/fp/fn.
+ * XL: the PSP rows are staged in LDS (split launches), else read from HBM. */
+template <bool XL>
 __device__ __forceinline__ void gd_atoms_block(
     const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
     float ay1, float az1, int l0, int l1, gd_blk &blk, float *ehpc_b, int &tp, int &fp,
@@ -188,6 +214,7 @@ __device__ __forceinline__ void gd_atoms_block(
       (void *) (P.pt_pos + (size_t) prt * P.pnp_pad), 0, P.pnp_pad * 16, GD_BUFFER_DWORD3);
   const __amdgpu_buffer_rsrc_t eh_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void *) (P.pt_eh + (size_t) prt * P.pnp_pad), 0, P.pnp_pad * 8, GD_BUFFER_DWORD3);
+  const gd_lds_f32 *psp_lds = (const gd_lds_f32 *) (smem + P.off_psp);
 
   for (int l = l0; l < l1; ++l) {
     const int src = l & (GD_WAVE - 1);
@@ -217,9 +244,13 @@ __device__ __forceinline__ void gd_atoms_block(
         /* the PSP read (chunk base uniform, lane offset fixed) is consumed
          * at the end of the pair; points outside class 2 hold PSP 0 */
         float pv = 0.0f;
-        if (decltype(psp_tag)::value)
-          pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-              psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
+        if (decltype(psp_tag)::value) {
+          if (XL)
+            pv = psp_lds[row * P.pnp_pad + k * GD_WAVE + lane];
+          else
+            pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
+        }
         const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
         if (decltype(hpc_tag)::value && d2 <= 81.0f)
           hacc += gd_hpc_pair(d2, eh.y);
@@ -288,6 +319,7 @@ __device__ __forceinline__ void gd_atoms_block(
  * coordinates come from the lane that holds the atom, the point carries the
  * pair's weight 1 / (segment length) in .w.  Returns this lane's sum over
  * rounds [r0, r1) from +0. */
+template <bool XL>
 __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *smem, float ax0,
                                                float ay0, float az0, float ax1, float ay1,
                                                float az1, int r0, int r1)
@@ -307,8 +339,10 @@ __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *sme
       const int left = hi ? n_hi - (r - rounds_lo) * GD_WAVE : n_lo - r * GD_WAVE;
       /* every lane runs the permutes (padding entries name lane 0 and
        * point 0); only the lanes with a pair keep the sum */
-      const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
-                                                              r * GD_WAVE * 4, 0);
+      const uint32_t e =
+          XL ? ((const __attribute__((address_space(3))) uint32_t *) (smem + P.off_pairs))
+                   [r * GD_WAVE + ulane]
+             : __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u, r * GD_WAVE * 4, 0);
       const int src = (int) (e & 0xffu);
       const float gx = hi ? ax1 : ax0, gy = hi ? ay1 : ay0, gz = hi ? az1 : az0;
       const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
@@ -348,22 +382,33 @@ __device__ __forceinline__ int gd_kde_round_count(const gd_pack &P)
 
 /* Position restraint m (calcenergy_d.cu:240-258): tcc_m sqrt(mean squared
  * distance of the atoms it names), 0 when it names none (uniform). */
-__device__ __forceinline__ float gd_lhm_one(const gd_pack &P, float ax0, float ay0, float az0,
-                                            float ax1, float ay1, float az1, int m)
+template <bool XL>
+__device__ __forceinline__ float gd_lhm_one(const gd_pack &P, const char *smem, float ax0,
+                                            float ay0, float az0, float ax1, float ay1,
+                                            float az1, int m)
 {
+  typedef const __attribute__((address_space(3))) gd_f4 *lds_f4;
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
   float s = 0.0f, c = 0.0f;
   const float4 *tm = P.mcs + (size_t) m * lna;
+  const lds_f4 tl = (lds_f4) (smem + P.off_mcs) + m * lna;
+  auto target = [&](int a) -> float4 {
+    if (XL) {
+      const gd_f4 v = tl[a];
+      return make_float4(v.x, v.y, v.z, v.w);
+    }
+    return tm[a];
+  };
   if (lane < lna) {
-    const float4 t = tm[lane];
+    const float4 t = target(lane);
     if (t.w != 0.0f) {
       s += gd_dist2(ax0 - t.x, ay0 - t.y, az0 - t.z);
       c += 1.0f;
     }
   }
   if (lane + GD_WAVE < lna) {
-    const float4 t = tm[lane + GD_WAVE];
+    const float4 t = target(lane + GD_WAVE);
     if (t.w != 0.0f) {
       s += gd_dist2(ax1 - t.x, ay1 - t.y, az1 - t.z);
       c += 1.0f;
@@ -374,29 +419,31 @@ __device__ __forceinline__ float gd_lhm_one(const gd_pack &P, float ax0, float a
   return c > 0.0f ? gd_uload(P.mcs_tcc + m) * gd_sqrt(s / c) : 0.0f;
 }
 
-/* From the lane totals to the energies: wave sums, KDE normalisation, the
- * restraint log, the distance term, averaging, normalisation and total
- * (calcenergy_d.cu:276-371); out[] (slot order) and *cmcc uniform. */
-__device__ __forceinline__ void gd_energy_tail(const gd_pack &P, const gd_blk &tot,
-                                               float ehpc, float kde_lane, float elhm_sum,
-                                               int tp, int fp, int fn, const float cen[3],
-                                               float out[GD_E_N], float *cmcc)
+/* From the summed terms to the energies: KDE normalisation, the restraint
+ * log, the distance term, averaging, normalisation and total
+ * (calcenergy_d.cu:276-371); out[] (slot order) and *cmcc uniform.  vdw ..
+ * kde are the wave sums of the lane totals. */
+__device__ __forceinline__ void gd_energy_final(const gd_pack &P, float vdw, float ele,
+                                                float pmf, float hdb, float psp, float ehpc,
+                                                float kde, float elhm_sum, int tp, int fp,
+                                                int fn, const float cen[3], float out[GD_E_N],
+                                                float *cmcc)
 {
   const int pos = P.pos;
-  const float ekde = gd_wave_sum(kde_lane) / P.kde3;
+  const float ekde = kde / P.kde3;
   const float elhm = (pos > 0) ? gd_log(elhm_sum / (float) pos) : 0.0f;
   /* distance to the pocket centre (calcenergy_d.cu:276-283) */
   const float edst = gd_sqrt(gd_dist2(cen[0] - P.pocket_center[0],
                                       cen[1] - P.pocket_center[1],
                                       cen[2] - P.pocket_center[2]));
   float sums[GD_F_NTERM];
-  sums[GD_F_VDW] = gd_wave_sum(tot.vdw);
-  sums[GD_F_ELE] = gd_wave_sum(tot.ele);
-  sums[GD_F_PMF] = gd_wave_sum(tot.pmf);
+  sums[GD_F_VDW] = vdw;
+  sums[GD_F_ELE] = ele;
+  sums[GD_F_PMF] = pmf;
   sums[GD_F_HPC] = ehpc;
-  sums[GD_F_HDB] = gd_wave_sum(tot.hdb);
+  sums[GD_F_HDB] = hdb;
   sums[GD_F_DST] = edst;
-  sums[GD_F_PSP] = gd_wave_sum(tot.psp);
+  sums[GD_F_PSP] = psp;
   sums[GD_F_KDE] = ekde;
   sums[GD_F_LHM] = elhm;
   gd_finalize(sums, (float) P.lna, P.a_para, P.b_para, P.w, P.term_order, P.combine, out);
@@ -405,6 +452,17 @@ __device__ __forceinline__ void gd_energy_tail(const gd_pack &P, const gd_blk &t
     const int tn = P.lna * P.pnp - tp - fp - fn;   /* padding points are never contacts */
     *cmcc = gd_mcc((float) tp, (float) fn, (float) fp, (float) tn);
   }
+}
+
+/* The same from the lane totals: their wave sums first. */
+__device__ __forceinline__ void gd_energy_tail(const gd_pack &P, const gd_blk &tot,
+                                               float ehpc, float kde_lane, float elhm_sum,
+                                               int tp, int fp, int fn, const float cen[3],
+                                               float out[GD_E_N], float *cmcc)
+{
+  gd_energy_final(P, gd_wave_sum(tot.vdw), gd_wave_sum(tot.ele), gd_wave_sum(tot.pmf),
+                  gd_wave_sum(tot.hdb), gd_wave_sum(tot.psp), ehpc, gd_wave_sum(kde_lane),
+                  elhm_sum, tp, fp, fn, cen, out, cmcc);
 }
 
 /* One wavefront evaluates every block in order.  On return out[] (slot
@@ -423,8 +481,8 @@ __device__ __forceinline__ void gd_energy_wave(
   for (int l0 = 0; l0 < lna; l0 += bs) {
     gd_blk blk;
     float eb;
-    gd_atoms_block(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, l0, min(l0 + bs, lna), blk, &eb,
-                   tp, fp, fn);
+    gd_atoms_block<false>(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, l0, min(l0 + bs, lna),
+                          blk, &eb, tp, fp, fn);
     tot.vdw += blk.vdw;
     tot.pmf += blk.pmf;
     tot.psp += blk.psp;
@@ -436,11 +494,11 @@ __device__ __forceinline__ void gd_energy_wave(
   const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
   float kde_lane = 0.0f;
   for (int r0 = 0; r0 < n_rounds; r0 += kbs)
-    kde_lane += gd_kde_rounds(P, smem, ax0, ay0, az0, ax1, ay1, az1, r0,
-                              min(r0 + kbs, n_rounds));
+    kde_lane += gd_kde_rounds<false>(P, smem, ax0, ay0, az0, ax1, ay1, az1, r0,
+                                     min(r0 + kbs, n_rounds));
   float elhm = 0.0f;
   for (int m = 0; m < P.pos; ++m)
-    elhm += gd_lhm_one(P, ax0, ay0, az0, ax1, ay1, az1, m);
+    elhm += gd_lhm_one<false>(P, smem, ax0, ay0, az0, ax1, ay1, az1, m);
   gd_energy_tail(P, tot, ehpc, kde_lane, elhm, tp, fp, fn, cen, out, cmcc);
 }
 
@@ -542,9 +600,11 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
 #define GD_SA_LHM (GD_SA_MCC + GD_MAX_BLOCKS * 3)
 #define GD_SA_XYZ (GD_SA_LHM + GD_SPLIT_MAX_POS)   /* the placed atoms, by lane */
 #define GD_SA_CEN (GD_SA_XYZ + 6 * GD_WAVE)       /* the placed centre */
-#define GD_SA_FLOATS (GD_SA_CEN + 4)
+#define GD_SA_SUMS (GD_SA_CEN + 4)                /* the summed terms */
+#define GD_SA_FLOATS (GD_SA_SUMS + 16)
 static_assert(GD_SA_FLOATS * 4 == GD_SPLIT_AREA_BYTES, "split area size");
 
+template <bool XL>
 __device__ __forceinline__ void gd_energy_items(
     const gd_pack &P, const char *smem, gd_lds_f32 *A, int role, int split, int prt,
     float ax0, float ay0, float az0, float ax1, float ay1, float az1)
@@ -562,8 +622,8 @@ __device__ __forceinline__ void gd_energy_items(
       gd_blk blk;
       float eb;
       int tp = 0, fp = 0, fn = 0;
-      gd_atoms_block(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, i * bs, min(i * bs + bs, lna),
-                     blk, &eb, tp, fp, fn);
+      gd_atoms_block<XL>(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, i * bs,
+                         min(i * bs + bs, lna), blk, &eb, tp, fp, fn);
       gd_lds_f32 *pb = A + GD_SA_PART + i * 5 * GD_WAVE + lane;
       pb[0 * GD_WAVE] = blk.vdw;
       pb[1 * GD_WAVE] = blk.pmf;
@@ -579,21 +639,23 @@ __device__ __forceinline__ void gd_energy_items(
     } else if (i < nb + nbk) {
       const int kb = i - nb;
       A[GD_SA_KPART + kb * GD_WAVE + lane] =
-          gd_kde_rounds(P, smem, ax0, ay0, az0, ax1, ay1, az1, kb * kbs,
-                        min(kb * kbs + kbs, n_rounds));
+          gd_kde_rounds<XL>(P, smem, ax0, ay0, az0, ax1, ay1, az1, kb * kbs,
+                            min(kb * kbs + kbs, n_rounds));
     } else {
       const int m = i - nb - nbk;
-      const float v = gd_lhm_one(P, ax0, ay0, az0, ax1, ay1, az1, m);
+      const float v = gd_lhm_one<XL>(P, smem, ax0, ay0, az0, ax1, ay1, az1, m);
       if (lane == 0)
         A[GD_SA_LHM + m] = v;
     }
   }
 }
 
-/* Wavefront 0, after the barrier: the partials added in block order. */
-__device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds_f32 *A,
-                                                  const float cen[3], float out[GD_E_N],
-                                                  float *cmcc)
+/* After the items: the partials added in block order, spread over the
+ * wavefronts -- job j (vdw, pmf, psp, hdb, ele lane totals and their wave
+ * sums; the KDE lane total and its wave sum; the restraint, MCC and
+ * position-restraint sums) on wavefront j % split -- into the sums area. */
+__device__ __forceinline__ void gd_energy_sums(const gd_pack &P, gd_lds_f32 *A, int role,
+                                               int split)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
@@ -602,29 +664,50 @@ __device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds
   const int n_rounds = gd_kde_round_count(P);
   const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
   const int nbk = (n_rounds + kbs - 1) / kbs;
-  gd_blk tot;
-  tot.vdw = 0.0f; tot.pmf = 0.0f; tot.psp = 0.0f; tot.hdb = 0.0f; tot.ele = 0.0f;
-  float ehpc = 0.0f;
-  int tp = 0, fp = 0, fn = 0;
-  for (int b = 0; b < nb; ++b) {
-    const gd_lds_f32 *pb = A + GD_SA_PART + b * 5 * GD_WAVE + lane;
-    tot.vdw += pb[0 * GD_WAVE];
-    tot.pmf += pb[1 * GD_WAVE];
-    tot.psp += pb[2 * GD_WAVE];
-    tot.hdb += pb[3 * GD_WAVE];
-    tot.ele += pb[4 * GD_WAVE];
-    ehpc += A[GD_SA_EHPC + b];
-    tp += __float_as_int(A[GD_SA_MCC + 3 * b + 0]);
-    fp += __float_as_int(A[GD_SA_MCC + 3 * b + 1]);
-    fn += __float_as_int(A[GD_SA_MCC + 3 * b + 2]);
+  for (int j = role; j < 7; j += split) {
+    if (j < 5) {
+      float t = 0.0f;
+      for (int b = 0; b < nb; ++b)
+        t += A[GD_SA_PART + (b * 5 + j) * GD_WAVE + lane];
+      t = gd_wave_sum(t);
+      if (lane == 0)
+        A[GD_SA_SUMS + j] = t;
+    } else if (j == 5) {
+      float t = 0.0f;
+      for (int kb = 0; kb < nbk; ++kb)
+        t += A[GD_SA_KPART + kb * GD_WAVE + lane];
+      t = gd_wave_sum(t);
+      if (lane == 0)
+        A[GD_SA_SUMS + 5] = t;
+    } else if (lane == 0) {
+      float ehpc = 0.0f;
+      int tp = 0, fp = 0, fn = 0;
+      for (int b = 0; b < nb; ++b) {
+        ehpc += A[GD_SA_EHPC + b];
+        tp += __float_as_int(A[GD_SA_MCC + 3 * b + 0]);
+        fp += __float_as_int(A[GD_SA_MCC + 3 * b + 1]);
+        fn += __float_as_int(A[GD_SA_MCC + 3 * b + 2]);
+      }
+      float elhm = 0.0f;
+      for (int m = 0; m < P.pos; ++m)
+        elhm += A[GD_SA_LHM + m];
+      A[GD_SA_SUMS + 6] = ehpc;
+      A[GD_SA_SUMS + 7] = elhm;
+      A[GD_SA_SUMS + 8] = __int_as_float(tp);
+      A[GD_SA_SUMS + 9] = __int_as_float(fp);
+      A[GD_SA_SUMS + 10] = __int_as_float(fn);
+    }
   }
-  float kde_lane = 0.0f;
-  for (int kb = 0; kb < nbk; ++kb)
-    kde_lane += A[GD_SA_KPART + kb * GD_WAVE + lane];
-  float elhm = 0.0f;
-  for (int m = 0; m < P.pos; ++m)
-    elhm += A[GD_SA_LHM + m];
-  gd_energy_tail(P, tot, ehpc, kde_lane, elhm, tp, fp, fn, cen, out, cmcc);
+}
+
+/* Wavefront 0, after the sums: the energies. */
+__device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds_f32 *A,
+                                                  const float cen[3], float out[GD_E_N],
+                                                  float *cmcc)
+{
+  const gd_lds_f32 *S = A + GD_SA_SUMS;
+  gd_energy_final(P, S[0], S[4], S[1], S[3], S[2], S[6], S[5], S[7], __float_as_int(S[8]),
+                  __float_as_int(S[9]), __float_as_int(S[10]), cen, out, cmcc);
 }
 
 /* n_steps MC steps from step s0 of replica r on SPLIT wavefronts (SPLIT = 1:
@@ -632,8 +715,8 @@ __device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds
  * in the replica's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in
  * registers; wavefront 0 keeps the decisions, records and state, and places
  * the moved ligand in the area for the others, which only evaluate items:
- * per step, wavefront 0 moves, barrier, all evaluate, barrier, wavefront 0
- * adds and decides. */
+ * per step, wavefront 0 moves, barrier, all evaluate items, barrier, all add
+ * partials, barrier, wavefront 0 finishes and decides. */
 template <int SPLIT>
 __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *smem,
                                                  gd_lds_f32 *ws, gd_lds_f32 *A, int role,
@@ -663,14 +746,20 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
 
   for (int s = s0; s < s0 + n_steps; ++s) {
     if (SPLIT > 1 && role != 0) {
-      /* evaluate the items of the pose wavefront 0 placed */
+      /* evaluate the items of the pose wavefront 0 placed, then the sums */
       __syncthreads();
       float xyz[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k)
         xyz[k] = A[GD_SA_XYZ + k * GD_WAVE + lane];
-      gd_energy_items(P, smem, A, role, SPLIT, prt, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4],
-                      xyz[5]);
+      if (P.split_extra)
+        gd_energy_items<true>(P, smem, A, role, SPLIT, prt, xyz[0], xyz[1], xyz[2], xyz[3],
+                              xyz[4], xyz[5]);
+      else
+        gd_energy_items<false>(P, smem, A, role, SPLIT, prt, xyz[0], xyz[1], xyz[2], xyz[3],
+                               xyz[4], xyz[5]);
+      __syncthreads();
+      gd_energy_sums(P, A, role, SPLIT);
       __syncthreads();
       continue;
     }
@@ -733,7 +822,12 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
       A[GD_SA_XYZ + 4 * GD_WAVE + lane] = ay1;
       A[GD_SA_XYZ + 5 * GD_WAVE + lane] = az1;
       __syncthreads();
-      gd_energy_items(P, smem, A, role, SPLIT, prt, ax0, ay0, az0, ax1, ay1, az1);
+      if (P.split_extra)
+        gd_energy_items<true>(P, smem, A, role, SPLIT, prt, ax0, ay0, az0, ax1, ay1, az1);
+      else
+        gd_energy_items<false>(P, smem, A, role, SPLIT, prt, ax0, ay0, az0, ax1, ay1, az1);
+      __syncthreads();
+      gd_energy_sums(P, A, role, SPLIT);
       __syncthreads();
       gd_energy_combine(P, A, cen, e_new, &cm_new);
     }
@@ -784,6 +878,148 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
       S.cmcc[r] = ws[GD_WS_CMCC];
     }
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Temperature exchange (exchangereplicas_d.cu:28-78), one wavefront per     */
+/* chain.  The pairs of one sweep are disjoint in temperature, so each       */
+/* pair's decision reads only the two slots it may swap: lane j decides pair */
+/* j, with the draws and arithmetic of the serial walk, and writes the two   */
+/* slots' new temperature indices and acceptance counts.  Then the step      */
+/* counters advance and the work queues are re-armed: by the one workgroup   */
+/* of a launch over few chains, else by the last workgroup to finish.        */
+/* ------------------------------------------------------------------------ */
+#define GD_MAXTMP_DEV 64
+#define GD_XWAVES 4        /* wavefronts per workgroup, many-chain form */
+#define GD_X1_WAVES 16     /* wavefronts of the one-workgroup form */
+#define GD_X1_MAX_CHAINS 32
+/* LDS words one exchanging wavefront uses */
+#define GD_X_WORDS (2 * GD_MAXTMP_DEV)
+
+__device__ __forceinline__ void gd_advance_all(const gd_pack *__restrict__ packs, int n_cx,
+                                               int *queue, int n_queue)
+{
+  const int n = n_cx > n_queue ? n_cx : n_queue;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (i < n_queue)
+      queue[i] = 0;
+    if (i < n_cx) {
+      const gd_pack &P = packs[i];
+      const int s0 = P.step[0];
+      const int k = gd_cycle_steps(P, s0);
+      if (k > 0) {
+        P.step[0] = s0 + k;
+        P.step[1] = P.step[1] + 1;
+      }
+    }
+  }
+}
+
+/* Chain g (none if g >= n_chain) by this wavefront, LDS scratch sx of
+ * GD_X_WORDS words; every wavefront of the workgroup calls it (it holds two
+ * workgroup barriers). */
+__device__ __forceinline__ void gd_exchange_wave(const gd_pack *__restrict__ packs,
+                                                 const gd_chain *__restrict__ chains,
+                                                 int n_chain, int g, int *sx)
+{
+  int *s_orders = sx;                                 /* temperature -> slot */
+  float *s_ener = (float *) (sx + GD_MAXTMP_DEV);     /* slot -> total energy */
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  int active = 0, n_tmp = 0, n_lig = 0, base = 0, x = 0, c = 0;
+  const gd_pack *P = nullptr;
+  if (g < n_chain) {
+    const gd_chain ch = chains[g];
+    P = &packs[ch.cx];
+    n_tmp = P->n_tmp;
+    active = P->exchange && n_tmp >= 2 && gd_cycle_steps(*P, P->step[0]) > 0;
+    if (active) {
+      x = P->step[1];
+      c = ch.c;
+      n_lig = P->n_lig;
+      const int p = c / n_lig;
+      base = n_tmp * n_lig * p + (c - p * n_lig);
+      if (lane < n_tmp) {
+        const int a = base + n_lig * lane;
+        s_ener[lane] = P->S.e[a * 10 + GD_E_TOTAL];
+        s_orders[P->S.idx_tmp[a]] = lane;
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+    /* cycle x is launch x % lpw of its steps_per_dump window; the mode
+     * alternates from 0 in every window (gpusingle.cu:14-18) */
+    const int mode = (x % P->lpw) & 1;
+    const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && mode == 1 ? 1 : 0);
+    const int j = lane;
+    if (j < maxpair) {
+      const gd_u32x4 u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE,
+                                           (uint32_t) (j >> 2), P->seed, P->cid);
+      const int q = j & 3;
+      const uint32_t uw = q == 0 ? u4.v[0] : q == 1 ? u4.v[1] : q == 2 ? u4.v[2] : u4.v[3];
+      const int i1 = 2 * j + mode;
+      const int i2 = i1 + 1;
+      const int o1 = s_orders[i1];
+      const int o2 = s_orders[i2];
+      const float delta = (P->minus_beta[i1] - P->minus_beta[i2]) * (s_ener[o2] - s_ener[o1]);
+      if (gd_exp(delta) > gd_u01(uw)) {
+        const gd_state S = P->S;
+        const int a1 = base + n_lig * o1;
+        const int a2 = base + n_lig * o2;
+        S.idx_tmp[a1] = i2;
+        S.idx_tmp[a2] = i1;
+        S.acs_x[a1] += 1;
+        S.acs_x[a2] += 1;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+/* Exchange of at most GD_X1_MAX_CHAINS chains by one workgroup, its
+ * wavefronts taking chains in turn, then the advance. */
+__global__ void __launch_bounds__(GD_WAVE * GD_X1_WAVES)
+gd_exchange1_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict__ chains,
+                    int n_chain, int n_cx, int *queue, int n_queue)
+{
+  __shared__ int s_x[GD_X1_WAVES * GD_X_WORDS];
+  const int nw = (int) (blockDim.x / GD_WAVE);
+  const int wv = threadIdx.x / GD_WAVE;
+  for (int g0 = 0; g0 < n_chain; g0 += nw)
+    gd_exchange_wave(packs, chains, n_chain, g0 + wv, s_x + wv * GD_X_WORDS);
+  gd_advance_all(packs, n_cx, queue, n_queue);
+}
+
+/* Exchange of chains GD_XWAVES per workgroup; the last workgroup to finish
+ * advances.  Its count orders only the other workgroups' reads of the step
+ * counters (all consumed before they count) before the advance's writes:
+ * nothing else written here is read here, so no fence is taken. */
+__global__ void __launch_bounds__(GD_WAVE * GD_XWAVES)
+gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict__ chains,
+                   int n_chain, int n_cx, int *queue, int n_queue, int *done)
+{
+  __shared__ int s_x[GD_XWAVES * GD_X_WORDS];
+  __shared__ int s_last;
+  const int wv = threadIdx.x / GD_WAVE;
+  gd_exchange_wave(packs, chains, n_chain, blockIdx.x * GD_XWAVES + wv, s_x + wv * GD_X_WORDS);
+  if (threadIdx.x == 0)
+    s_last = atomicAdd(done, 1) == (int) gridDim.x - 1;
+  __syncthreads();
+  if (s_last) {
+    gd_advance_all(packs, n_cx, queue, n_queue);
+    if (threadIdx.x == 0)
+      atomicExch(done, 0);
+  }
+}
+
+/* After MC (no temperature chains to exchange): step[0] += steps of this
+ * cycle, step[1] += 1; threads 0 .. n_queue-1 re-arm the persistent work
+ * queues. */
+__global__ void __launch_bounds__(256)
+gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue, int n_queue)
+{
+  if (blockIdx.x == 0)
+    gd_advance_all(packs, n_cx, queue, n_queue);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -889,6 +1125,8 @@ gd_mc_split_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict_
       if (task.cx != staged_cx || task.prt != staged_prt) {
         __syncthreads();
         gd_stage(P, task.prt, smem);
+        if (P.split_extra)
+          gd_stage_split_extra(P, task.prt, smem);
         __syncthreads();
         staged_cx = task.cx;
         staged_prt = task.prt;
@@ -900,125 +1138,6 @@ gd_mc_split_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict_
   }
 }
 
-/* ------------------------------------------------------------------------ */
-/* Temperature exchange (exchangereplicas_d.cu:28-78), one wavefront per     */
-/* chain: its lanes gather the chain's temperature indices and energies into */
-/* LDS in parallel, lane 0 walks the temperature pairs, the lanes write the  */
-/* indices back.  The last workgroup to finish then advances every complex's */
-/* step counters and re-arms the work queues (what gd_advance_kernel does    */
-/* alone), so a cycle is two launches.                                       */
-/* ------------------------------------------------------------------------ */
-#define GD_MAXTMP_DEV 64
-#define GD_XWAVES 4
-
-__device__ __forceinline__ void gd_advance_all(const gd_pack *__restrict__ packs, int n_cx,
-                                               int *queue, int n_queue)
-{
-  const int n = n_cx > n_queue ? n_cx : n_queue;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    if (i < n_queue)
-      queue[i] = 0;
-    if (i < n_cx) {
-      const gd_pack &P = packs[i];
-      const int s0 = P.step[0];
-      const int k = gd_cycle_steps(P, s0);
-      if (k > 0) {
-        P.step[0] = s0 + k;
-        P.step[1] = P.step[1] + 1;
-      }
-    }
-  }
-}
-
-__global__ void __launch_bounds__(GD_WAVE * GD_XWAVES)
-gd_exchange_kernel(const gd_pack *__restrict__ packs, const gd_chain *__restrict__ chains,
-                   int n_chain, int n_cx, int *queue, int n_queue, int *done)
-{
-  __shared__ int s_temps[GD_XWAVES][GD_MAXTMP_DEV];
-  __shared__ int s_orders[GD_XWAVES][GD_MAXTMP_DEV];
-  __shared__ float s_ener[GD_XWAVES][GD_MAXTMP_DEV];
-  __shared__ int s_last;
-  const int wv = threadIdx.x / GD_WAVE;
-  const int lane = threadIdx.x & (GD_WAVE - 1);
-  const int g = blockIdx.x * GD_XWAVES + wv;
-  int active = 0, n_tmp = 0, n_lig = 0, base = 0, x = 0, c = 0;
-  const gd_pack *P = nullptr;
-  if (g < n_chain) {
-    const gd_chain ch = chains[g];
-    P = &packs[ch.cx];
-    n_tmp = P->n_tmp;
-    active = P->exchange && n_tmp >= 2 && gd_cycle_steps(*P, P->step[0]) > 0;
-    if (active) {
-      x = P->step[1];
-      c = ch.c;
-      n_lig = P->n_lig;
-      const int p = c / n_lig;
-      base = n_tmp * n_lig * p + (c - p * n_lig);
-      if (lane < n_tmp) {
-        const int a = base + n_lig * lane;
-        const int t = P->S.idx_tmp[a];
-        s_temps[wv][lane] = t;
-        s_ener[wv][lane] = P->S.e[a * 10 + GD_E_TOTAL];
-        s_orders[wv][t] = lane;
-      }
-    }
-  }
-  __syncthreads();
-  if (active && lane == 0) {
-    const gd_state S = P->S;
-    int *temps = s_temps[wv];
-    const int *orders = s_orders[wv];
-    const float *ener = s_ener[wv];
-    /* cycle x is launch x % lpw of its steps_per_dump window; the mode
-     * alternates from 0 in every window (gpusingle.cu:14-18) */
-    const int mode = (x % P->lpw) & 1;
-    const int maxpair = n_tmp / 2 - ((n_tmp % 2 == 0) && mode == 1 ? 1 : 0);
-    gd_u32x4 u4;
-    for (int j = 0; j < maxpair; ++j) {
-      if ((j & 3) == 0)
-        u4 = gd_philox4x32_10((uint32_t) c, (uint32_t) x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2),
-                              P->seed, P->cid);
-      const int i1 = 2 * j + mode;
-      const int i2 = i1 + 1;
-      const int o1 = orders[i1];
-      const int o2 = orders[i2];
-      const float delta = (P->minus_beta[i1] - P->minus_beta[i2]) * (ener[o2] - ener[o1]);
-      if (gd_exp(delta) > gd_u01(u4.v[j & 3])) {
-        const int tt = temps[o1];
-        temps[o1] = temps[o2];
-        temps[o2] = tt;
-        S.acs_x[base + n_lig * o1] += 1;
-        S.acs_x[base + n_lig * o2] += 1;
-      }
-    }
-  }
-  __syncthreads();
-  if (active && lane < n_tmp)
-    P->S.idx_tmp[base + n_lig * lane] = s_temps[wv][lane];
-  /* the last workgroup advances the counters every workgroup has read */
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(done, 1) == (int) gridDim.x - 1;
-  }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
-    gd_advance_all(packs, n_cx, queue, n_queue);
-    if (threadIdx.x == 0)
-      *done = 0;
-  }
-}
-
-/* After MC (no temperature chains to exchange): step[0] += steps of this
- * cycle, step[1] += 1; threads 0 .. n_queue-1 re-arm the persistent work
- * queues. */
-__global__ void __launch_bounds__(256)
-gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue, int n_queue)
-{
-  if (blockIdx.x == 0)
-    gd_advance_all(packs, n_cx, queue, n_queue);
-}
 
 /* ------------------------------------------------------------------------ */
 /* Energies of given poses (FP32).                                           */
@@ -1339,6 +1458,12 @@ hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int 
 {
   if (n_chain <= 0)
     return gd_launch_advance(packs, n_cx, queue, n_queue, st);
+  if (n_chain <= GD_X1_MAX_CHAINS) {
+    const int nw = n_chain < GD_X1_WAVES ? n_chain : GD_X1_WAVES;
+    hipLaunchKernelGGL(gd_exchange1_kernel, dim3(1), dim3(GD_WAVE * nw), 0, st, packs, chains,
+                       n_chain, n_cx, queue, n_queue);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(gd_exchange_kernel, dim3((n_chain + GD_XWAVES - 1) / GD_XWAVES),
                      dim3(GD_WAVE * GD_XWAVES), 0, st, packs, chains, n_chain, n_cx, queue,
                      n_queue, done);

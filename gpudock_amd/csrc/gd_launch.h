@@ -19,8 +19,9 @@ typedef struct gd_launch_cfg {
 
 hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,
                         int n_tasks, int *queue, int grid, int mode, hipStream_t st);
-/* exchange over chains[0 .. n_chain), then (the last workgroup) the advance
- * of gd_launch_advance; *done is a zeroed counter the launch leaves zeroed.
+/* exchange over chains[0 .. n_chain), then the advance of
+ * gd_launch_advance: by one workgroup for few chains, else by the last
+ * workgroup to finish (*done: a zeroed counter the launch leaves zeroed).
  * With no chains, the advance alone. */
 hipError_t gd_launch_exchange(const gd_pack *packs, const gd_chain *chains, int n_chain,
                               int n_cx, int *queue, int n_queue, int *done, hipStream_t st);

@@ -166,6 +166,19 @@ def test_run_variants(n_tmp, exchange, away, term_order, steps, spe):
     _compare_runs(g, r)
 
 
+@pytest.mark.parametrize("n_lig,n_prt,n_tmp", [(4, 8, 7), (8, 5, 4), (17, 3, 2)])
+def test_exchange_many_chains(n_lig, n_prt, n_tmp):
+    """32 chains are exchanged by one workgroup, more by many (the last to
+    finish advancing the counters): both forms make the replay's decisions."""
+    cx, temps, mc, cfg = synth.make_config("c0", lna=8, pnp=60, pnk=40, n_lig=n_lig,
+                                           n_prt=n_prt, n_tmp=n_tmp, steps=80, spe=4)
+    opt = remc.default_options(seed=21, complex_id=3)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    assert g["acs_exchg"].sum() > 0
+
+
 @pytest.mark.parametrize("steps, spd, spe, n_tmp", [
     (100, 50, 10, 4),    # five launches per window: the mode restarts in each window
     (60, 25, 10, 5),     # spe does not divide the window: 3 windows x 3 launches, 90 steps
