@@ -459,8 +459,12 @@ def main():
     cx, temps, mc, cfg = synth.make_config(args.config, seed=args.seed)
     n_rep = cfg["n_lig"] * cfg["n_prt"] * cfg["n_tmp"]
     spe = cfg["spe"]
-    # every warm-up and timed cycle must do a full spe steps
-    mc.steps_total = max(mc.steps_total, (args.warmup + args.steps) * spe)
+    # kernel durations come from a separate event-timed run after the
+    # throughput run (events around every launch cost the graph replay its
+    # edge on short cycles)
+    t_cycles = max(5, min(args.steps, 50))
+    # every warm-up, timed and event-timed cycle must do a full spe steps
+    mc.steps_total = max(mc.steps_total, (args.warmup + args.steps + t_cycles) * spe)
     f_pose, m_kde = flops_per_pose(cx)
     opt = remc.default_options(seed=args.seed, mcc=int(args.mcc))
     job = remc.JobSpec(cx, temps, mc, opt)
@@ -470,12 +474,14 @@ def main():
     if args.warmup:
         eng.cycles(args.warmup)
     eng.sync()
-    eng.set_timing(True)
-    mc0, n0, x0, nx0 = eng.timing()
     t0 = time.perf_counter()
     eng.cycles(args.steps)
     eng.sync()
     elapsed = time.perf_counter() - t0
+    eng.set_timing(True)
+    mc0, n0, x0, nx0 = eng.timing()
+    eng.cycles(t_cycles)
+    eng.sync()
     mc_ms, mc_n, x_ms, x_n = eng.timing()
     mc_ms, mc_n, x_ms, x_n = mc_ms - mc0, mc_n - n0, x_ms - x0, x_n - nx0
     layout = eng.layout()
@@ -530,7 +536,8 @@ def main():
             "parallelism": "single GPU",
         },
         "pair_interactions_per_s": value * cfg["lna"] * cfg["pnp"],
-        "mc_acceptance": float(res["acs_mc"].sum()) / max(1, n_rep * spe * (args.steps + args.warmup)),
+        "mc_acceptance": float(res["acs_mc"].sum()) / max(1, n_rep * spe * (args.steps + args.warmup
+                                                                           + t_cycles)),
         "roofline": {
             "bound": "valu",
             "kernel": "gd_mc_kernel",

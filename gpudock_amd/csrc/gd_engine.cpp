@@ -215,9 +215,14 @@ int gd_engine::choose_split() const
 {
   long long reps = 0;
   for (auto *s : slots) {
-    if (s->img->P.pos > GD_SPLIT_MAX_POS)
+    const gd_pack &P = s->img->P;
+    /* restraints beyond the area's slots, or staging that leaves no room
+     * for the area: one wavefront per replica */
+    if (P.pos > GD_SPLIT_MAX_POS ||
+        std::max(P.lds_bytes, P.split_lds_bytes) + GD_WSLOT_FLOATS * 4 +
+                (int) GD_SPLIT_AREA_BYTES > max_lds)
       return 1;
-    reps += s->img->P.n_rep;
+    reps += P.n_rep;
   }
   if (lc.split == 1 || lc.split == 4 || lc.split == 8 || lc.split == 16)
     return lc.split;

@@ -423,19 +423,22 @@ __device__ __forceinline__ float gd_lhm_one(const gd_pack &P, const char *smem, 
  * log, the distance term, averaging, normalisation and total
  * (calcenergy_d.cu:276-371); out[] (slot order) and *cmcc uniform.  vdw ..
  * kde are the wave sums of the lane totals. */
-__device__ __forceinline__ void gd_energy_final(const gd_pack &P, float vdw, float ele,
-                                                float pmf, float hdb, float psp, float ehpc,
-                                                float kde, float elhm_sum, int tp, int fp,
-                                                int fn, const float cen[3], float out[GD_E_N],
-                                                float *cmcc)
+/* distance of the placed centre to the pocket centre (calcenergy_d.cu:276-283) */
+__device__ __forceinline__ float gd_edst(const gd_pack &P, const float cen[3])
+{
+  return gd_sqrt(gd_dist2(cen[0] - P.pocket_center[0], cen[1] - P.pocket_center[1],
+                          cen[2] - P.pocket_center[2]));
+}
+
+__device__ __forceinline__ void gd_energy_final_d(const gd_pack &P, float vdw, float ele,
+                                                  float pmf, float hdb, float psp, float ehpc,
+                                                  float kde, float elhm_sum, int tp, int fp,
+                                                  int fn, float edst, float out[GD_E_N],
+                                                  float *cmcc)
 {
   const int pos = P.pos;
   const float ekde = kde / P.kde3;
   const float elhm = (pos > 0) ? gd_log(elhm_sum / (float) pos) : 0.0f;
-  /* distance to the pocket centre (calcenergy_d.cu:276-283) */
-  const float edst = gd_sqrt(gd_dist2(cen[0] - P.pocket_center[0],
-                                      cen[1] - P.pocket_center[1],
-                                      cen[2] - P.pocket_center[2]));
   float sums[GD_F_NTERM];
   sums[GD_F_VDW] = vdw;
   sums[GD_F_ELE] = ele;
@@ -452,6 +455,16 @@ __device__ __forceinline__ void gd_energy_final(const gd_pack &P, float vdw, flo
     const int tn = P.lna * P.pnp - tp - fp - fn;   /* padding points are never contacts */
     *cmcc = gd_mcc((float) tp, (float) fn, (float) fp, (float) tn);
   }
+}
+
+__device__ __forceinline__ void gd_energy_final(const gd_pack &P, float vdw, float ele,
+                                                float pmf, float hdb, float psp, float ehpc,
+                                                float kde, float elhm_sum, int tp, int fp,
+                                                int fn, const float cen[3], float out[GD_E_N],
+                                                float *cmcc)
+{
+  gd_energy_final_d(P, vdw, ele, pmf, hdb, psp, ehpc, kde, elhm_sum, tp, fp, fn,
+                    gd_edst(P, cen), out, cmcc);
 }
 
 /* The same from the lane totals: their wave sums first. */
@@ -588,36 +601,73 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
 /* One replica on SPLIT wavefronts.  Its area in LDS (floats): lane partials */
 /* of every atom block (5 terms x 64 lanes) and of every block of KDE rounds */
 /* (64 lanes); per atom block the restraint sum and the MCC counts; per      */
-/* position restraint its value.  Wavefront `role` evaluates items role,     */
-/* role + SPLIT, ... of the list (atom blocks, KDE blocks, restraints); after */
-/* a barrier wavefront 0 adds the partials in block order -- the order one   */
-/* wavefront keeps alone -- so the energies have the same bits.              */
+/* position restraint its value; three pose buffers (placed atoms, their   */
+/* movematrix and centre); the summed terms; an item counter and the index  */
+/* of the pose being evaluated.  The wavefronts take items (atom blocks,     */
+/* then KDE blocks, then  */
+/* restraints) from the counter until none is left, so one that finishes    */
+/* early takes more.  Each item writes its own slots and the sums read them  */
+/* in block order -- the order one wavefront keeps alone -- so the energies  */
+/* have the same bits whichever wavefront evaluated which item.              */
 /* ------------------------------------------------------------------------ */
 #define GD_SA_PART 0
 #define GD_SA_KPART (GD_MAX_BLOCKS * 5 * GD_WAVE)
 #define GD_SA_EHPC (GD_SA_KPART + GD_MAX_BLOCKS * GD_WAVE)
 #define GD_SA_MCC (GD_SA_EHPC + GD_MAX_BLOCKS)
 #define GD_SA_LHM (GD_SA_MCC + GD_MAX_BLOCKS * 3)
-#define GD_SA_XYZ (GD_SA_LHM + GD_SPLIT_MAX_POS)   /* the placed atoms, by lane */
-#define GD_SA_CEN (GD_SA_XYZ + 6 * GD_WAVE)       /* the placed centre */
-#define GD_SA_SUMS (GD_SA_CEN + 4)                /* the summed terms */
+/* pose buffer i: atoms k * 64 + lane (x0 y0 z0 x1 y1 z1), movematrix at
+ * GD_PB_M, centre at GD_PB_CEN */
+#define GD_PB_M (6 * GD_WAVE)
+#define GD_PB_CEN (GD_PB_M + 6)
+#define GD_PB_FLOATS (6 * GD_WAVE + 16)
+#define GD_SA_POSE (GD_SA_LHM + GD_SPLIT_MAX_POS)
+#define GD_SA_SUMS (GD_SA_POSE + 3 * GD_PB_FLOATS)  /* the summed terms */
+#define GD_SA_SEL (GD_SA_SUMS + 14)               /* the pose buffer of this step (int) */
+#define GD_SA_CTR (GD_SA_SUMS + 15)               /* the item counter (int) */
 #define GD_SA_FLOATS (GD_SA_SUMS + 16)
 static_assert(GD_SA_FLOATS * 4 == GD_SPLIT_AREA_BYTES, "split area size");
 
+typedef __attribute__((address_space(3))) int gd_lds_i32;
+
+/* Item counts of a complex, read once per cycle (not after every barrier). */
+struct gd_split_sizes {
+  int bs, nb;          /* atom block size and count */
+  int n_rounds, kbs, nbk;   /* KDE rounds, rounds per block, blocks */
+  int pos, n_items;
+};
+
+__device__ __forceinline__ gd_split_sizes gd_split_sizes_of(const gd_pack &P)
+{
+  gd_split_sizes z;
+  z.bs = gd_block_size(P.lna);
+  z.nb = (P.lna + z.bs - 1) / z.bs;
+  z.n_rounds = gd_kde_round_count(P);
+  z.kbs = gd_block_size(z.n_rounds > 0 ? z.n_rounds : 1);
+  z.nbk = (z.n_rounds + z.kbs - 1) / z.kbs;
+  z.pos = P.pos;
+  z.n_items = z.nb + z.nbk + z.pos;
+  return z;
+}
+
 template <bool XL>
 __device__ __forceinline__ void gd_energy_items(
-    const gd_pack &P, const char *smem, gd_lds_f32 *A, int role, int split, int prt,
+    const gd_pack &P, const char *smem, gd_lds_f32 *A, int prt, const gd_split_sizes &Z,
     float ax0, float ay0, float az0, float ax1, float ay1, float az1)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
-  const int bs = gd_block_size(lna);
-  const int nb = (lna + bs - 1) / bs;
-  const int n_rounds = gd_kde_round_count(P);
-  const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
-  const int nbk = (n_rounds + kbs - 1) / kbs;
-  const int n_items = nb + nbk + P.pos;
-  for (int i = role; i < n_items; i += split) {
+  const int bs = Z.bs, nb = Z.nb, n_rounds = Z.n_rounds, kbs = Z.kbs, nbk = Z.nbk;
+  const int n_items = Z.n_items;
+  gd_lds_i32 *ctr = (gd_lds_i32 *) (A + GD_SA_CTR);
+  for (;;) {
+    /* every lane adds 1 (one LDS atomic of 64 for the wavefront) and the
+     * first lane's old value / 64 is the item: no lane-divergent branch
+     * feeds the loop */
+    const int got = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int i = __builtin_amdgcn_readfirstlane(got) >> 6;
+    if (i >= n_items)
+      break;
     if (i < nb) {
       gd_blk blk;
       float eb;
@@ -653,17 +703,14 @@ __device__ __forceinline__ void gd_energy_items(
 /* After the items: the partials added in block order, spread over the
  * wavefronts -- job j (vdw, pmf, psp, hdb, ele lane totals and their wave
  * sums; the KDE lane total and its wave sum; the restraint, MCC and
- * position-restraint sums) on wavefront j % split -- into the sums area. */
-__device__ __forceinline__ void gd_energy_sums(const gd_pack &P, gd_lds_f32 *A, int role,
-                                               int split)
+ * position-restraint sums) on wavefront j % split -- into the sums area.
+ * The last job reads block b's values in lane b and adds them in block
+ * order from the lanes (the MCC counts are integers: any order). */
+__device__ __forceinline__ void gd_energy_sums(gd_lds_f32 *A, int role, int split,
+                                               const gd_split_sizes &Z)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
-  const int lna = P.lna;
-  const int bs = gd_block_size(lna);
-  const int nb = (lna + bs - 1) / bs;
-  const int n_rounds = gd_kde_round_count(P);
-  const int kbs = gd_block_size(n_rounds > 0 ? n_rounds : 1);
-  const int nbk = (n_rounds + kbs - 1) / kbs;
+  const int nb = Z.nb, nbk = Z.nbk;
   for (int j = role; j < 7; j += split) {
     if (j < 5) {
       float t = 0.0f;
@@ -679,64 +726,234 @@ __device__ __forceinline__ void gd_energy_sums(const gd_pack &P, gd_lds_f32 *A, 
       t = gd_wave_sum(t);
       if (lane == 0)
         A[GD_SA_SUMS + 5] = t;
-    } else if (lane == 0) {
+    } else {
+      const int pos = Z.pos;
+      const bool in_b = lane < nb;
+      const float ev = in_b ? A[GD_SA_EHPC + lane] : 0.0f;
+      const int tv = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 0]) : 0;
+      const int fv = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 1]) : 0;
+      const int nv = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 2]) : 0;
+      const float lv = lane < pos ? A[GD_SA_LHM + lane] : 0.0f;
       float ehpc = 0.0f;
       int tp = 0, fp = 0, fn = 0;
       for (int b = 0; b < nb; ++b) {
-        ehpc += A[GD_SA_EHPC + b];
-        tp += __float_as_int(A[GD_SA_MCC + 3 * b + 0]);
-        fp += __float_as_int(A[GD_SA_MCC + 3 * b + 1]);
-        fn += __float_as_int(A[GD_SA_MCC + 3 * b + 2]);
+        ehpc += gd_bcast(ev, b);
+        tp += __builtin_amdgcn_readlane(tv, b);
+        fp += __builtin_amdgcn_readlane(fv, b);
+        fn += __builtin_amdgcn_readlane(nv, b);
       }
       float elhm = 0.0f;
-      for (int m = 0; m < P.pos; ++m)
-        elhm += A[GD_SA_LHM + m];
-      A[GD_SA_SUMS + 6] = ehpc;
-      A[GD_SA_SUMS + 7] = elhm;
-      A[GD_SA_SUMS + 8] = __int_as_float(tp);
-      A[GD_SA_SUMS + 9] = __int_as_float(fp);
-      A[GD_SA_SUMS + 10] = __int_as_float(fn);
+      for (int m = 0; m < pos; ++m)
+        elhm += gd_bcast(lv, m);
+      if (lane == 0) {
+        A[GD_SA_SUMS + 6] = ehpc;
+        A[GD_SA_SUMS + 7] = elhm;
+        A[GD_SA_SUMS + 8] = __int_as_float(tp);
+        A[GD_SA_SUMS + 9] = __int_as_float(fp);
+        A[GD_SA_SUMS + 10] = __int_as_float(fn);
+      }
     }
   }
 }
 
-/* Wavefront 0, after the sums: the energies. */
-__device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds_f32 *A,
-                                                  const float cen[3], float out[GD_E_N],
-                                                  float *cmcc)
+/* What the energies need of a complex, read once per cycle: lane k < 10
+ * holds a_para[k], b_para[k], w[k]. */
+struct gd_final_consts {
+  float a_l, b_l, w_l;
+  float lna_f, kde3;
+  int pos, term_order, combine;
+};
+
+__device__ __forceinline__ gd_final_consts gd_final_consts_of(const gd_pack &P)
 {
-  const gd_lds_f32 *S = A + GD_SA_SUMS;
-  gd_energy_final(P, S[0], S[4], S[1], S[3], S[2], S[6], S[5], S[7], __float_as_int(S[8]),
-                  __float_as_int(S[9]), __float_as_int(S[10]), cen, out, cmcc);
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  gd_final_consts F;
+  F.a_l = lane < 10 ? P.a_para[lane] : 0.0f;
+  F.b_l = lane < 10 ? P.b_para[lane] : 0.0f;
+  F.w_l = lane < 10 ? P.w[lane] : 0.0f;
+  F.lna_f = (float) P.lna;
+  F.kde3 = P.kde3;
+  F.pos = P.pos;
+  F.term_order = P.term_order;
+  F.combine = P.combine;
+  return F;
 }
 
-/* n_steps MC steps from step s0 of replica r on SPLIT wavefronts (SPLIT = 1:
- * one wavefront, no area, no barrier).  The movematrices and energies live
- * in the replica's LDS slot ws[GD_WSLOT_FLOATS]; only totals stay in
- * registers; wavefront 0 keeps the decisions, records and state, and places
- * the moved ligand in the area for the others, which only evaluate items:
- * per step, wavefront 0 moves, barrier, all evaluate items, barrier, all add
- * partials, barrier, wavefront 0 finishes and decides. */
-template <int SPLIT>
+/* Wavefront 0, after the sums: the energies (edst: the distance term of the
+ * placed centre).  The weighted-sum total (combine 0) with one term per
+ * lane: lane L takes parameter L and the term it applies to -- file-order
+ * term L (term_order 0) or the term of slot L (term_order 1) -- and forms
+ * it with the operations of gd_energy_final_d / gd_finalize (one division
+ * sequence serves every term); the total adds w x value over the lanes in
+ * parameter order, as gd_finalize does.  Bitwise equal to
+ * gd_energy_final_d, which the Bayesian total still uses. */
+__device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds_f32 *A,
+                                                  const gd_final_consts &F, float edst,
+                                                  float out[GD_E_N], float *cmcc)
+{
+  const gd_lds_f32 *S = A + GD_SA_SUMS;
+  if (F.combine != 0) {
+    gd_energy_final_d(P, S[0], S[4], S[1], S[3], S[2], S[6], S[5], S[7],
+                      __float_as_int(S[8]), __float_as_int(S[9]), __float_as_int(S[10]), edst,
+                      out, cmcc);
+    return;
+  }
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const int L = lane < GD_F_NTERM ? lane : 0;
+  /* file-order term of each slot (slot i holds file term s2f[i]), 4 bits each */
+  const unsigned long long s2f = 0x587346210ull;
+  /* the sums-area word of each file-order term (term 5 is edst) */
+  const unsigned long long fword = 0x752036140ull;
+  const int f = F.term_order ? (int) ((s2f >> (4 * L)) & 0xfu) : L;
+  float x = S[(int) ((fword >> (4 * f)) & 0xfu)];
+  if (f == GD_F_KDE)
+    x = x / F.kde3;
+  float y = x / F.lna_f;
+  if (f == GD_F_HDB)
+    y = y / 0x1.40d932p+1f * -1.0f;   /* sqrt(2 pi) */
+  if (f == GD_F_DST)
+    y = edst;
+  if (f == GD_F_LHM)
+    y = F.pos > 0 ? gd_log(x / (float) F.pos) : 0.0f;
+  const float v = gd_fma(F.a_l, y, F.b_l);
+  float total = 0.0f;
+#pragma unroll
+  for (int k = 0; k < GD_F_NTERM; ++k)
+    total = gd_fma(gd_bcast(F.w_l, k), gd_bcast(v, k), total);
+#pragma unroll
+  for (int i = 0; i < GD_F_NTERM; ++i) {
+    /* the lane that formed slot i */
+    const int src = F.term_order ? i : (int) ((s2f >> (4 * i)) & 0xfu);
+    out[i] = gd_bcast(v, src);
+  }
+  out[GD_E_TOTAL] = total;
+  *cmcc = 0.0f;
+  if (P.mcc_ref) {
+    const int tp = __float_as_int(S[8]), fp = __float_as_int(S[9]), fn = __float_as_int(S[10]);
+    const int tn = P.lna * P.pnp - tp - fp - fn;
+    *cmcc = gd_mcc((float) tp, (float) fn, (float) fp, (float) tn);
+  }
+}
+
+/* The draws of step s of replica r (move_d.cu:73-84, accept_d.cu:20-24): the
+ * movematrix displacement d[6] and the acceptance draw. */
+__device__ __forceinline__ void gd_draws(const gd_pack &P, int r, int s, uint32_t seed,
+                                         uint32_t cid, float d[6], float *u_acc)
+{
+  const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
+  const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
+  d[0] = P.move_scale[0] * (2.0f * gd_u01(ra.v[0]) - 1.0f);
+  d[1] = P.move_scale[1] * (2.0f * gd_u01(ra.v[1]) - 1.0f);
+  d[2] = P.move_scale[2] * (2.0f * gd_u01(ra.v[2]) - 1.0f);
+  d[3] = P.move_scale[3] * (2.0f * gd_u01(ra.v[3]) - 1.0f);
+  d[4] = P.move_scale[4] * (2.0f * gd_u01(rb.v[0]) - 1.0f);
+  d[5] = P.move_scale[5] * (2.0f * gd_u01(rb.v[1]) - 1.0f);
+  *u_acc = gd_u01(rb.v[2]);
+}
+
+/* RecordLigand_d (util_d.cu:72-94): the state before step s, one word per
+ * lane. */
+__device__ __forceinline__ void gd_record(const gd_pack &P, const gd_lds_f32 *ws, int r,
+                                          int itmp, int s)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  uint32_t *rec = P.rec + ((size_t) r * P.rec_per_window +
+                           (s % P.rec_window) / P.rec_stride) * GD_REC_WORDS;
+  float fv = 0.0f;
+  if (lane >= 4 && lane < 14)
+    fv = ws[GD_WS_EOLD + lane - 4];
+  else if (lane == 14)
+    fv = ws[GD_WS_CMCC];
+  else if (lane >= 15 && lane < 21)
+    fv = ws[GD_WS_MOLD + lane - 15];
+  uint32_t v = __float_as_uint(fv);
+  if (lane == 0) v = (uint32_t) r;
+  if (lane == 1) v = (uint32_t) P.rep_prt[r];
+  if (lane == 2) v = (uint32_t) itmp;
+  if (lane == 3) v = (uint32_t) P.rep_lig[r];
+  if (lane == 21) v = (uint32_t) s;
+  if (lane < GD_REC_WORDS)
+    rec[lane] = v;
+}
+
+/* Metropolis (accept_d.cu:12-52) on wavefront-uniform energies: the state
+ * in the slot, the best pose in HBM; returns 1 if the move was taken. */
+__device__ __forceinline__ int gd_accept(const gd_pack &P, gd_lds_f32 *ws, int r, int s,
+                                         float u_acc, float mb, const float e_new[GD_E_N],
+                                         float cm_new, float *e_old_total, float *best_total)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const gd_state S = P.S;
+  const float de = e_new[GD_E_TOTAL] - *e_old_total;
+  if (!(u_acc < gd_exp(de * mb)))
+    return 0;
+  *e_old_total = e_new[GD_E_TOTAL];
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < GD_E_N; ++i)
+      ws[GD_WS_EOLD + i] = e_new[i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      ws[GD_WS_MOLD + k] = ws[GD_WS_MNEW + k];
+    ws[GD_WS_CMCC] = cm_new;
+  }
+  if (e_new[GD_E_TOTAL] < *best_total) {
+    *best_total = e_new[GD_E_TOTAL];
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        S.mv_best[r * 8 + k] = ws[GD_WS_MNEW + k];
+#pragma unroll
+      for (int i = 0; i < GD_E_N; ++i)
+        S.e_best[r * 10 + i] = e_new[i];
+      S.cmcc_best[r] = cm_new;
+      S.step_best[r] = s;
+    }
+  }
+  return 1;
+}
+
+/* The slot from the replica's state in HBM (wavefront 0 of the replica). */
+__device__ __forceinline__ void gd_slot_load(const gd_pack &P, gd_lds_f32 *ws, int r)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const gd_state S = P.S;
+  if (lane < 6)
+    ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
+  if (lane < GD_E_N)
+    ws[GD_WS_EOLD + lane] = S.e[r * 10 + lane];
+  if (lane == 0)
+    ws[GD_WS_CMCC] = S.cmcc[r];
+}
+
+/* The replica's state in HBM from the slot, and the cycle's accept count. */
+__device__ __forceinline__ void gd_slot_store(const gd_pack &P, const gd_lds_f32 *ws, int r,
+                                              int acs)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  const gd_state S = P.S;
+  if (lane < 6)
+    S.mv[r * 8 + lane] = ws[GD_WS_MOLD + lane];
+  if (lane < GD_E_N)
+    S.e[r * 10 + lane] = ws[GD_WS_EOLD + lane];
+  if (lane == 0) {
+    S.acs_mc[r] += acs;
+    S.cmcc[r] = ws[GD_WS_CMCC];
+  }
+}
+
+/* n_steps MC steps from step s0 of replica r on one wavefront.  The
+ * movematrices and energies live in the replica's LDS slot
+ * ws[GD_WSLOT_FLOATS]; only totals stay in registers. */
 __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *smem,
-                                                 gd_lds_f32 *ws, gd_lds_f32 *A, int role,
-                                                 int r, int prt, int s0, int n_steps)
+                                                 gd_lds_f32 *ws, int r, int prt, int s0,
+                                                 int n_steps)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const gd_state S = P.S;
   const int lconf = gd_uload(P.rep_lig + r);
-  if (role == 0) {
-    if (lane < 6)
-      ws[GD_WS_MOLD + lane] = S.mv[r * 8 + lane];
-    if (lane < GD_E_N)
-      ws[GD_WS_EOLD + lane] = S.e[r * 10 + lane];
-    if (lane == 0)
-      ws[GD_WS_CMCC] = S.cmcc[r];
-  }
-  if (SPLIT > 1)
-    __syncthreads();
-  else
-    __builtin_amdgcn_wave_barrier();
+  gd_slot_load(P, ws, r);
+  __builtin_amdgcn_wave_barrier();
   float e_old_total = S.e[r * 10 + GD_E_TOTAL];
   float best_total = S.e_best[r * 10 + GD_E_TOTAL];
   const int itmp = S.idx_tmp[r];
@@ -745,55 +962,14 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
   int acs = 0;
 
   for (int s = s0; s < s0 + n_steps; ++s) {
-    if (SPLIT > 1 && role != 0) {
-      /* evaluate the items of the pose wavefront 0 placed, then the sums */
-      __syncthreads();
-      float xyz[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-        xyz[k] = A[GD_SA_XYZ + k * GD_WAVE + lane];
-      if (P.split_extra)
-        gd_energy_items<true>(P, smem, A, role, SPLIT, prt, xyz[0], xyz[1], xyz[2], xyz[3],
-                              xyz[4], xyz[5]);
-      else
-        gd_energy_items<false>(P, smem, A, role, SPLIT, prt, xyz[0], xyz[1], xyz[2], xyz[3],
-                               xyz[4], xyz[5]);
-      __syncthreads();
-      gd_energy_sums(P, A, role, SPLIT);
-      __syncthreads();
-      continue;
-    }
-    if (P.rec != nullptr && s % P.rec_stride == 0) {
-      /* RecordLigand_d (util_d.cu:72-94): the state before this step, one
-       * word per lane */
-      uint32_t *rec = P.rec + ((size_t) r * P.rec_per_window +
-                               (s % P.rec_window) / P.rec_stride) * GD_REC_WORDS;
-      float fv = 0.0f;
-      if (lane >= 4 && lane < 14)
-        fv = ws[GD_WS_EOLD + lane - 4];
-      else if (lane == 14)
-        fv = ws[GD_WS_CMCC];
-      else if (lane >= 15 && lane < 21)
-        fv = ws[GD_WS_MOLD + lane - 15];
-      uint32_t v = __float_as_uint(fv);
-      if (lane == 0) v = (uint32_t) r;
-      if (lane == 1) v = (uint32_t) P.rep_prt[r];
-      if (lane == 2) v = (uint32_t) itmp;
-      if (lane == 3) v = (uint32_t) P.rep_lig[r];
-      if (lane == 21) v = (uint32_t) s;
-      if (lane < GD_REC_WORDS)
-        rec[lane] = v;
-    }
-    const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
-    const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
+    if (P.rec != nullptr && s % P.rec_stride == 0)
+      gd_record(P, ws, r, itmp, s);
+    float d[6], u_acc;
+    gd_draws(P, r, s, seed, cid, d, &u_acc);
     float m_new[6];
-    m_new[0] = P.move_scale[0] * (2.0f * gd_u01(ra.v[0]) - 1.0f) + ws[GD_WS_MOLD + 0];
-    m_new[1] = P.move_scale[1] * (2.0f * gd_u01(ra.v[1]) - 1.0f) + ws[GD_WS_MOLD + 1];
-    m_new[2] = P.move_scale[2] * (2.0f * gd_u01(ra.v[2]) - 1.0f) + ws[GD_WS_MOLD + 2];
-    m_new[3] = P.move_scale[3] * (2.0f * gd_u01(ra.v[3]) - 1.0f) + ws[GD_WS_MOLD + 3];
-    m_new[4] = P.move_scale[4] * (2.0f * gd_u01(rb.v[0]) - 1.0f) + ws[GD_WS_MOLD + 4];
-    m_new[5] = P.move_scale[5] * (2.0f * gd_u01(rb.v[1]) - 1.0f) + ws[GD_WS_MOLD + 5];
-    const float u_acc = gd_u01(rb.v[2]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      m_new[k] = d[k] + ws[GD_WS_MOLD + k];
 
     /* the conformer is re-read every step (an L1 hit) rather than held in
      * registers across the energy evaluation */
@@ -802,82 +978,163 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
     float ax0, ay0, az0, ax1, ay1, az1, cen[3], e_new[GD_E_N], cm_new;
     gd_place(P, m_new, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1,
              &az1, cen);
-    if (SPLIT == 1) {
-      if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-          ws[GD_WS_MNEW + k] = m_new[k];
-      }
-      gd_energy_wave(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, cen, e_new, &cm_new);
-    } else {
-      if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-          ws[GD_WS_MNEW + k] = m_new[k];
-      }
-      A[GD_SA_XYZ + 0 * GD_WAVE + lane] = ax0;
-      A[GD_SA_XYZ + 1 * GD_WAVE + lane] = ay0;
-      A[GD_SA_XYZ + 2 * GD_WAVE + lane] = az0;
-      A[GD_SA_XYZ + 3 * GD_WAVE + lane] = ax1;
-      A[GD_SA_XYZ + 4 * GD_WAVE + lane] = ay1;
-      A[GD_SA_XYZ + 5 * GD_WAVE + lane] = az1;
-      __syncthreads();
-      if (P.split_extra)
-        gd_energy_items<true>(P, smem, A, role, SPLIT, prt, ax0, ay0, az0, ax1, ay1, az1);
-      else
-        gd_energy_items<false>(P, smem, A, role, SPLIT, prt, ax0, ay0, az0, ax1, ay1, az1);
-      __syncthreads();
-      gd_energy_sums(P, A, role, SPLIT);
-      __syncthreads();
-      gd_energy_combine(P, A, cen, e_new, &cm_new);
-    }
-
-    {
-      const float de = e_new[GD_E_TOTAL] - e_old_total;
-      if (u_acc < gd_exp(de * mb)) {
-        ++acs;
-        e_old_total = e_new[GD_E_TOTAL];
-        if (lane == 0) {
-#pragma unroll
-          for (int i = 0; i < GD_E_N; ++i)
-            ws[GD_WS_EOLD + i] = e_new[i];
-#pragma unroll
-          for (int k = 0; k < 6; ++k)
-            ws[GD_WS_MOLD + k] = ws[GD_WS_MNEW + k];
-          ws[GD_WS_CMCC] = cm_new;
-        }
-        if (e_new[GD_E_TOTAL] < best_total) {
-          best_total = e_new[GD_E_TOTAL];
-          if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-              S.mv_best[r * 8 + k] = ws[GD_WS_MNEW + k];
-#pragma unroll
-            for (int i = 0; i < GD_E_N; ++i)
-              S.e_best[r * 10 + i] = e_new[i];
-            S.cmcc_best[r] = cm_new;
-            S.step_best[r] = s;
-          }
-        }
-      }
-    }
-    /* (split: the next step's first barrier orders the new state and the
-     * next placement before the other wavefronts read them, and this step's
-     * partials are read before any wavefront writes the next ones) */
-    if (SPLIT == 1)
-      __builtin_amdgcn_wave_barrier();
-  }
-
-  if (role == 0) {
-    if (lane < 6)
-      S.mv[r * 8 + lane] = ws[GD_WS_MOLD + lane];
-    if (lane < GD_E_N)
-      S.e[r * 10 + lane] = ws[GD_WS_EOLD + lane];
     if (lane == 0) {
-      S.acs_mc[r] += acs;
-      S.cmcc[r] = ws[GD_WS_CMCC];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        ws[GD_WS_MNEW + k] = m_new[k];
     }
+    gd_energy_wave(P, smem, prt, ax0, ay0, az0, ax1, ay1, az1, cen, e_new, &cm_new);
+    acs += gd_accept(P, ws, r, s, u_acc, mb, e_new, cm_new, &e_old_total, &best_total);
+    __builtin_amdgcn_wave_barrier();
   }
+  gd_slot_store(P, ws, r, acs);
+}
+
+/* Wavefront 0: pose buffer pb from movematrix m (lane-wise atoms; the
+ * movematrix and centre by lane 0). */
+__device__ __forceinline__ void gd_pose_put(const gd_pack &P, gd_lds_f32 *pb, const float m[6],
+                                            float bx0, float by0, float bz0, float bx1,
+                                            float by1, float bz1)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  float ax0, ay0, az0, ax1, ay1, az1, cen[3];
+  gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
+  pb[0 * GD_WAVE + lane] = ax0;
+  pb[1 * GD_WAVE + lane] = ay0;
+  pb[2 * GD_WAVE + lane] = az0;
+  pb[3 * GD_WAVE + lane] = ax1;
+  pb[4 * GD_WAVE + lane] = ay1;
+  pb[5 * GD_WAVE + lane] = az1;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      pb[GD_PB_M + k] = m[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      pb[GD_PB_CEN + k] = cen[k];
+  }
+}
+
+/* Items of the pose in buffer pb. */
+__device__ __forceinline__ void gd_pose_items(const gd_pack &P, const char *smem,
+                                              gd_lds_f32 *A, const gd_lds_f32 *pb, int prt,
+                                              const gd_split_sizes &Z)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  float xyz[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    xyz[k] = pb[k * GD_WAVE + lane];
+  if (P.split_extra)
+    gd_energy_items<true>(P, smem, A, prt, Z, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4], xyz[5]);
+  else
+    gd_energy_items<false>(P, smem, A, prt, Z, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4], xyz[5]);
+}
+
+/* n_steps MC steps from step s0 of replica r on the SPLIT wavefronts of the
+ * workgroup, `role` = this wavefront's index.  Per step: wavefront 0 names
+ * the pose buffer of the step and clears the item counter; barrier; every
+ * wavefront takes items -- wavefront 0 only after placing, in the two other
+ * buffers, both poses step s + 1 may start from (the move from the current
+ * pose if this one is rejected, from the proposed one if it is accepted:
+ * the same draws and arithmetic as placing it after the decision); barrier;
+ * the sums; barrier; wavefront 0 adds the terms up, decides, and takes the
+ * buffer that the decision selects.  Wavefront 0 keeps the state, the
+ * decisions and the records. */
+template <int SPLIT>
+__device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const char *smem,
+                                                       gd_lds_f32 *ws, gd_lds_f32 *A,
+                                                       int role, int r, int prt, int s0,
+                                                       int n_steps)
+{
+  const int lane = threadIdx.x & (GD_WAVE - 1);
+  gd_lds_f32 *pose = A + GD_SA_POSE;
+  const gd_split_sizes Z = gd_split_sizes_of(P);
+  if (role != 0) {
+    for (int s = s0; s < s0 + n_steps; ++s) {
+      __syncthreads();
+      const int cur = __builtin_amdgcn_readfirstlane(*(const gd_lds_i32 *) (A + GD_SA_SEL));
+      gd_pose_items(P, smem, A, pose + cur * GD_PB_FLOATS, prt, Z);
+      __syncthreads();
+      gd_energy_sums(A, role, SPLIT, Z);
+      __syncthreads();
+    }
+    return;
+  }
+
+  const gd_state S = P.S;
+  const int lconf = gd_uload(P.rep_lig + r);
+  gd_slot_load(P, ws, r);
+  __builtin_amdgcn_wave_barrier();
+  float e_old_total = S.e[r * 10 + GD_E_TOTAL];
+  float best_total = S.e_best[r * 10 + GD_E_TOTAL];
+  const int itmp = S.idx_tmp[r];
+  const float mb = P.minus_beta[itmp];
+  const uint32_t seed = P.seed, cid = P.cid;
+  const gd_final_consts F = gd_final_consts_of(P);
+  int acs = 0;
+  float bx0, by0, bz0, bx1, by1, bz1;
+  gd_load_conformer(P, lconf, &bx0, &by0, &bz0, &bx1, &by1, &bz1);
+
+  /* the pose of the first step, in buffer 0 */
+  int cur = 0;
+  float d[6], u_acc, m[6];
+  gd_draws(P, r, s0, seed, cid, d, &u_acc);
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    m[k] = d[k] + ws[GD_WS_MOLD + k];
+  gd_pose_put(P, pose, m, bx0, by0, bz0, bx1, by1, bz1);
+  __builtin_amdgcn_wave_barrier();
+
+  for (int s = s0; s < s0 + n_steps; ++s) {
+    gd_lds_f32 *pc = pose + cur * GD_PB_FLOATS;
+    if (P.rec != nullptr && s % P.rec_stride == 0)
+      gd_record(P, ws, r, itmp, s);
+    if (lane == 0) {
+      *(gd_lds_i32 *) (A + GD_SA_CTR) = 0;
+      *(gd_lds_i32 *) (A + GD_SA_SEL) = cur;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        ws[GD_WS_MNEW + k] = pc[GD_PB_M + k];
+    }
+    __syncthreads();
+
+    /* both poses of step s + 1: buffer ia if this move is rejected, ib if
+     * it is accepted */
+    const bool more = s + 1 < s0 + n_steps;
+    const int ia = cur == 2 ? 0 : cur + 1;
+    const int ib = ia == 2 ? 0 : ia + 1;
+    float u_next = 0.0f;
+    if (more) {
+      gd_draws(P, r, s + 1, seed, cid, d, &u_next);
+      float ma[6], mbv[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        ma[k] = d[k] + ws[GD_WS_MOLD + k];
+        mbv[k] = d[k] + pc[GD_PB_M + k];
+      }
+      gd_pose_put(P, pose + ia * GD_PB_FLOATS, ma, bx0, by0, bz0, bx1, by1, bz1);
+      gd_pose_put(P, pose + ib * GD_PB_FLOATS, mbv, bx0, by0, bz0, bx1, by1, bz1);
+    }
+    float cen[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      cen[k] = pc[GD_PB_CEN + k];
+    const float edst = gd_edst(P, cen);
+    gd_pose_items(P, smem, A, pc, prt, Z);
+    __syncthreads();
+    gd_energy_sums(A, 0, SPLIT, Z);
+    __syncthreads();
+    float e_new[GD_E_N], cm_new;
+    gd_energy_combine(P, A, F, edst, e_new, &cm_new);
+    const int took = gd_accept(P, ws, r, s, u_acc, mb, e_new, cm_new, &e_old_total,
+                               &best_total);
+    acs += took;
+    cur = took ? ib : ia;
+    u_acc = u_next;
+    __builtin_amdgcn_wave_barrier();
+  }
+  gd_slot_store(P, ws, r, acs);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1077,7 +1334,7 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
       if (w < task.r_count) {
         const int r = task.r_begin + w;
         if (mode == 0)
-          gd_replica_cycle<1>(P, smem, ws, nullptr, 0, r, task.prt, s0, n_steps);
+          gd_replica_cycle(P, smem, ws, r, task.prt, s0, n_steps);
         else
           gd_replica_init(P, smem, r);
       }
@@ -1131,7 +1388,8 @@ gd_mc_split_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict_
         staged_cx = task.cx;
         staged_prt = task.prt;
       }
-      gd_replica_cycle<SPLIT>(P, smem, ws, A, role, task.r_begin, task.prt, s0, n_steps);
+      gd_replica_cycle_split<SPLIT>(P, smem, ws, A, role, task.r_begin, task.prt, s0,
+                                    n_steps);
     }
     if (!queue)
       t = -2;
