@@ -169,12 +169,14 @@ typedef struct gd_chain {
 #define GD_WS_CMCC 32     /* energy_old.cmcc */
 
 /* A replica split over several wavefronts keeps the partial sums of its
- * summation blocks in LDS beside its slot, with three placed poses: this
- * many bytes (atom and KDE blocks, restraint values, poses, sums);
+ * summation blocks in LDS beside its slot, with three placed poses and
+ * the conformer: this many bytes (atom and KDE blocks, restraint values,
+ * poses, conformer, sums);
  * complexes with more position restraints than GD_SPLIT_MAX_POS run one
  * wavefront per replica. */
 #define GD_SPLIT_MAX_POS 64
 #define GD_SPLIT_AREA_BYTES \
-  ((GD_MAX_BLOCKS * 6 * 64 + GD_MAX_BLOCKS * 4 + GD_SPLIT_MAX_POS + 3 * (6 * 64 + 16) + 16) * 4)
+  ((GD_MAX_BLOCKS * 6 * 64 + GD_MAX_BLOCKS * 4 + GD_SPLIT_MAX_POS + 3 * (6 * 64 + 16) + \
+    6 * 64 + 16) * 4)
 
 #endif /* GD_DEVICE_H */

@@ -216,11 +216,12 @@ int gd_engine::choose_split() const
   long long reps = 0;
   for (auto *s : slots) {
     const gd_pack &P = s->img->P;
-    /* restraints beyond the area's slots, or staging that leaves no room
-     * for the area: one wavefront per replica */
-    if (P.pos > GD_SPLIT_MAX_POS ||
-        std::max(P.lds_bytes, P.split_lds_bytes) + GD_WSLOT_FLOATS * 4 +
-                (int) GD_SPLIT_AREA_BYTES > max_lds)
+    /* split launches stage every table, the pair list, the restraint
+     * targets and the PSP rows in LDS beside the area: restraints beyond the
+     * area's slots, a complex whose extras do not fit, or KDE points kept in
+     * HBM mean one wavefront per replica */
+    if (P.pos > GD_SPLIT_MAX_POS || !P.split_extra || kde_global ||
+        P.split_lds_bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES > max_lds)
       return 1;
     reps += P.n_rep;
   }
@@ -270,6 +271,8 @@ int gd_engine::rebuild()
     if (kde_global && p.kde_in_lds) {
       p.kde_in_lds = 0;
       p.lds_bytes = p.off_kde;
+      p.split_extra = 0;
+      p.split_lds_bytes = p.lds_bytes;
     }
     packs.push_back(p);
     const int stage = std::max(16, split_eff > 1 ? std::max(p.lds_bytes, p.split_lds_bytes)

@@ -175,7 +175,9 @@ struct gd_blk {
 /* Pair loop + hydrophobic restraint (calcenergy_d.cu:56-149) over ligand
  * atoms [l0, l1): lane partials from +0 into blk, the block's restraint sum
  * (uniform) into *ehpc_b, MCC confusion counts (uniform) added to tp/fp/fn.
- * XL: the PSP rows are staged in LDS (split launches), else read from HBM. */
+ * XL (split launches, whose complexes all stage the extras): pocket points,
+ * KDE points and PSP rows are all in LDS; else the PSP rows come from HBM
+ * and the pocket from LDS or HBM as staged. */
 template <bool XL>
 __device__ __forceinline__ void gd_atoms_block(
     const gd_pack &P, const char *smem, int prt, float ax0, float ay0, float az0, float ax1,
@@ -282,7 +284,7 @@ __device__ __forceinline__ void gd_atoms_block(
         chunks(read, off, off, off, kp, nchunk);
       }
     };
-    if (pocket_lds) {
+    if (XL || pocket_lds) {
       rows([&](int k, float4 &q, float2 &eh) {
         q = s_pos[lane + k * GD_WAVE];
         eh = s_eh[lane + k * GD_WAVE];
@@ -356,7 +358,7 @@ __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *sme
       lane_sum = (int) ulane < left ? sum : lane_sum;
     }
   };
-  if (P.kde_in_lds) {
+  if (XL || P.kde_in_lds) {
     const __attribute__((address_space(3))) char *pts =
         (const __attribute__((address_space(3))) char *) (smem + P.off_kde);
     rounds([&](uint32_t off) {
@@ -621,7 +623,8 @@ __device__ __forceinline__ void gd_replica_init(const gd_pack &P, const char *sm
 #define GD_PB_CEN (GD_PB_M + 6)
 #define GD_PB_FLOATS (6 * GD_WAVE + 16)
 #define GD_SA_POSE (GD_SA_LHM + GD_SPLIT_MAX_POS)
-#define GD_SA_SUMS (GD_SA_POSE + 3 * GD_PB_FLOATS)  /* the summed terms */
+#define GD_SA_CONF (GD_SA_POSE + 3 * GD_PB_FLOATS)  /* the conformer, by lane */
+#define GD_SA_SUMS (GD_SA_CONF + 6 * GD_WAVE)       /* the summed terms */
 #define GD_SA_SEL (GD_SA_SUMS + 14)               /* the pose buffer of this step (int) */
 #define GD_SA_CTR (GD_SA_SUMS + 15)               /* the item counter (int) */
 #define GD_SA_FLOATS (GD_SA_SUMS + 16)
@@ -634,6 +637,7 @@ struct gd_split_sizes {
   int bs, nb;          /* atom block size and count */
   int n_rounds, kbs, nbk;   /* KDE rounds, rounds per block, blocks */
   int pos, n_items;
+  int mcc;             /* 1: MCC counts are kept */
 };
 
 __device__ __forceinline__ gd_split_sizes gd_split_sizes_of(const gd_pack &P)
@@ -646,6 +650,7 @@ __device__ __forceinline__ gd_split_sizes gd_split_sizes_of(const gd_pack &P)
   z.nbk = (z.n_rounds + z.kbs - 1) / z.kbs;
   z.pos = P.pos;
   z.n_items = z.nb + z.nbk + z.pos;
+  z.mcc = P.mcc_ref != nullptr;
   return z;
 }
 
@@ -700,66 +705,89 @@ __device__ __forceinline__ void gd_energy_items(
   }
 }
 
+/* t + src[0] + src[stride] + ... + src[(n - 1) stride], in that order, for
+ * n <= GD_MAX_BLOCKS: the values are read eight at a time before they are
+ * added, so the LDS loads overlap. */
+__device__ __forceinline__ float gd_ordered_sum(const gd_lds_f32 *src, int stride, int n)
+{
+  float t = 0.0f;
+#pragma unroll
+  for (int b0 = 0; b0 < GD_MAX_BLOCKS; b0 += 8) {
+    if (b0 >= n)
+      break;
+    float v[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      v[b] = src[(b0 + b) * stride];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (b0 + b < n)
+        t += v[b];
+  }
+  return t;
+}
+
 /* After the items: the partials added in block order, spread over the
- * wavefronts -- job j (vdw, pmf, psp, hdb, ele lane totals and their wave
- * sums; the KDE lane total and its wave sum; the restraint, MCC and
- * position-restraint sums) on wavefront j % split -- into the sums area.
- * The last job reads block b's values in lane b and adds them in block
- * order from the lanes (the MCC counts are integers: any order). */
+ * wavefronts -- job j on wavefront j % split: jobs 0-4 the vdw, pmf, psp,
+ * hdb, ele lane totals over the atom blocks and their wave sums, job 5 the
+ * KDE lane total and its wave sum, job 6 the restraint sum and the MCC
+ * counts, job 7 the position-restraint sum -- into the sums area.  Every
+ * job reads all its slots before adding (the area has GD_MAX_BLOCKS of
+ * each; the adds use the first nb / nbk in order). */
 __device__ __forceinline__ void gd_energy_sums(gd_lds_f32 *A, int role, int split,
                                                const gd_split_sizes &Z)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int nb = Z.nb, nbk = Z.nbk;
-  for (int j = role; j < 7; j += split) {
-    if (j < 5) {
-      float t = 0.0f;
-      for (int b = 0; b < nb; ++b)
-        t += A[GD_SA_PART + (b * 5 + j) * GD_WAVE + lane];
-      t = gd_wave_sum(t);
+  for (int j = role; j < 8; j += split) {
+    if (j < 6) {
+      const float t = gd_wave_sum(
+          j < 5 ? gd_ordered_sum(A + GD_SA_PART + j * GD_WAVE + lane, 5 * GD_WAVE, nb)
+                : gd_ordered_sum(A + GD_SA_KPART + lane, GD_WAVE, nbk));
       if (lane == 0)
         A[GD_SA_SUMS + j] = t;
-    } else if (j == 5) {
-      float t = 0.0f;
-      for (int kb = 0; kb < nbk; ++kb)
-        t += A[GD_SA_KPART + kb * GD_WAVE + lane];
-      t = gd_wave_sum(t);
-      if (lane == 0)
-        A[GD_SA_SUMS + 5] = t;
-    } else {
-      const int pos = Z.pos;
-      const bool in_b = lane < nb;
-      const float ev = in_b ? A[GD_SA_EHPC + lane] : 0.0f;
-      const int tv = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 0]) : 0;
-      const int fv = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 1]) : 0;
-      const int nv = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 2]) : 0;
-      const float lv = lane < pos ? A[GD_SA_LHM + lane] : 0.0f;
-      float ehpc = 0.0f;
+    } else if (j == 6) {
+      const float ehpc = gd_ordered_sum(A + GD_SA_EHPC, 1, nb);
+      /* the MCC counts are integers: any order */
       int tp = 0, fp = 0, fn = 0;
-      for (int b = 0; b < nb; ++b) {
-        ehpc += gd_bcast(ev, b);
-        tp += __builtin_amdgcn_readlane(tv, b);
-        fp += __builtin_amdgcn_readlane(fv, b);
-        fn += __builtin_amdgcn_readlane(nv, b);
+      if (Z.mcc) {
+        const bool in_b = lane < nb;
+        tp = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 0]) : 0;
+        fp = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 1]) : 0;
+        fn = in_b ? __float_as_int(A[GD_SA_MCC + 3 * lane + 2]) : 0;
+#pragma unroll
+        for (int off = 1; off < GD_MAX_BLOCKS; off <<= 1) {
+          tp += __shfl_xor(tp, off, GD_WAVE);
+          fp += __shfl_xor(fp, off, GD_WAVE);
+          fn += __shfl_xor(fn, off, GD_WAVE);
+        }
       }
-      float elhm = 0.0f;
-      for (int m = 0; m < pos; ++m)
-        elhm += gd_bcast(lv, m);
       if (lane == 0) {
         A[GD_SA_SUMS + 6] = ehpc;
-        A[GD_SA_SUMS + 7] = elhm;
         A[GD_SA_SUMS + 8] = __int_as_float(tp);
         A[GD_SA_SUMS + 9] = __int_as_float(fp);
         A[GD_SA_SUMS + 10] = __int_as_float(fn);
       }
+    } else {
+      /* position restraints in order, value m in lane m */
+      const int pos = Z.pos;
+      float elhm = 0.0f;
+      if (pos > 0) {
+        const float lv = lane < pos ? A[GD_SA_LHM + lane] : 0.0f;
+        for (int m = 0; m < pos; ++m)
+          elhm += gd_bcast(lv, m);
+      }
+      if (lane == 0)
+        A[GD_SA_SUMS + 7] = elhm;
     }
   }
 }
 
 /* What the energies need of a complex, read once per cycle: lane k < 10
- * holds a_para[k], b_para[k], w[k]. */
+ * holds a_para[k] and b_para[k]; the weights are uniform. */
 struct gd_final_consts {
-  float a_l, b_l, w_l;
+  float a_l, b_l;
+  float w[GD_F_NTERM];   /* uniform */
   float lna_f, kde3;
   int pos, term_order, combine;
 };
@@ -770,7 +798,9 @@ __device__ __forceinline__ gd_final_consts gd_final_consts_of(const gd_pack &P)
   gd_final_consts F;
   F.a_l = lane < 10 ? P.a_para[lane] : 0.0f;
   F.b_l = lane < 10 ? P.b_para[lane] : 0.0f;
-  F.w_l = lane < 10 ? P.w[lane] : 0.0f;
+#pragma unroll
+  for (int k = 0; k < GD_F_NTERM; ++k)
+    F.w[k] = P.w[k];
   F.lna_f = (float) P.lna;
   F.kde3 = P.kde3;
   F.pos = P.pos;
@@ -819,7 +849,7 @@ __device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds
   float total = 0.0f;
 #pragma unroll
   for (int k = 0; k < GD_F_NTERM; ++k)
-    total = gd_fma(gd_bcast(F.w_l, k), gd_bcast(v, k), total);
+    total = gd_fma(F.w[k], gd_bcast(v, k), total);
 #pragma unroll
   for (int i = 0; i < GD_F_NTERM; ++i) {
     /* the lane that formed slot i */
@@ -990,13 +1020,15 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
   gd_slot_store(P, ws, r, acs);
 }
 
-/* Wavefront 0: pose buffer pb from movematrix m (lane-wise atoms; the
- * movematrix and centre by lane 0). */
+/* Wavefront 0: pose buffer pb from movematrix m and the conformer staged
+ * at conf (lane-wise atoms; the movematrix and centre by lane 0). */
 __device__ __forceinline__ void gd_pose_put(const gd_pack &P, gd_lds_f32 *pb, const float m[6],
-                                            float bx0, float by0, float bz0, float bx1,
-                                            float by1, float bz1)
+                                            const gd_lds_f32 *conf)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
+  const float bx0 = conf[0 * GD_WAVE + lane], by0 = conf[1 * GD_WAVE + lane];
+  const float bz0 = conf[2 * GD_WAVE + lane], bx1 = conf[3 * GD_WAVE + lane];
+  const float by1 = conf[4 * GD_WAVE + lane], bz1 = conf[5 * GD_WAVE + lane];
   float ax0, ay0, az0, ax1, ay1, az1, cen[3];
   gd_place(P, m, bx0, by0, bz0, bx1, by1, bz1, &ax0, &ay0, &az0, &ax1, &ay1, &az1, cen);
   pb[0 * GD_WAVE + lane] = ax0;
@@ -1015,7 +1047,7 @@ __device__ __forceinline__ void gd_pose_put(const gd_pack &P, gd_lds_f32 *pb, co
   }
 }
 
-/* Items of the pose in buffer pb. */
+/* Items of the pose in buffer pb (split launches stage the extras). */
 __device__ __forceinline__ void gd_pose_items(const gd_pack &P, const char *smem,
                                               gd_lds_f32 *A, const gd_lds_f32 *pb, int prt,
                                               const gd_split_sizes &Z)
@@ -1025,10 +1057,7 @@ __device__ __forceinline__ void gd_pose_items(const gd_pack &P, const char *smem
 #pragma unroll
   for (int k = 0; k < 6; ++k)
     xyz[k] = pb[k * GD_WAVE + lane];
-  if (P.split_extra)
-    gd_energy_items<true>(P, smem, A, prt, Z, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4], xyz[5]);
-  else
-    gd_energy_items<false>(P, smem, A, prt, Z, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4], xyz[5]);
+  gd_energy_items<true>(P, smem, A, prt, Z, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4], xyz[5]);
 }
 
 /* n_steps MC steps from step s0 of replica r on the SPLIT wavefronts of the
@@ -1040,7 +1069,8 @@ __device__ __forceinline__ void gd_pose_items(const gd_pack &P, const char *smem
  * the same draws and arithmetic as placing it after the decision); barrier;
  * the sums; barrier; wavefront 0 adds the terms up, decides, and takes the
  * buffer that the decision selects.  Wavefront 0 keeps the state, the
- * decisions and the records. */
+ * decisions and the records.  One code path for every role (the item and
+ * sum code is shared, not repeated per role). */
 template <int SPLIT>
 __device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const char *smem,
                                                        gd_lds_f32 *ws, gd_lds_f32 *A,
@@ -1050,91 +1080,100 @@ __device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const c
   const int lane = threadIdx.x & (GD_WAVE - 1);
   gd_lds_f32 *pose = A + GD_SA_POSE;
   const gd_split_sizes Z = gd_split_sizes_of(P);
-  if (role != 0) {
-    for (int s = s0; s < s0 + n_steps; ++s) {
-      __syncthreads();
-      const int cur = __builtin_amdgcn_readfirstlane(*(const gd_lds_i32 *) (A + GD_SA_SEL));
-      gd_pose_items(P, smem, A, pose + cur * GD_PB_FLOATS, prt, Z);
-      __syncthreads();
-      gd_energy_sums(A, role, SPLIT, Z);
-      __syncthreads();
-    }
-    return;
-  }
+  const bool lead = role == 0;
 
   const gd_state S = P.S;
-  const int lconf = gd_uload(P.rep_lig + r);
-  gd_slot_load(P, ws, r);
-  __builtin_amdgcn_wave_barrier();
-  float e_old_total = S.e[r * 10 + GD_E_TOTAL];
-  float best_total = S.e_best[r * 10 + GD_E_TOTAL];
-  const int itmp = S.idx_tmp[r];
-  const float mb = P.minus_beta[itmp];
+  float e_old_total = 0.0f, best_total = 0.0f, mb = 0.0f;
+  int itmp = 0, acs = 0, cur = 0;
+  gd_lds_f32 *conf = A + GD_SA_CONF;
+  float d[6], u_acc = 0.0f;
+  gd_final_consts F;
   const uint32_t seed = P.seed, cid = P.cid;
-  const gd_final_consts F = gd_final_consts_of(P);
-  int acs = 0;
-  float bx0, by0, bz0, bx1, by1, bz1;
-  gd_load_conformer(P, lconf, &bx0, &by0, &bz0, &bx1, &by1, &bz1);
-
-  /* the pose of the first step, in buffer 0 */
-  int cur = 0;
-  float d[6], u_acc, m[6];
-  gd_draws(P, r, s0, seed, cid, d, &u_acc);
-#pragma unroll
-  for (int k = 0; k < 6; ++k)
-    m[k] = d[k] + ws[GD_WS_MOLD + k];
-  gd_pose_put(P, pose, m, bx0, by0, bz0, bx1, by1, bz1);
-  __builtin_amdgcn_wave_barrier();
-
-  for (int s = s0; s < s0 + n_steps; ++s) {
-    gd_lds_f32 *pc = pose + cur * GD_PB_FLOATS;
-    if (P.rec != nullptr && s % P.rec_stride == 0)
-      gd_record(P, ws, r, itmp, s);
-    if (lane == 0) {
-      *(gd_lds_i32 *) (A + GD_SA_CTR) = 0;
-      *(gd_lds_i32 *) (A + GD_SA_SEL) = cur;
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-        ws[GD_WS_MNEW + k] = pc[GD_PB_M + k];
+  if (lead) {
+    gd_slot_load(P, ws, r);
+    __builtin_amdgcn_wave_barrier();
+    e_old_total = S.e[r * 10 + GD_E_TOTAL];
+    best_total = S.e_best[r * 10 + GD_E_TOTAL];
+    itmp = S.idx_tmp[r];
+    mb = P.minus_beta[itmp];
+    F = gd_final_consts_of(P);
+    {
+      float bx0, by0, bz0, bx1, by1, bz1;
+      gd_load_conformer(P, gd_uload(P.rep_lig + r), &bx0, &by0, &bz0, &bx1, &by1, &bz1);
+      conf[0 * GD_WAVE + lane] = bx0;
+      conf[1 * GD_WAVE + lane] = by0;
+      conf[2 * GD_WAVE + lane] = bz0;
+      conf[3 * GD_WAVE + lane] = bx1;
+      conf[4 * GD_WAVE + lane] = by1;
+      conf[5 * GD_WAVE + lane] = bz1;
     }
-    __syncthreads();
-
-    /* both poses of step s + 1: buffer ia if this move is rejected, ib if
-     * it is accepted */
-    const bool more = s + 1 < s0 + n_steps;
-    const int ia = cur == 2 ? 0 : cur + 1;
-    const int ib = ia == 2 ? 0 : ia + 1;
-    float u_next = 0.0f;
-    if (more) {
-      gd_draws(P, r, s + 1, seed, cid, d, &u_next);
-      float ma[6], mbv[6];
+    /* the pose of the first step, in buffer 0 */
+    float m[6];
+    gd_draws(P, r, s0, seed, cid, d, &u_acc);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        ma[k] = d[k] + ws[GD_WS_MOLD + k];
-        mbv[k] = d[k] + pc[GD_PB_M + k];
-      }
-      gd_pose_put(P, pose + ia * GD_PB_FLOATS, ma, bx0, by0, bz0, bx1, by1, bz1);
-      gd_pose_put(P, pose + ib * GD_PB_FLOATS, mbv, bx0, by0, bz0, bx1, by1, bz1);
-    }
-    float cen[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      cen[k] = pc[GD_PB_CEN + k];
-    const float edst = gd_edst(P, cen);
-    gd_pose_items(P, smem, A, pc, prt, Z);
-    __syncthreads();
-    gd_energy_sums(A, 0, SPLIT, Z);
-    __syncthreads();
-    float e_new[GD_E_N], cm_new;
-    gd_energy_combine(P, A, F, edst, e_new, &cm_new);
-    const int took = gd_accept(P, ws, r, s, u_acc, mb, e_new, cm_new, &e_old_total,
-                               &best_total);
-    acs += took;
-    cur = took ? ib : ia;
-    u_acc = u_next;
+    for (int k = 0; k < 6; ++k)
+      m[k] = d[k] + ws[GD_WS_MOLD + k];
+    gd_pose_put(P, pose, m, conf);
     __builtin_amdgcn_wave_barrier();
   }
-  gd_slot_store(P, ws, r, acs);
+
+  for (int s = s0; s < s0 + n_steps; ++s) {
+    if (lead) {
+      const gd_lds_f32 *pc = pose + cur * GD_PB_FLOATS;
+      if (P.rec != nullptr && s % P.rec_stride == 0)
+        gd_record(P, ws, r, itmp, s);
+      if (lane == 0) {
+        *(gd_lds_i32 *) (A + GD_SA_CTR) = 0;
+        *(gd_lds_i32 *) (A + GD_SA_SEL) = cur;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          ws[GD_WS_MNEW + k] = pc[GD_PB_M + k];
+      }
+    }
+    __syncthreads();
+    const int sel = __builtin_amdgcn_readfirstlane(*(const gd_lds_i32 *) (A + GD_SA_SEL));
+    const gd_lds_f32 *pc = pose + sel * GD_PB_FLOATS;
+
+    /* wavefront 0: both poses of step s + 1, buffer ia if this move is
+     * rejected, ib if it is accepted */
+    const int ia = sel == 2 ? 0 : sel + 1;
+    const int ib = ia == 2 ? 0 : ia + 1;
+    float u_next = 0.0f, edst = 0.0f;
+    if (lead) {
+      if (s + 1 < s0 + n_steps) {
+        gd_draws(P, r, s + 1, seed, cid, d, &u_next);
+        float ma[6], mbv[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          ma[k] = d[k] + ws[GD_WS_MOLD + k];
+          mbv[k] = d[k] + pc[GD_PB_M + k];
+        }
+        gd_pose_put(P, pose + ia * GD_PB_FLOATS, ma, conf);
+        gd_pose_put(P, pose + ib * GD_PB_FLOATS, mbv, conf);
+      }
+      float cen[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        cen[k] = pc[GD_PB_CEN + k];
+      edst = gd_edst(P, cen);
+    }
+    gd_pose_items(P, smem, A, pc, prt, Z);
+    __syncthreads();
+    gd_energy_sums(A, role, SPLIT, Z);
+    __syncthreads();
+    if (lead) {
+      float e_new[GD_E_N], cm_new;
+      gd_energy_combine(P, A, F, edst, e_new, &cm_new);
+      const int took = gd_accept(P, ws, r, s, u_acc, mb, e_new, cm_new, &e_old_total,
+                                 &best_total);
+      acs += took;
+      cur = took ? ib : ia;
+      u_acc = u_next;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (lead)
+    gd_slot_store(P, ws, r, acs);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1382,8 +1421,7 @@ gd_mc_split_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict_
       if (task.cx != staged_cx || task.prt != staged_prt) {
         __syncthreads();
         gd_stage(P, task.prt, smem);
-        if (P.split_extra)
-          gd_stage_split_extra(P, task.prt, smem);
+        gd_stage_split_extra(P, task.prt, smem);
         __syncthreads();
         staged_cx = task.cx;
         staged_prt = task.prt;
