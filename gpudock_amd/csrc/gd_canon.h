@@ -50,8 +50,12 @@
  * the values computed). */
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GD_KEEP_BRANCH() asm volatile("")
+/* Has x computed at this point (no effect on its value): lets a register
+ * whose last use follows be reused instead of copied. */
+#define GD_COMPUTED(x) asm volatile("" : : "v"(x))
 #else
 #define GD_KEEP_BRANCH() ((void) 0)
+#define GD_COMPUTED(x) ((void) (x))
 #endif
 
 /* ------------------------------------------------------------------ */
@@ -141,9 +145,12 @@ GD_HD float gd_exp(float x)
  * which no energy sum can tell from 0).  n = round(x / ln2) comes from
  * adding 1.5 * 2^23 in one fma (the integer lands in the low mantissa bits,
  * so 2^n is one shift-and-add of those bits); then a one-constant
- * Cody-Waite step (|r| error <= 2.4e-7 at |x| = 86) and a degree-6
- * polynomial fitted here for minimax relative error on |r| <= ln2/2
- * (6.5e-8 in FP32); total error below 4 ulp.  Returns the polynomial p and
+ * Cody-Waite step (|r| error <= 2.4e-7 at |x| = 86) and a degree-5
+ * polynomial 1 + r + r^2 q(r) fitted here for minimax relative error on
+ * |r| <= ln2/2 (1.05e-7 before FP32 rounding); total error below 3 ulp on
+ * [-20, 20] (mean 0.75 ulp) and below 5 ulp down to -86, a relative 1.8e-7
+ * at most: two orders under the energies' 1e-5 bar, for one fma per
+ * exponential fewer than a degree-6 fit.  Returns the polynomial p and
  * sets *scale = 2^n; e^x = p * 2^n. */
 GD_HD float gd_exp_hot_split(float x, float *scale)
 {
@@ -151,11 +158,10 @@ GD_HD float gd_exp_hot_split(float x, float *scale)
   const float t = gd_fma(xc, 0x1.715476p+0f, 0x1.8p+23f);
   const float n = t - 0x1.8p+23f;
   const float r = gd_fma(-n, 0x1.62e430p-1f, xc);
-  float p = 0x1.6ab926p-10f;
-  p = gd_fma(p, r, 0x1.126d64p-7f);
-  p = gd_fma(p, r, 0x1.55589cp-5f);
-  p = gd_fma(p, r, 0x1.55540ap-3f);
-  p = gd_fma(p, r, 0x1.fffffap-2f);
+  float p = 0x1.106288p-7f;
+  p = gd_fma(p, r, 0x1.5729f2p-5f);
+  p = gd_fma(p, r, 0x1.5557aep-3f);
+  p = gd_fma(p, r, 0x1.fffdfcp-2f);
   p = gd_fma(p, r, 1.0f);
   p = gd_fma(p, r, 1.0f);
   /* bits(t) = 0x4b400000 + n: (bits << 23) + (127 << 23) = bits of 2^n */
@@ -332,12 +338,16 @@ typedef struct __attribute__((aligned(16))) {
  * entry in a table row: type << GD_PT_TYPE_SHIFT (sizeof(gd_pair_param)). */
 #define GD_PT_TYPE_SHIFT 5
 
-/* Scalars shared by every pair.  The short-range electrostatics
- * el0 + a1 s^2 + b1 s^3, s = el1 d (calcenergy_d.cu:99-105) is evaluated
- * as el0 + d^2 (ea + eb d) with ea = a1 el1^2, eb = b1 el1^3, for
- * d^2 < dcut2 = 1 / el1^2 (s < 1). */
+/* Scalars shared by every pair.  The distance-dependent electrostatics
+ * g(s), s = el1 d (calcenergy_d.cu:99-105), is 1 / s for s >= 1 and
+ * el0 + a1 s^2 + b1 s^3 below; the pose sums ele_pt q g.  The pair loop
+ * takes el1 g instead -- 1 / d far out, el1 el0 + d^2 (ea1 + eb1 d) for
+ * d^2 < dcut2 = 1 / el1^2, with ea1 = a1 el1^3, eb1 = b1 el1^4 -- and the
+ * pose's electrostatic sum is multiplied by inv_el1 = 1 / el1 once.  The
+ * soft L-J denominator 1 + lj0 p1 (1 + lj1 d^2) is 1 + p1 (lj0 + lj01 d^2),
+ * lj01 = lj0 lj1. */
 typedef struct {
-  float lj0, lj1, el0, el1, a1, b1, inv_el1, ea, eb, dcut2, pad0, pad1;
+  float lj0, lj1, el0, el1, a1, b1, inv_el1, ea, eb, dcut2, lj01, ea1, eb1, el01, pad0, pad1;
 } gd_pair_scalars;
 
 GD_HD void gd_pair_scalars_init(gd_pair_scalars *sc, float lj0, float lj1, float el0,
@@ -353,9 +363,14 @@ GD_HD void gd_pair_scalars_init(gd_pair_scalars *sc, float lj0, float lj1, float
   sc->ea = (a1 * el1) * el1;
   sc->eb = ((b1 * el1) * el1) * el1;
   sc->dcut2 = 1.0f / (el1 * el1);
+  sc->lj01 = lj0 * lj1;
+  sc->ea1 = sc->ea * el1;
+  sc->eb1 = sc->eb * el1;
+  sc->el01 = el0 * el1;
   sc->pad0 = 0.0f;
   sc->pad1 = 0.0f;
 }
+
 
 /* Accumulators of one lane. */
 typedef struct {
@@ -378,7 +393,8 @@ GD_HD float gd_hpc_pair(float d2, float hpp)
   return gd_fma(-hpp, t, hpp);
 }
 
-/* The five directly summed pair terms.  psp_val is the PSP table value
+/* The five directly summed pair terms (ele: times el1, see
+ * gd_pair_scalars).  psp_val is the PSP table value
  * psp[ligand type][residue] of the point (any value for points that are
  * not class 2); it is added for class-2 points inside the PMF radius
  * (calcenergy_d.cu:123-126).  acc->ele collects ele_point * g for the
@@ -402,17 +418,21 @@ GD_HD int gd_pair_terms(float d2, float ele_pt, const gd_pair_param *pp,
   /* soft L-J numerator and denominator */
   const float p1 = pp->p1a * r9;
   const float p2 = pp->p2a * r6;
-  const float p4 = gd_fma(p1 * sc->lj0, gd_fma(sc->lj1, d2, 1.0f), 1.0f);
+  const float p4 = gd_fma(p1, gd_fma(sc->lj01, d2, sc->lj0), 1.0f);
 
-  /* distance-dependent electrostatics; the short-range form only where
-   * some pair of the call's group (near_any) is inside it */
-  float g = r1 * sc->inv_el1;
+  /* distance-dependent electrostatics times el1 (the pose's sum is scaled
+   * by 1 / el1 once); the short-range form only where some pair of the
+   * call's group (near_any) is inside it (r1's other uses come first, so
+   * the far form takes r1 in place) */
+  GD_COMPUTED(r9);
+  GD_COMPUTED(d);
   if (near_any) {
     GD_KEEP_BRANCH();
-    if (d2 < sc->dcut2)
-      g = gd_fma(d2, gd_fma(d, sc->eb, sc->ea), sc->el0);
+    const float gn = gd_fma(d2, gd_fma(d, sc->eb1, sc->ea1), sc->el01);
+    acc->ele = gd_fma(ele_pt, d2 < sc->dcut2 ? gn : r1, acc->ele);
+  } else {
+    acc->ele = gd_fma(ele_pt, r1, acc->ele);
   }
-  acc->ele = gd_fma(ele_pt, g, acc->ele);
 
   /* PMF contact denominator */
   const float dm = d - pp->pmf0;

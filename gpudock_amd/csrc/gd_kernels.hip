@@ -192,10 +192,11 @@ __device__ __forceinline__ void gd_atoms_block(
   const int nchunk = P.pnp_pad / GD_WAVE;   /* padding points add exact zeros */
   const int n_rows = P.n_rows;
   const gd_pair_scalars sc = P.sc;
-  /* the near-field coefficients sit beside other scalars in one fma: held
-   * in vector registers, the loop needs no copy per pair */
+  /* the near-field coefficients and lj0 sit beside other scalars in one
+   * fma (one scalar operand per instruction): held in vector registers, the
+   * loop needs no copy per pair */
   gd_pair_scalars sc_v = sc;
-  asm volatile("" : "+v"(sc_v.ea), "+v"(sc_v.eb));
+  asm volatile("" : "+v"(sc_v.ea1), "+v"(sc_v.eb1), "+v"(sc_v.lj0));
 
   gd_pair_acc acc;
   acc.vdw = 0.0f; acc.ele = 0.0f; acc.pmf = 0.0f; acc.psp = 0.0f; acc.hdb = 0.0f;
@@ -443,7 +444,7 @@ __device__ __forceinline__ void gd_energy_final_d(const gd_pack &P, float vdw, f
   const float elhm = (pos > 0) ? gd_log(elhm_sum / (float) pos) : 0.0f;
   float sums[GD_F_NTERM];
   sums[GD_F_VDW] = vdw;
-  sums[GD_F_ELE] = ele;
+  sums[GD_F_ELE] = ele * P.sc.inv_el1;   /* the pair loop sums el1 g */
   sums[GD_F_PMF] = pmf;
   sums[GD_F_HPC] = ehpc;
   sums[GD_F_HDB] = hdb;
@@ -788,7 +789,7 @@ __device__ __forceinline__ void gd_energy_sums(gd_lds_f32 *A, int role, int spli
 struct gd_final_consts {
   float a_l, b_l;
   float w[GD_F_NTERM];   /* uniform */
-  float lna_f, kde3;
+  float lna_f, kde3, inv_el1;
   int pos, term_order, combine;
 };
 
@@ -803,6 +804,7 @@ __device__ __forceinline__ gd_final_consts gd_final_consts_of(const gd_pack &P)
     F.w[k] = P.w[k];
   F.lna_f = (float) P.lna;
   F.kde3 = P.kde3;
+  F.inv_el1 = P.sc.inv_el1;
   F.pos = P.pos;
   F.term_order = P.term_order;
   F.combine = P.combine;
@@ -836,6 +838,8 @@ __device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds
   const unsigned long long fword = 0x752036140ull;
   const int f = F.term_order ? (int) ((s2f >> (4 * L)) & 0xfu) : L;
   float x = S[(int) ((fword >> (4 * f)) & 0xfu)];
+  if (f == GD_F_ELE)
+    x = x * F.inv_el1;   /* the pair loop sums el1 g */
   if (f == GD_F_KDE)
     x = x / F.kde3;
   float y = x / F.lna_f;

@@ -853,7 +853,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
   float sums[GD_F_NTERM];
   for (int j = 0; j < W; ++j) t[j] = tot[j].vdw;
   sums[GD_F_VDW] = bfly(t);
-  sums[GD_F_ELE] = bfly(eele_tot);
+  sums[GD_F_ELE] = bfly(eele_tot) * c->sc.inv_el1;   /* the loop sums el1 g */
   for (int j = 0; j < W; ++j) t[j] = tot[j].pmf;
   sums[GD_F_PMF] = bfly(t);
   sums[GD_F_HPC] = ehpc;

@@ -174,12 +174,14 @@ def test_canonical_exp_accuracy():
 
 
 def test_canonical_exp_hot_accuracy():
-    """The energy-loop exp: within 1.5 ulp on [-20, 20] (the PMF range),
-    5 ulp down to -86 (where e^x < 1e-17), e^-86 (its value at -86) below."""
+    """The energy-loop exp (degree-5 fit): within 3 ulp on [-20, 20] (the PMF
+    range) with a mean under 1 ulp, 5 ulp down to -86 (where e^x < 1e-17),
+    e^-86 (its value at -86) below."""
     rng = np.random.default_rng(6)
     x = np.concatenate([rng.uniform(-20, 20, 300000), rng.uniform(-1, 1, 100000),
                         [0.0, 18.0]]).astype(np.float32)
-    assert np.max(_ulp_err(O.canon_math(6, x), np.exp(x.astype(np.float64)))) < 1.5
+    err = _ulp_err(O.canon_math(6, x), np.exp(x.astype(np.float64)))
+    assert np.max(err) < 3.0 and np.mean(err) < 1.0
     x = rng.uniform(-86, -20, 300000).astype(np.float32)
     assert np.max(_ulp_err(O.canon_math(6, x), np.exp(x.astype(np.float64)))) < 5.0
     lo = O.canon_math(6, np.array([-86.0], np.float32))[0]
