@@ -173,7 +173,14 @@ typedef struct gd_replica {
   int idx_rep, idx_prt, idx_tmp, idx_lig;
 } gd_replica;
 
-/* McPara (dock.h:460-470) without the output paths */
+/* McPara (dock.h:460-470) without the output paths.  The run follows the
+ * reference's launch loop (gpusingle.cu:8-20): windows of steps_per_dump
+ * steps, each run as whole launches of steps_per_exchange MC steps with an
+ * exchange after each, the exchange mode (0: temperature pairs (0,1),
+ * (2,3), ...; 1: (1,2), ...) alternating from 0 in every window; the last,
+ * partial window runs whole.  Steps run = ceil(steps_total / steps_per_dump)
+ * * ceil(steps_per_dump / steps_per_exchange) * steps_per_exchange.  Both
+ * step sizes must be >= 1 (GD_ERR_ARG otherwise). */
 typedef struct gd_mcpara {
   int steps_total;
   int steps_per_dump;
@@ -382,7 +389,12 @@ typedef struct gd_engine_config {
                           from the resident replica count (several when they
                           are fewer than the device's SIMDs), 1 = one, or 4, 8,
                           16 (one replica per workgroup); the energies are the
-                          same bits for every choice */
+                          same bits for every choice.  One wavefront per
+                          replica is used whatever is asked when a resident
+                          complex has more than 64 position restraints, its
+                          pair list, restraint targets and PSP rows do not fit
+                          in LDS beside the split area, or the KDE points are
+                          kept in HBM */
 } gd_engine_config;
 
 #define GD_TUNE_MAX 16
