@@ -284,12 +284,46 @@ GD_HD float gd_u01(uint32_t w) { return (float) ((w >> 8) + 1u) * 0x1p-24f; }
 /* Counter layout.  key = (seed, complex id).
  *   MC step s of replica r: ctr (r, s, 0, 0) gives u0..u3, ctr (r, s, 1, 0)
  *   gives u4..u7; u0..u5 perturb the six degrees of freedom, u6 is the
- *   Metropolis draw.
+ *   Metropolis draw, u7 the second draw of the reversed test.
  *   Exchange number x of temperature chain c, temperature pair j:
  *   ctr (c, x, 2, j >> 2), element j & 3. */
 #define GD_CTR_MOVE_A 0u
 #define GD_CTR_MOVE_B 1u
 #define GD_CTR_EXCHANGE 2u
+
+/* The reference's draw toggles (toggle.h:9,24).  fixed_draws = IS_RANDOM 0:
+ * every draw is 20 (util_d.cu:106-108), so each move adds 39 x move_scale
+ * (move_d.cu:81-82), every MC step is accepted (accept_d.cu:19-20) and a
+ * temperature pair swaps only when its probability exceeds 20
+ * (exchangereplicas_d.cu:61).  reverse = IS_REVERSE 1 (accept_d.cu:24-27):
+ * the Metropolis test is taken again on a second draw with the exponent's
+ * sign flipped, and that second test decides. */
+#define GD_FIXED_DRAW 20.0f
+
+/* The perturbation of one degree of freedom in [-1, 1] (move_d.cu:81). */
+GD_HD float gd_perturbation(int fixed_draws, uint32_t w)
+{
+  return 2.0f * (fixed_draws ? GD_FIXED_DRAW : gd_u01(w)) - 1.0f;
+}
+
+/* Accept_d's decision (accept_d.cu:17-27): u and u2 the step's two draws,
+ * minus_beta = -1 / T. */
+GD_HD int gd_metropolis(float e_new, float e_old, float minus_beta, float u, float u2,
+                        int fixed_draws, int reverse)
+{
+  if (fixed_draws)
+    return 1;
+  const float delta_energy = e_new - e_old;
+  if (reverse)
+    return u2 < gd_exp(-delta_energy * minus_beta);
+  return u < gd_exp(delta_energy * minus_beta);
+}
+
+/* The draw a temperature pair's swap probability is compared with. */
+GD_HD float gd_exchange_draw(int fixed_draws, uint32_t w)
+{
+  return fixed_draws ? GD_FIXED_DRAW : gd_u01(w);
+}
 
 /* ------------------------------------------------------------------ */
 /* Rigid move (reference move_d.cu:79-132)                              */

@@ -62,6 +62,8 @@ typedef struct gd_pack {
                             exchange mode restarts at 0 in every window */
   int exchange;          /* temperature swaps on/off */
   int away;              /* start 44.5 * move_scale away (IS_AWAY) */
+  int fixed_draws;       /* IS_RANDOM 0: every draw is 20 (gd_perturbation) */
+  int reverse;           /* IS_REVERSE 1: the reversed Metropolis test */
   uint32_t seed, cid;    /* Philox key */
   int *step;             /* [2] device counters: next MC step, exchanges done */
 

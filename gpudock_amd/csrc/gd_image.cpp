@@ -54,6 +54,8 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
   }
   P.exchange = o.exchange;
   P.away = o.away;
+  P.fixed_draws = o.fixed_draws != 0;
+  P.reverse = o.reverse != 0;
   P.seed = o.seed;
   P.cid = o.complex_id;
   if (o.record_stride > 0 && mc) {

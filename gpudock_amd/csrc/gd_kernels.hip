@@ -869,20 +869,23 @@ __device__ __forceinline__ void gd_energy_combine(const gd_pack &P, const gd_lds
   }
 }
 
-/* The draws of step s of replica r (move_d.cu:73-84, accept_d.cu:20-24): the
- * movematrix displacement d[6] and the acceptance draw. */
+/* The draws of step s of replica r (move_d.cu:73-84, accept_d.cu:20-27): the
+ * movematrix displacement d[6] and the two acceptance draws (the second only
+ * for the reversed test). */
 __device__ __forceinline__ void gd_draws(const gd_pack &P, int r, int s, uint32_t seed,
-                                         uint32_t cid, float d[6], float *u_acc)
+                                         uint32_t cid, float d[6], float u_acc[2])
 {
   const gd_u32x4 ra = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_A, 0u, seed, cid);
   const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u, seed, cid);
-  d[0] = P.move_scale[0] * (2.0f * gd_u01(ra.v[0]) - 1.0f);
-  d[1] = P.move_scale[1] * (2.0f * gd_u01(ra.v[1]) - 1.0f);
-  d[2] = P.move_scale[2] * (2.0f * gd_u01(ra.v[2]) - 1.0f);
-  d[3] = P.move_scale[3] * (2.0f * gd_u01(ra.v[3]) - 1.0f);
-  d[4] = P.move_scale[4] * (2.0f * gd_u01(rb.v[0]) - 1.0f);
-  d[5] = P.move_scale[5] * (2.0f * gd_u01(rb.v[1]) - 1.0f);
-  *u_acc = gd_u01(rb.v[2]);
+  const int fx = P.fixed_draws;
+  d[0] = P.move_scale[0] * gd_perturbation(fx, ra.v[0]);
+  d[1] = P.move_scale[1] * gd_perturbation(fx, ra.v[1]);
+  d[2] = P.move_scale[2] * gd_perturbation(fx, ra.v[2]);
+  d[3] = P.move_scale[3] * gd_perturbation(fx, ra.v[3]);
+  d[4] = P.move_scale[4] * gd_perturbation(fx, rb.v[0]);
+  d[5] = P.move_scale[5] * gd_perturbation(fx, rb.v[1]);
+  u_acc[0] = gd_u01(rb.v[2]);
+  u_acc[1] = gd_u01(rb.v[3]);
 }
 
 /* RecordLigand_d (util_d.cu:72-94): the state before step s, one word per
@@ -913,13 +916,14 @@ __device__ __forceinline__ void gd_record(const gd_pack &P, const gd_lds_f32 *ws
 /* Metropolis (accept_d.cu:12-52) on wavefront-uniform energies: the state
  * in the slot, the best pose in HBM; returns 1 if the move was taken. */
 __device__ __forceinline__ int gd_accept(const gd_pack &P, gd_lds_f32 *ws, int r, int s,
-                                         float u_acc, float mb, const float e_new[GD_E_N],
-                                         float cm_new, float *e_old_total, float *best_total)
+                                         const float u_acc[2], float mb,
+                                         const float e_new[GD_E_N], float cm_new,
+                                         float *e_old_total, float *best_total)
 {
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const gd_state S = P.S;
-  const float de = e_new[GD_E_TOTAL] - *e_old_total;
-  if (!(u_acc < gd_exp(de * mb)))
+  if (!gd_metropolis(e_new[GD_E_TOTAL], *e_old_total, mb, u_acc[0], u_acc[1], P.fixed_draws,
+                     P.reverse))
     return 0;
   *e_old_total = e_new[GD_E_TOTAL];
   if (lane == 0) {
@@ -998,8 +1002,8 @@ __device__ __forceinline__ void gd_replica_cycle(const gd_pack &P, const char *s
   for (int s = s0; s < s0 + n_steps; ++s) {
     if (P.rec != nullptr && s % P.rec_stride == 0)
       gd_record(P, ws, r, itmp, s);
-    float d[6], u_acc;
-    gd_draws(P, r, s, seed, cid, d, &u_acc);
+    float d[6], u_acc[2];
+    gd_draws(P, r, s, seed, cid, d, u_acc);
     float m_new[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
@@ -1090,7 +1094,7 @@ __device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const c
   float e_old_total = 0.0f, best_total = 0.0f, mb = 0.0f;
   int itmp = 0, acs = 0, cur = 0;
   gd_lds_f32 *conf = A + GD_SA_CONF;
-  float d[6], u_acc = 0.0f;
+  float d[6], u_acc[2] = {0.0f, 0.0f};
   gd_final_consts F;
   const uint32_t seed = P.seed, cid = P.cid;
   if (lead) {
@@ -1113,7 +1117,7 @@ __device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const c
     }
     /* the pose of the first step, in buffer 0 */
     float m[6];
-    gd_draws(P, r, s0, seed, cid, d, &u_acc);
+    gd_draws(P, r, s0, seed, cid, d, u_acc);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
       m[k] = d[k] + ws[GD_WS_MOLD + k];
@@ -1142,10 +1146,10 @@ __device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const c
      * rejected, ib if it is accepted */
     const int ia = sel == 2 ? 0 : sel + 1;
     const int ib = ia == 2 ? 0 : ia + 1;
-    float u_next = 0.0f, edst = 0.0f;
+    float u_next[2] = {0.0f, 0.0f}, edst = 0.0f;
     if (lead) {
       if (s + 1 < s0 + n_steps) {
-        gd_draws(P, r, s + 1, seed, cid, d, &u_next);
+        gd_draws(P, r, s + 1, seed, cid, d, u_next);
         float ma[6], mbv[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -1172,7 +1176,8 @@ __device__ __forceinline__ void gd_replica_cycle_split(const gd_pack &P, const c
                                  &best_total);
       acs += took;
       cur = took ? ib : ia;
-      u_acc = u_next;
+      u_acc[0] = u_next[0];
+      u_acc[1] = u_next[1];
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -1257,12 +1262,13 @@ __device__ __forceinline__ void gd_exchange_wave(const gd_pack *__restrict__ pac
                                            (uint32_t) (j >> 2), P->seed, P->cid);
       const int q = j & 3;
       const uint32_t uw = q == 0 ? u4.v[0] : q == 1 ? u4.v[1] : q == 2 ? u4.v[2] : u4.v[3];
+      const float u = gd_exchange_draw(P->fixed_draws, uw);
       const int i1 = 2 * j + mode;
       const int i2 = i1 + 1;
       const int o1 = s_orders[i1];
       const int o2 = s_orders[i2];
       const float delta = (P->minus_beta[i1] - P->minus_beta[i2]) * (s_ener[o2] - s_ener[o1]);
-      if (gd_exp(delta) > gd_u01(uw)) {
+      if (gd_exp(delta) > u) {
         const gd_state S = P->S;
         const int a1 = base + n_lig * o1;
         const int a2 = base + n_lig * o2;

@@ -103,7 +103,8 @@ class Options(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("complex_id", C.c_uint32), ("exchange", C.c_int),
                 ("away", C.c_int), ("term_order", C.c_int), ("device", C.c_int),
                 ("record_stride", C.c_int), ("record_prefix", C.c_char_p),
-                ("mcc", C.c_int), ("combine", C.c_int)]
+                ("mcc", C.c_int), ("combine", C.c_int), ("fixed_draws", C.c_int),
+                ("reverse", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -519,10 +520,11 @@ def optimize_enepara(ep0):
 
 
 def default_options(seed=1, complex_id=0, exchange=1, away=0, term_order=0, device=0,
-                    record_stride=0, record_prefix=None, mcc=0, combine=0):
+                    record_stride=0, record_prefix=None, mcc=0, combine=0, fixed_draws=0,
+                    reverse=0):
     return Options(seed, complex_id, exchange, away, term_order, device, record_stride,
                    record_prefix.encode() if isinstance(record_prefix, str) else record_prefix,
-                   mcc, combine)
+                   mcc, combine, fixed_draws, reverse)
 
 
 def mcpara(steps_total, steps_per_exchange, move_scale, steps_per_dump=None):

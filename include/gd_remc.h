@@ -218,6 +218,14 @@ typedef struct gd_options {
                            each pose's contacts against the contacts of
                            replica 0's ligand at its start pose */
   int combine;          /* enum gd_combine: the total energy */
+  int fixed_draws;      /* IS_RANDOM == 0 (toggle.h:9) when 1: the reference's
+                           deterministic validation mode -- every draw is 20,
+                           so each move adds 39 x move_scale, every MC step is
+                           accepted and a temperature pair swaps only when its
+                           probability exceeds 20 (util_d.cu:106-108) */
+  int reverse;          /* IS_REVERSE (toggle.h:24) when 1: the Metropolis test
+                           is taken on a second draw with the exponent's sign
+                           flipped (accept_d.cu:24-27) */
 } gd_options;
 
 /* LigRecordSingleStep (dock.h:487-494): the state of one replica slot

@@ -380,6 +380,14 @@ ORC_EXPORT int orc_accept(int canonical, float e_new, float e_old, float minus_b
   return u < orc_expf(canonical, delta_energy * minus_beta);
 }
 
+/* Accept_d with the reference's draw toggles in the canonical arithmetic
+ * (gd_metropolis; accept_d.cu:17-27): u and u2 the step's two draws. */
+ORC_EXPORT int orc_metropolis(float e_new, float e_old, float minus_beta, float u, float u2,
+                              int fixed_draws, int reverse)
+{
+  return gd_metropolis(e_new, e_old, minus_beta, u, u2, fixed_draws, reverse);
+}
+
 /* ExchangeReplicas_d for one temperature chain (exchangereplicas_d.cu:28-78).
  * temps[t]: temperature index held by chain slot t (updated in place);
  * ener[t]: its total energy; minus_beta[i]: of temperature i; mode_t 0 pairs
@@ -440,15 +448,15 @@ ORC_EXPORT int orc_schedule(int steps_total, int steps_per_dump, int steps_per_e
 }
 
 /* The exchange draws of chain c after launch x: Philox counter (c, x, 2,
- * j >> 2), element j & 3, for pair j. */
+ * j >> 2), element j & 3, for pair j (20 for every pair with fixed_draws). */
 static void exchange_draws(uint32_t c, uint32_t x, int n_pair, uint32_t seed, uint32_t cid,
-                           float *u)
+                           int fixed_draws, float *u)
 {
   gd_u32x4 u4;
   for (int j = 0; j < n_pair; ++j) {
     if ((j & 3) == 0)
       u4 = gd_philox4x32_10(c, x, GD_CTR_EXCHANGE, (uint32_t) (j >> 2), seed, cid);
-    u[j] = gd_u01(u4.v[j & 3]);
+    u[j] = gd_exchange_draw(fixed_draws, u4.v[j & 3]);
   }
 }
 
@@ -546,7 +554,7 @@ static int ref_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp *ps
           ener[t] = e[(size_t) (base + sz.n_lig * t) * 10 + 9];
           acx[t] = 0;
         }
-        exchange_draws((uint32_t) c, (uint32_t) x, sz.n_tmp / 2, seed, cid, u);
+        exchange_draws((uint32_t) c, (uint32_t) x, sz.n_tmp / 2, seed, cid, 0, u);
         orc_exchange_chain(0, sz.n_tmp, mbs, temps, ener, modes[x], u, acx);
         for (int t = 0; t < sz.n_tmp; ++t) {
           it[base + sz.n_lig * t] = temps[t];
@@ -1000,13 +1008,16 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
                                              opt->seed, opt->complex_id);
         const gd_u32x4 rb = gd_philox4x32_10((uint32_t) r, (uint32_t) s, GD_CTR_MOVE_B, 0u,
                                              opt->seed, opt->complex_id);
-        const uint32_t w[7] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3], rb.v[0], rb.v[1], rb.v[2]};
+        const uint32_t w[8] = {ra.v[0], ra.v[1], ra.v[2], ra.v[3],
+                               rb.v[0], rb.v[1], rb.v[2], rb.v[3]};
         for (int k = 0; k < 6; ++k)
-          mn[k] = mc->move_scale[k] * (2.0f * gd_u01(w[k]) - 1.0f) + R->move_final[k];
+          mn[k] = mc->move_scale[k] * gd_perturbation(opt->fixed_draws, w[k]) +
+                  R->move_final[k];
         rp_place(&c, lc[r], mn, ax, ay, az, cen);
         float cmn = 0.0f;
         rp_energy(&c, pc[r], ax, ay, az, cen, en, ref, &cmn);
-        if (orc_accept(1, en[GD_E_TOTAL], R->energy_final[GD_E_TOTAL], mb, gd_u01(w[6]))) {
+        if (gd_metropolis(en[GD_E_TOTAL], R->energy_final[GD_E_TOTAL], mb, gd_u01(w[6]),
+                          gd_u01(w[7]), opt->fixed_draws, opt->reverse)) {
           R->acs_mc++;
           memcpy(R->move_final, mn, sizeof mn);
           memcpy(R->energy_final, en, sizeof en);
@@ -1035,7 +1046,8 @@ ORC_EXPORT int orc_replay_run_rec(const gd_ligand *lig, const gd_protein *prt,
           ener[t] = res[base + sz.n_lig * t].energy_final[GD_E_TOTAL];
           acx[t] = 0;
         }
-        exchange_draws((uint32_t) ch, (uint32_t) x, sz.n_tmp / 2, opt->seed, opt->complex_id, u);
+        exchange_draws((uint32_t) ch, (uint32_t) x, sz.n_tmp / 2, opt->seed, opt->complex_id,
+                       opt->fixed_draws, u);
         orc_exchange_chain(1, sz.n_tmp, mbs, temps, ener, modes[x], u, acx);
         for (int t = 0; t < sz.n_tmp; ++t) {
           it[base + sz.n_lig * t] = temps[t];

@@ -46,6 +46,7 @@ def lib():
                                                 C.c_uint32, C.c_uint32, C.c_int, C.c_int, IP,
                                                 IP, FP, IP, C.c_int, C.c_int]
         L.orc_accept.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.orc_metropolis.argtypes = [C.c_float] * 5 + [C.c_int, C.c_int]
         L.orc_exchange_chain.argtypes = [C.c_int, C.c_int, FP, IP, FP, C.c_int, FP, IP]
         L.orc_schedule.argtypes = [C.c_int, C.c_int, C.c_int, IP, C.c_int]
         L.orc_bayes_pdf.argtypes = [C.c_int, C.c_int, FP, C.c_float, C.c_float, FP]
@@ -166,6 +167,13 @@ def ref_run_samples(cx, temps, scale, n_steps, spe, stride, exchange=0, seed=1, 
 def accept(e_new, e_old, minus_beta, u, canonical):
     """The oracle's Metropolis decision (Accept_d, accept_d.cu:22-23)."""
     return bool(lib().orc_accept(int(canonical), e_new, e_old, minus_beta, u))
+
+
+def metropolis(e_new, e_old, minus_beta, u, u2, fixed_draws=0, reverse=0):
+    """The canonical decision with the reference's draw toggles (IS_RANDOM,
+    IS_REVERSE; accept_d.cu:17-27)."""
+    return bool(lib().orc_metropolis(e_new, e_old, minus_beta, u, u2, int(fixed_draws),
+                                     int(reverse)))
 
 
 def exchange_chain(minus_beta, temps, ener, mode, u, canonical):

@@ -112,6 +112,67 @@ def test_metropolis_edges(canonical):
     assert not O.accept(200.0, 0.0, -1.0, 2.0 ** -24, canonical)
 
 
+def test_reversed_metropolis_closed_form():
+    """IS_REVERSE (accept_d.cu:24-27): the second draw decides, against
+    exp(-(E_new - E_old) * minus_beta); the first draw plays no part."""
+    rng = np.random.default_rng(12)
+    n_checked = 0
+    for _ in range(3000):
+        T = rng.uniform(0.05, 5.0)
+        mb = _f32(-1.0 / T)
+        e_old = _f32(rng.uniform(-20.0, 20.0))
+        e_new = _f32(e_old + rng.normal(0.0, 3.0 * T))
+        de = np.float32(e_new) - np.float32(e_old)
+        p = math.exp(-float(de) * mb)
+        for u in (_f32(2.0 ** -24), _f32(1.0)):
+            if p >= 1.0:
+                assert O.metropolis(e_new, e_old, mb, u, _f32(1.0 - 2.0 ** -24), reverse=1)
+                continue
+            if p < 2.0 ** -20:
+                continue
+            assert O.metropolis(e_new, e_old, mb, u, _f32(p * (1.0 - 1e-4)), reverse=1)
+            assert not O.metropolis(e_new, e_old, mb, u, _f32(p * (1.0 + 1e-4)), reverse=1)
+            n_checked += 1
+    assert n_checked > 1000
+    # the reversed test takes uphill moves the plain one refuses, and back
+    assert O.metropolis(1.0, 0.0, -5.0, 1.0, 0.5, reverse=1)
+    assert not O.metropolis(1.0, 0.0, -5.0, 0.5, 1.0)
+    assert not O.metropolis(-1.0, 0.0, -5.0, 1.0, 0.5, reverse=1)
+
+
+def test_fixed_draws_accept_everything():
+    """IS_RANDOM 0 (accept_d.cu:19-20): every step is accepted, whatever the
+    energies and draws; the move perturbation is 2 x 20 - 1 = 39 (move_d.cu:81)."""
+    for e_new, e_old in ((1e6, -1e6), (0.0, 0.0), (-3.0, 5.0)):
+        for u in (2.0 ** -24, 1.0):
+            assert O.metropolis(e_new, e_old, -1.0, u, u, fixed_draws=1)
+            assert O.metropolis(e_new, e_old, -1.0, u, u, fixed_draws=1, reverse=1)
+
+
+def test_fixed_draws_replay_moves_deterministically():
+    """The replay with IS_RANDOM 0: every step accepted, each movematrix the
+    float32 sum of its start and step x (39 x move_scale), and a temperature
+    pair swaps only when exp(delta) > 20 (exchangereplicas_d.cu:61)."""
+    cx, temps, mc, cfg = synth.make_config("c0", lna=8, pnp=60, pnk=40, n_lig=2, n_prt=1,
+                                           n_tmp=4, steps=30, spe=10)
+    opt = remc.default_options(seed=4, fixed_draws=1)
+    r = O.replay_run(cx, temps, mc, opt)
+    n_rep = 2 * 4
+    assert np.all(r["acs_mc"] == 30)
+    scale = np.asarray(mc.move_scale, np.float32)
+    step = scale * np.float32(39.0)
+    m = np.zeros(6, np.float32)
+    for _ in range(30):
+        m = step + m
+    for i in range(n_rep):
+        np.testing.assert_array_equal(r["move_final"][i], m)
+    # every pose of a replica is the same on every step, so the exchanges
+    # follow from the energies alone: swap iff exp((mb1 - mb2)(E2 - E1)) > 20
+    r2 = O.replay_run(cx, temps, mc, remc.default_options(seed=99, fixed_draws=1))
+    np.testing.assert_array_equal(r["idx_tmp_final"], r2["idx_tmp_final"])
+    np.testing.assert_array_equal(r["acs_exchg"], r2["acs_exchg"])
+
+
 # ------------------------------------------------------------------ exchange
 def _always(n):
     """Draws that every pair with equal energies passes: exp(0) = 1 > 0.5."""

@@ -166,6 +166,24 @@ def test_run_variants(n_tmp, exchange, away, term_order, steps, spe):
     _compare_runs(g, r)
 
 
+@pytest.mark.parametrize("fixed_draws,reverse", [(1, 0), (0, 1), (1, 1)])
+def test_draw_toggles(fixed_draws, reverse):
+    """The reference's IS_RANDOM 0 (every draw 20: fixed moves, every step
+    accepted, swaps only above probability 20) and IS_REVERSE (the second
+    draw against the sign-flipped exponent) toggles (toggle.h:9,24): every
+    decision equals the replay's, one wavefront per replica or split."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=2, n_prt=2, n_tmp=4, steps=60)
+    opt = remc.default_options(seed=8, fixed_draws=fixed_draws, reverse=reverse)
+    r = O.replay_run(cx, temps, mc, opt)
+    g, _ = remc.run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    res, rep = remc.batch_run([remc.JobSpec(cx, temps, mc, opt)], split=1)
+    assert rep["layout"]["split"] == 1
+    _compare_runs(res[0], r)
+    if fixed_draws:
+        assert np.all(g["acs_mc"] == 60)
+
+
 @pytest.mark.parametrize("n_lig,n_prt,n_tmp", [(4, 8, 7), (8, 5, 4), (17, 3, 2)])
 def test_exchange_many_chains(n_lig, n_prt, n_tmp):
     """32 chains are exchanged by one workgroup, more by many (the last to
