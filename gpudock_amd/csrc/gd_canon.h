@@ -416,15 +416,17 @@ GD_HD float gd_dist2(float dx, float dy, float dz)
   return gd_fma(dz, dz, gd_fma(dy, dy, dx * dx));
 }
 
-/* hydrophobic contact kernel 1 - (3.5 u - 4.5 u^2 + 2.5 u^3 - 0.5 u^4),
- * u = d^2 / 81, written as a polynomial in d^2 */
-GD_HD float gd_hpc_pair(float d2, float hpp)
+/* Hydrophobic contact kernel of one pair (calcenergy_d.cu:75-81):
+ * hpp (1 - (3.5 u - 4.5 u^2 + 2.5 u^3 - 0.5 u^4)) for u = d^2 / 81 <= 1, else
+ * 0.  The polynomial is (1 - u)^3 (1 - u / 2) = w^3 (1 + w) / 2, w = 1 - u:
+ * with w clamped at 0 the value is an exact 0 beyond the cut, so no test is
+ * needed.  The pair adds twice the kernel, hacc + w^3 hpp (1 + w); the
+ * caller halves the atom's sum once (an exact scaling). */
+GD_HD float gd_hpc_acc2(float d2, float hpp, float hacc)
 {
-  float t = gd_fma(d2, -0x1.8f1924p-27f, 0x1.3bb164p-18f);
-  t = gd_fma(d2, t, -0x1.67980ep-11f);
-  t = gd_fma(d2, t, 0x1.61f9aep-5f);
-  t = d2 * t;
-  return gd_fma(-hpp, t, hpp);
+  const float w = __builtin_fmaxf(gd_fma(d2, -0x1.948b10p-7f, 1.0f), 0.0f);   /* 1/81 */
+  const float w3 = (w * w) * w;
+  return gd_fma(w3, gd_fma(w, hpp, hpp), hacc);
 }
 
 /* The five directly summed pair terms (ele: times el1, see

@@ -181,8 +181,8 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     I.kde_seg[2 * r + 1] = (int) I.kde.size();
   }
   P.pnk = (int) I.kde.size();
-  if (P.pnk > 0xffff)
-    return gd_fail(GD_ERR_ARG, "%d KDE points of the ligand's atom types (at most 65535)",
+  if (P.pnk > 0xfffe)
+    return gd_fail(GD_ERR_ARG, "%d KDE points of the ligand's atom types (at most 65534)",
                    P.pnk);
   I.kde_inv.resize(P.n_rows);
   for (int r = 0; r < P.n_rows; ++r) {
@@ -193,6 +193,10 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     for (int k = I.kde_seg[2 * r]; k < I.kde_seg[2 * r + 1]; ++k)
       I.kde[k].w = I.kde_inv[r];
   }
+  /* one more point after the segments, of weight 0: the padding pairs name
+   * it, so they add exact zeros and need no mask */
+  I.kde.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  const uint32_t kde_pad = (uint32_t) P.pnk << 16;
   /* the pose's (atom, KDE point) pairs, atoms < 64 then atoms >= 64 (lane
    * a holds atoms a and a + 64), each group padded to whole wavefronts */
   I.kde_pairs.clear();
@@ -207,10 +211,10 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     }
     const int n = (int) (I.kde_pairs.size() - start);
     (g ? P.kde_n_hi : P.kde_n_lo) = n;
-    I.kde_pairs.resize(start + (size_t) (n + GD_WAVE - 1) / GD_WAVE * GD_WAVE, 0u);
+    I.kde_pairs.resize(start + (size_t) (n + GD_WAVE - 1) / GD_WAVE * GD_WAVE, kde_pad);
   }
   if (I.kde_pairs.empty())
-    I.kde_pairs.push_back(0u);
+    I.kde_pairs.push_back(kde_pad);
 
   I.mcs.resize((size_t) sz.pos * sz.lna);
   I.tcc.resize(sz.pos);
@@ -374,7 +378,7 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
     P.off_tab = 0;
   }
   P.off_kde = al16(P.off_tab + tab_bytes);
-  const int with_kde = al16(P.off_kde + P.pnk * 16);
+  const int with_kde = al16(P.off_kde + (P.pnk + 1) * 16);   /* with the padding point */
   if (kde_policy != GD_KDE_GLOBAL && with_kde + reserve <= max_lds) {
     P.kde_in_lds = 1;
     P.lds_bytes = with_kde;

@@ -133,8 +133,8 @@ __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
     s_tab[i] = g_tab[i];
   if (P.kde_in_lds) {
     float4 *s_kde = (float4 *) (smem + P.off_kde);
-    const int pnk = P.pnk;
-    for (int i = threadIdx.x; i < pnk; i += blockDim.x)
+    const int n_kde = P.pnk + 1;   /* with the zero-weight padding point */
+    for (int i = threadIdx.x; i < n_kde; i += blockDim.x)
       s_kde[i] = P.kde[i];
   }
 }
@@ -255,8 +255,8 @@ __device__ __forceinline__ void gd_atoms_block(
                 psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
         }
         const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-        if (decltype(hpc_tag)::value && d2 <= 81.0f)
-          hacc += gd_hpc_pair(d2, eh.y);
+        if (decltype(hpc_tag)::value)
+          hacc = gd_hpc_acc2(d2, eh.y, hacc);
         const gd_pair_param pp =
             *(const gd_pair_param *) ((const char *) trow + bits);
         const int near_any = __ballot(d2 < sc_v.dcut2) != 0;
@@ -303,7 +303,7 @@ __device__ __forceinline__ void gd_atoms_block(
       });
     }
     eele = gd_fma(lq, acc.ele, eele);
-    const float hs = gd_wave_sum(hacc);
+    const float hs = 0.5f * gd_wave_sum(hacc);   /* the pairs added twice the kernel */
     const float z = (hs - ar.hpl0) * ar.inv_hpl1;
     ehpc += 0.5f * z * z - ar.hpl2;
   }
@@ -320,7 +320,9 @@ __device__ __forceinline__ void gd_atoms_block(
  * padded to whole rounds -- so that no lane idles at the end of an atom's
  * short segment.  Lane j of round r takes pair 64 r + j: its atom's
  * coordinates come from the lane that holds the atom, the point carries the
- * pair's weight 1 / (segment length) in .w.  Returns this lane's sum over
+ * pair's weight 1 / (segment length) in .w; the padding pairs name a point
+ * of weight 0 after the segments, so they add exact zeros (the sums are
+ * never -0) and every lane keeps its sum.  Returns this lane's sum over
  * rounds [r0, r1) from +0. */
 template <bool XL>
 __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *smem, float ax0,
@@ -336,28 +338,28 @@ __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *sme
       (void *) P.kde_pairs, 0, n_all * 4, GD_BUFFER_DWORD3);
   float lane_sum = 0.0f;
   auto rounds = [&](auto read) {
-    for (int r = r0; r < r1; ++r) {
-      const bool hi = r >= rounds_lo;
-      /* pairs of this round's group left from its first lane on */
-      const int left = hi ? n_hi - (r - rounds_lo) * GD_WAVE : n_lo - r * GD_WAVE;
-      /* every lane runs the permutes (padding entries name lane 0 and
-       * point 0); only the lanes with a pair keep the sum */
-      const uint32_t e =
-          XL ? ((const __attribute__((address_space(3))) uint32_t *) (smem + P.off_pairs))
-                   [r * GD_WAVE + ulane]
-             : __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u, r * GD_WAVE * 4, 0);
-      const int src = (int) (e & 0xffu);
-      const float gx = hi ? ax1 : ax0, gy = hi ? ay1 : ay0, gz = hi ? az1 : az0;
-      const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
-      const float ly = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gy)));
-      const float lz = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gz)));
-      const gd_f4 q = read(e >> 12);   /* byte offset of the point */
-      const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-      float es;
-      const float ep = gd_exp_hot_split(kde2 * d2, &es);
-      const float sum = gd_fma(ep * es, q.w, lane_sum);   /* ep * 2^n is exact */
-      lane_sum = (int) ulane < left ? sum : lane_sum;
-    }
+    /* rounds [ra, rb) of one group, whose atoms lanes hold in (gx, gy, gz);
+     * padding entries name lane 0 and the zero-weight point */
+    auto group = [&](int ra, int rb, float gx, float gy, float gz) {
+      for (int r = ra; r < rb; ++r) {
+        const uint32_t e =
+            XL ? ((const __attribute__((address_space(3))) uint32_t *) (smem + P.off_pairs))
+                     [r * GD_WAVE + ulane]
+               : __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
+                                                      r * GD_WAVE * 4, 0);
+        const int src = (int) (e & 0xffu);
+        const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
+        const float ly = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gy)));
+        const float lz = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gz)));
+        const gd_f4 q = read(e >> 12);   /* byte offset of the point */
+        const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+        float es;
+        const float ep = gd_exp_hot_split(kde2 * d2, &es);
+        lane_sum = gd_fma(ep * es, q.w, lane_sum);   /* ep * 2^n is exact */
+      }
+    };
+    group(r0, r1 < rounds_lo ? r1 : rounds_lo, ax0, ay0, az0);
+    group(r0 > rounds_lo ? r0 : rounds_lo, r1, ax1, ay1, az1);
   };
   if (XL || P.kde_in_lds) {
     const __attribute__((address_space(3))) char *pts =
@@ -368,7 +370,7 @@ __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *sme
   } else {
     /* from HBM as a raw buffer */
     const __amdgpu_buffer_rsrc_t kde_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *) P.kde, 0, P.pnk * 16, GD_BUFFER_DWORD3);
+        (void *) P.kde, 0, (P.pnk + 1) * 16, GD_BUFFER_DWORD3);
     rounds([&](uint32_t off) {
       return __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(kde_rsrc, off, 0,
                                                                              0));

@@ -759,8 +759,8 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
       for (int p = j; p < pnp; p += W) {
         const size_t q = po + c->perm[po + p];
         const float d2 = gd_dist2(lx - prt->x[q], ly - prt->y[q], lz - prt->z[q]);
-        if (prt->c0_and_d12_or_c2[q] && d2 <= 81.0f)
-          hacc[j] += gd_hpc_pair(d2, prt->hpp[q]);
+        if (prt->c0_and_d12_or_c2[q])
+          hacc[j] = gd_hpc_acc2(d2, prt->hpp[q], hacc[j]);
         const int pt = prt->t[q];
         gd_pair_param pp;
         pp.p1a = ep->p1a[lt][pt]; pp.p2a = ep->p2a[lt][pt];
@@ -781,7 +781,7 @@ static void rp_energy(const rp_ctx *c, int prt_conf, const float *ax, const floa
     }
     for (int j = 0; j < W; ++j)
       eele[j] = gd_fma(lq, acc[j].ele, eele[j]);
-    const float hs = bfly(hacc);
+    const float hs = 0.5f * bfly(hacc);   /* the pairs added twice the kernel */
     const float z = (hs - ep->hpl0[lt]) * (1.0f / ep->hpl1[lt]);
     ehpc_b += 0.5f * z * z - ep->hpl2[lt];
   }
