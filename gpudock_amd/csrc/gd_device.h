@@ -15,6 +15,8 @@
 #include "gd_canon.h"
 
 #define GD_WAVE 64
+/* bytes of one staged 64-point pocket chunk: three 8-byte planes */
+#define GD_LDS_CHUNK_BYTES (3 * 8 * GD_WAVE)
 
 /* Replica state of one complex in HBM, structure of arrays over its n_rep
  * replicas (local replica index r). */
@@ -67,8 +69,9 @@ typedef struct gd_pack {
   uint32_t seed, cid;    /* Philox key */
   int *step;             /* [2] device counters: next MC step, exchanges done */
 
-  /* LDS carve-up for this complex (bytes) */
-  int off_eh, off_tab, off_kde, lds_bytes;
+  /* LDS carve-up for this complex (bytes); the pocket, when staged, starts
+   * at 0 in 64-point chunks of GD_LDS_CHUNK_BYTES (gd_stage) */
+  int off_tab, off_kde, lds_bytes;
   int kde_in_lds;
   int pocket_in_lds;     /* 0: pocket points read through the caches */
   /* launches that split a replica over several wavefronts (one replica per

@@ -363,18 +363,17 @@ int gd_build_image(GdImage &I, const gd_ligand *lig, const gd_protein *prt,
 
   /* LDS carve-up */
   auto al16 = [](int v) { return (v + 15) & ~15; };
-  /* pocket points staged as two planes, positions + type (16 B) and
-   * ele + hpp (8 B): contiguous per lane, so free of bank conflicts; a
-   * pocket too large for LDS beside the table rows is read through the
-   * caches instead (whole-protein pockets) */
+  /* pocket points staged in 64-point chunks of three 8-byte planes,
+   * {x, y}, {z, type}, {ele, hpp}: contiguous per lane, so free of bank
+   * conflicts, and one address per chunk; a pocket too large for LDS
+   * beside the table rows is read through the caches instead
+   * (whole-protein pockets) */
   const int reserve = 16 * GD_WSLOT_FLOATS * 4 + 64;   /* wave slots of up to 16 waves */
   const int tab_bytes = P.n_rows * GD_ROW_STRIDE * (int) sizeof(gd_pair_param);
   P.pocket_in_lds = 1;
-  P.off_eh = pnp_pad * 16;
-  P.off_tab = al16(P.off_eh + pnp_pad * 8);
+  P.off_tab = (pnp_pad / GD_WAVE) * GD_LDS_CHUNK_BYTES;
   if (P.off_tab + tab_bytes + reserve > max_lds) {
     P.pocket_in_lds = 0;
-    P.off_eh = 0;
     P.off_tab = 0;
   }
   P.off_kde = al16(P.off_tab + tab_bytes);

@@ -87,19 +87,21 @@ __device__ __forceinline__ float gd_dpp(float v)
 /* Sum over the 64 lanes with the pairing tree of an xor butterfly taken
  * near-first (lane pairs j^1, then j^2, j^4, j^8, j^16, j^32): DPP
  * quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror (pairs the two quads
- * of a half-row), row_mirror (pairs the two halves of a row), then the four
- * row sums as (r0 + r1) + (r2 + r3).  Every partial is the same float
- * addition as in that butterfly (addition commutes), so the result is
- * bit-identical to it; it is returned uniform. */
+ * of a half-row), row_mirror (pairs the two halves of a row) leave every
+ * lane of row i with the row sum r_i; row_bcast:15 adds r0 to row 1 and r2
+ * to row 3, row_bcast:31 adds row 1's (r0 + r1) to row 3's (r2 + r3), read
+ * from lane 63 (what the other rows receive is not read).  Every partial is the same float addition
+ * as in that butterfly (addition commutes), so the result is bit-identical
+ * to it; it is returned uniform. */
 __device__ __forceinline__ float gd_wave_sum(float v)
 {
   v += gd_dpp<0xB1>(v);
   v += gd_dpp<0x4E>(v);
   v += gd_dpp<0x141>(v);
   v += gd_dpp<0x140>(v);
-  const float r0 = gd_bcast(v, 0), r1 = gd_bcast(v, 16);
-  const float r2 = gd_bcast(v, 32), r3 = gd_bcast(v, 48);
-  return (r0 + r1) + (r2 + r3);
+  v += gd_dpp<0x142>(v);   /* rows 1 and 3 are the ones read */
+  v += gd_dpp<0x143>(v);   /* row 3 is the one read */
+  return gd_bcast(v, 63);
 }
 
 __device__ __forceinline__ double gd_wave_sum64(double v)
@@ -115,17 +117,19 @@ __device__ __forceinline__ double gd_wave_sum64(double v)
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void gd_stage(const gd_pack &P, int prt, char *smem)
 {
-  float4 *s_pos = (float4 *) smem;
-  float2 *s_eh = (float2 *) (smem + P.off_eh);
   float4 *s_tab = (float4 *) (smem + P.off_tab);
   const int pnp_pad = P.pnp_pad;
   const float4 *g_pos = P.pt_pos + (size_t) prt * pnp_pad;
   const float2 *g_eh = P.pt_eh + (size_t) prt * pnp_pad;
+  /* chunk c of 64 points: {x, y} of its points, then {z, type}, then
+   * {ele, hpp}, 512 B each */
   if (P.pocket_in_lds)
     for (int i = threadIdx.x; i < pnp_pad; i += blockDim.x) {
       const float4 q = g_pos[i];
-      s_pos[i] = q;
-      s_eh[i] = g_eh[i];
+      float2 *c = (float2 *) (smem + (i / GD_WAVE) * GD_LDS_CHUNK_BYTES) + (i & (GD_WAVE - 1));
+      c[0] = make_float2(q.x, q.y);
+      c[GD_WAVE] = make_float2(q.z, q.w);
+      c[2 * GD_WAVE] = g_eh[i];
     }
   const float4 *g_tab = (const float4 *) P.tab;
   const int ntab4 = P.n_rows * GD_ROW_STRIDE * 2;
@@ -184,8 +188,6 @@ __device__ __forceinline__ void gd_atoms_block(
     float ay1, float az1, int l0, int l1, gd_blk &blk, float *ehpc_b, int &tp, int &fp,
     int &fn)
 {
-  const float4 *__restrict__ s_pos = (const float4 *) smem;
-  const float2 *__restrict__ s_eh = (const float2 *) (smem + P.off_eh);
   const gd_pair_param *__restrict__ s_tab = (const gd_pair_param *) (smem + P.off_tab);
   const int lane = threadIdx.x & (GD_WAVE - 1);
   const int lna = P.lna;
@@ -287,8 +289,11 @@ __device__ __forceinline__ void gd_atoms_block(
     };
     if (XL || pocket_lds) {
       rows([&](int k, float4 &q, float2 &eh) {
-        q = s_pos[lane + k * GD_WAVE];
-        eh = s_eh[lane + k * GD_WAVE];
+        const float2 *__restrict__ c =
+            (const float2 *) (smem + k * GD_LDS_CHUNK_BYTES) + lane;
+        const float2 xy = c[0], zw = c[GD_WAVE];
+        q = make_float4(xy.x, xy.y, zw.x, zw.y);
+        eh = c[2 * GD_WAVE];
       });
     } else {
       /* whole-protein pockets: the conformer's points as raw buffers, lane
