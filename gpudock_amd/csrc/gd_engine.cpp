@@ -1687,3 +1687,54 @@ extern "C" int gd_valu_probe(int device, int waves_per_simd, double *ns_per_inst
   *n_simd = n_cu * 4;
   return GD_OK;
 }
+
+static const char *const gd_valu_kind_names[GD_VALU_KINDS] = {
+    "v_fmac_f32 (vector operands)", "v_fma_f32 (one scalar-register operand)", "v_mul_f32",
+    "v_max_f32", "v_cndmask_b32 (scalar-register mask)", "v_lshl_add_u32",
+    "v_cmp_gt_f32 (to vcc)", "v_add_f32_dpp", "v_readlane_b32",
+    "v_fmac_f32 and v_max_f32 alternating"};
+
+extern "C" const char *gd_valu_kind_name(int kind)
+{
+  return kind >= 0 && kind < GD_VALU_KINDS ? gd_valu_kind_names[kind] : nullptr;
+}
+
+extern "C" int gd_valu_probe_kinds(int device, int waves_per_simd, double *ns_per_inst,
+                                   int n_kinds)
+{
+  if (!ns_per_inst || n_kinds != GD_VALU_KINDS || waves_per_simd < 1 || waves_per_simd > 8)
+    return gd_fail(GD_ERR_ARG, "waves_per_simd must be 1..8, n_kinds %d and ns_per_inst "
+                               "non-null", GD_VALU_KINDS);
+  if (gd_device_count() <= 0)
+    return gd_fail(GD_ERR_NODEV, "no HIP device");
+  GD_HIP_TRY(hipSetDevice(device));
+  int n_cu = 0;
+  GD_HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+  const int blocks = n_cu * waves_per_simd;
+  const int iters = 200;
+  float *out = nullptr;
+  GD_HIP_TRY(hipMalloc(&out, (size_t) blocks * 256 * sizeof(float)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int k = 0; k < GD_VALU_KINDS && e == hipSuccess; ++k) {
+    e = gd_launch_valu_kind(k, out, blocks, 8, 0);   /* warm-up */
+    if (e == hipSuccess) e = hipEventRecord(e0, 0);
+    if (e == hipSuccess) e = gd_launch_valu_kind(k, out, blocks, iters, 0);
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    /* wavefront instructions per SIMD: 8 chains x 16 rounds per iteration,
+     * two instructions per chain step in the alternating kind */
+    const double per_simd = (double) blocks * 4.0 / (n_cu * 4.0) * iters * 16.0 * 8.0 *
+                            (k == GD_VALU_KINDS - 1 ? 2.0 : 1.0);
+    ns_per_inst[k] = (double) ms * 1e6 / per_simd;
+  }
+  if (e0) (void) hipEventDestroy(e0);
+  if (e1) (void) hipEventDestroy(e1);
+  (void) hipFree(out);
+  if (e != hipSuccess)
+    return gd_fail(GD_ERR_HIP, "valu kind probe: %s", hipGetErrorString(e));
+  return GD_OK;
+}

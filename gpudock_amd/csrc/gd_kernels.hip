@@ -1980,3 +1980,97 @@ hipError_t gd_launch_valu_probe(float *out, int blocks, int iters, hipStream_t s
   hipLaunchKernelGGL(gd_valu_probe_kernel, dim3(blocks), dim3(256), 0, st, out, iters);
   return hipGetLastError();
 }
+
+/* Issue rate of one VALU instruction kind: eight independent register
+ * chains per lane, 16 rounds of them per loop iteration, nothing but
+ * registers inside the loop and one store per lane at the end.  The kinds
+ * are the ones the energy loops are made of (gd_valu_kind_names). */
+#define GD_K8(...)                                                              \
+  do {                                                                          \
+    _Pragma("unroll") for (int k_ = 0; k_ < 16; ++k_) {                        \
+      GD_K1(x0, __VA_ARGS__); GD_K1(x1, __VA_ARGS__); GD_K1(x2, __VA_ARGS__);  \
+      GD_K1(x3, __VA_ARGS__); GD_K1(x4, __VA_ARGS__); GD_K1(x5, __VA_ARGS__);  \
+      GD_K1(x6, __VA_ARGS__); GD_K1(x7, __VA_ARGS__);                          \
+    }                                                                           \
+  } while (0)
+template <int KIND>
+__global__ void __launch_bounds__(256)
+    gd_valu_kind_kernel(float *out, int iters, unsigned long long mask, float sv)
+{
+  float x0 = (float) threadIdx.x * 1e-3f, x1 = x0 + 1.0f, x2 = x0 + 2.0f, x3 = x0 + 3.0f;
+  float x4 = x0 + 4.0f, x5 = x0 + 5.0f, x6 = x0 + 6.0f, x7 = x0 + 7.0f;
+  const float a = 0.999f, b = 1e-3f;
+  int sink = 0;
+  for (int i = 0; i < iters; ++i) {
+    if (KIND == 0) {
+#define GD_K1(x, ...) asm volatile("v_fmac_f32 %0, %1, %2" : "+v"(x) : "v"(a), "v"(b))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 1) {
+#define GD_K1(x, ...) asm volatile("v_fma_f32 %0, %1, %2, %0" : "+v"(x) : "s"(sv), "v"(b))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 2) {
+#define GD_K1(x, ...) asm volatile("v_mul_f32 %0, %0, %1" : "+v"(x) : "v"(a))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 3) {
+#define GD_K1(x, ...) asm volatile("v_max_f32 %0, %0, %1" : "+v"(x) : "v"(b))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 4) {
+#define GD_K1(x, ...) asm volatile("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(x) : "v"(a), "s"(mask))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 5) {
+#define GD_K1(x, ...) asm volatile("v_lshl_add_u32 %0, %0, 1, %1" : "+v"(x) : "v"(b))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 6) {
+#define GD_K1(x, ...) asm volatile("v_cmp_gt_f32 vcc, %0, %1" : : "v"(x), "v"(a) : "vcc")
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 7) {
+#define GD_K1(x, ...) \
+  asm volatile("v_add_f32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(x))
+      GD_K8(0);
+#undef GD_K1
+    } else if (KIND == 8) {
+#define GD_K1(x, ...)                                                    \
+  do {                                                                   \
+    int s_;                                                              \
+    asm volatile("v_readlane_b32 %0, %1, 5" : "=s"(s_) : "v"(x));        \
+    sink ^= s_;                                                          \
+  } while (0)
+      GD_K8(0);
+#undef GD_K1
+    } else {
+      /* FMA and max alternating: how much of a slower kind hides behind
+       * the full-rate one */
+#define GD_K1(x, ...) asm volatile("v_fmac_f32 %0, %1, %2\n\tv_max_f32 %0, %0, %1" : "+v"(x) : "v"(a), "v"(b))
+      GD_K8(0);
+#undef GD_K1
+    }
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] =
+      ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7)) + (float) sink;
+}
+#undef GD_K8
+
+hipError_t gd_launch_valu_kind(int kind, float *out, int blocks, int iters, hipStream_t st)
+{
+  const unsigned long long m = 0x5555555555555555ull;
+  switch (kind) {
+  case 0: hipLaunchKernelGGL(gd_valu_kind_kernel<0>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 1: hipLaunchKernelGGL(gd_valu_kind_kernel<1>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 2: hipLaunchKernelGGL(gd_valu_kind_kernel<2>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 3: hipLaunchKernelGGL(gd_valu_kind_kernel<3>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 4: hipLaunchKernelGGL(gd_valu_kind_kernel<4>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 5: hipLaunchKernelGGL(gd_valu_kind_kernel<5>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 6: hipLaunchKernelGGL(gd_valu_kind_kernel<6>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 7: hipLaunchKernelGGL(gd_valu_kind_kernel<7>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  case 8: hipLaunchKernelGGL(gd_valu_kind_kernel<8>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  default: hipLaunchKernelGGL(gd_valu_kind_kernel<9>, dim3(blocks), dim3(256), 0, st, out, iters, m, 1.0f); break;
+  }
+  return hipGetLastError();
+}

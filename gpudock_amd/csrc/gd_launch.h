@@ -34,6 +34,8 @@ hipError_t gd_launch_score(const gd_pack *packs, const gd_task *tasks, int n_tas
 /* pose i: movematrix at move[i * move_stride .. +6] */
 /* blocks x 256 lanes, each 128 x iters FMAs; out holds blocks x 256 floats */
 hipError_t gd_launch_valu_probe(float *out, int blocks, int iters, hipStream_t st);
+/* one kind of gd_valu_probe_kinds (0 .. GD_VALU_KINDS - 1) */
+hipError_t gd_launch_valu_kind(int kind, float *out, int blocks, int iters, hipStream_t st);
 hipError_t gd_launch_rescore64(const gd_pack *packs, int cx_index, const int *lig_conf,
                                const int *prt_conf, const float *move, int move_stride,
                                int n_pose, double *energy, hipStream_t st);

@@ -293,6 +293,8 @@ SYMBOLS = {
     "gd_load_norpara": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(EnePara0)]),
     "gd_last_error": (C.c_char_p, []),
     "gd_device_count": (C.c_int, []),
+    "gd_valu_probe_kinds": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int]),
+    "gd_valu_kind_name": (C.c_char_p, [C.c_int]),
     "gd_valu_probe": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "gd_version": (C.c_char_p, []),
@@ -583,6 +585,17 @@ def valu_probe(waves_per_simd=8, device=0):
     ns, tf, n = C.c_double(), C.c_double(), C.c_int()
     check(lib().gd_valu_probe(device, waves_per_simd, C.byref(ns), C.byref(tf), C.byref(n)))
     return ns.value, tf.value, n.value
+
+
+VALU_KINDS = 10
+
+
+def valu_probe_kinds(waves_per_simd=8, device=0):
+    """gd_valu_probe_kinds: {instruction kind: ns per wavefront instruction
+    per SIMD} for the kinds the energy loops are made of."""
+    ns = (C.c_double * VALU_KINDS)()
+    check(lib().gd_valu_probe_kinds(device, waves_per_simd, ns, VALU_KINDS))
+    return {lib().gd_valu_kind_name(k).decode(): ns[k] for k in range(VALU_KINDS)}
 
 
 def score_poses(cx, lig_conf, prt_conf, moves, term_order=0, device=0, fp64=True,

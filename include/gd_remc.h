@@ -636,6 +636,17 @@ int gd_load_norpara(const char *path_a, const char *path_b,
 int gd_valu_probe(int device, int waves_per_simd, double *ns_per_inst, double *tflops,
                   int *n_simd);
 
+/* Issue rate per instruction kind: the same probe for each of the
+ * GD_VALU_KINDS instruction kinds the energy loops are made of (FMA with
+ * vector operands, FMA with a scalar-register operand, mul, max, cndmask
+ * with a scalar mask, integer shift-add, compare, DPP add, readlane, and
+ * FMA/max alternating); ns_per_inst[k] = time per wavefront instruction per
+ * SIMD of kind k, named by gd_valu_kind_name(k); n_kinds must be
+ * GD_VALU_KINDS. */
+#define GD_VALU_KINDS 10
+int gd_valu_probe_kinds(int device, int waves_per_simd, double *ns_per_inst, int n_kinds);
+const char *gd_valu_kind_name(int kind);
+
 /* ---------------------------------------------------------------------- */
 const char *gd_last_error(void);
 int gd_device_count(void);

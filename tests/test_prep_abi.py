@@ -151,6 +151,29 @@ def test_valu_probe_arguments():
         assert rc == -4
 
 
+def test_valu_probe_kinds_arguments():
+    ns = (C.c_double * remc.VALU_KINDS)()
+    rc = remc.lib().gd_valu_probe_kinds(0, 8, ns, remc.VALU_KINDS - 1)
+    assert rc == -1 and b"n_kinds" in remc.lib().gd_last_error()
+    rc = remc.lib().gd_valu_probe_kinds(0, 9, ns, remc.VALU_KINDS)
+    assert rc == -1
+    names = [remc.lib().gd_valu_kind_name(k) for k in range(remc.VALU_KINDS)]
+    assert all(names) and len(set(names)) == remc.VALU_KINDS
+    assert remc.lib().gd_valu_kind_name(remc.VALU_KINDS) is None
+    assert remc.lib().gd_valu_kind_name(-1) is None
+    if remc.lib().gd_device_count() == 0:
+        assert remc.lib().gd_valu_probe_kinds(0, 8, ns, remc.VALU_KINDS) == -4
+
+
+@pytest.mark.gpu
+def test_valu_probe_kinds_on_gpu():
+    """Issue time per instruction kind; only plausibility is checked (every
+    kind between 0.2 and 50 ns per wavefront instruction per SIMD)."""
+    rates = remc.valu_probe_kinds(8)
+    assert len(rates) == remc.VALU_KINDS
+    assert all(0.2 < v < 50.0 for v in rates.values()), rates
+
+
 @pytest.mark.gpu
 def test_valu_probe_on_gpu():
     """The FP32 FMA issue ceiling: about one wavefront instruction per 1.1 ns
