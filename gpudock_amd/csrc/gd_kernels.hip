@@ -220,105 +220,109 @@ __device__ __forceinline__ void gd_atoms_block(
       (void *) (P.pt_eh + (size_t) prt * P.pnp_pad), 0, P.pnp_pad * 8, GD_BUFFER_DWORD3);
   const gd_lds_f32 *psp_lds = (const gd_lds_f32 *) (smem + P.off_psp);
 
-  /* atoms l < 64 sit in lane l of (ax0, ay0, az0), atoms l >= 64 in lane
-   * l - 64 of (ax1, ay1, az1): one pass per register set, so an atom's
-   * coordinates are three readlanes */
+  /* the atom loops, for one way of reading the pocket: the choice is made
+   * once per call, not per atom */
+  auto atoms = [&](auto read) {
+    /* atoms l < 64 sit in lane l of (ax0, ay0, az0), atoms l >= 64 in lane
+     * l - 64 of (ax1, ay1, az1): one pass per register set, so an atom's
+     * coordinates are three readlanes */
 #pragma unroll 1
-  for (int g = l0 / GD_WAVE; g * GD_WAVE < l1; ++g) {
-    const float gx = g ? ax1 : ax0, gy = g ? ay1 : ay0, gz = g ? az1 : az0;
-    const int la = l0 > g * GD_WAVE ? l0 : g * GD_WAVE;
-    const int lb = l1 < (g + 1) * GD_WAVE ? l1 : (g + 1) * GD_WAVE;
-    for (int l = la; l < lb; ++l) {
-      const int src = l & (GD_WAVE - 1);
-      const float lx = gd_bcast(gx, src);
-      const float ly = gd_bcast(gy, src);
-      const float lz = gd_bcast(gz, src);
-      const gd_atom_rec ar = gd_load_atom(P.atom + l);
-      const int row = ar.row;
-      const float lq = ar.q;
-      const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
-      /* the PSP row as a buffer: lane offset fixed, chunk offset scalar */
-      const __amdgpu_buffer_rsrc_t psp_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-          (void *) (psp_prt + (size_t) row * P.pnp_pad), 0, P.pnp_pad * 4, GD_BUFFER_DWORD3);
-      float hacc = 0.0f;
-      acc.ele = 0.0f;
-      /* chunks [0, kh) hold the hydrophobic-class points (the canonical order
-       * puts them first) and [0, kp) the class-2 points, kp >= kh; beyond kp
-       * every hydrophobic weight and every PSP value is 0, whose terms would
-       * add exact zeros, so that loop leaves the terms (and the PSP read) out */
-      auto chunks = [&](auto read, auto hpc_tag, auto hdb_tag, auto psp_tag, int k0, int k1) {
-        for (int k = k0; k < k1; ++k) {
-          float4 q;
-          float2 eh;
-          read(k, q, eh);
-          const uint32_t bits = __float_as_uint(q.w);
-          /* the PSP read (chunk base uniform, lane offset fixed) is consumed
-           * at the end of the pair; points outside class 2 hold PSP 0 */
-          float pv = 0.0f;
-          if (decltype(psp_tag)::value) {
-            if (XL)
-              pv = psp_lds[row * P.pnp_pad + k * GD_WAVE + lane];
-            else
-              pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                  psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
+    for (int g = l0 / GD_WAVE; g * GD_WAVE < l1; ++g) {
+      const float gx = g ? ax1 : ax0, gy = g ? ay1 : ay0, gz = g ? az1 : az0;
+      const int la = l0 > g * GD_WAVE ? l0 : g * GD_WAVE;
+      const int lb = l1 < (g + 1) * GD_WAVE ? l1 : (g + 1) * GD_WAVE;
+      for (int l = la; l < lb; ++l) {
+        const int src = l & (GD_WAVE - 1);
+        const float lx = gd_bcast(gx, src);
+        const float ly = gd_bcast(gy, src);
+        const float lz = gd_bcast(gz, src);
+        const gd_atom_rec ar = gd_load_atom(P.atom + l);
+        const int row = ar.row;
+        const float lq = ar.q;
+        const gd_pair_param *trow = s_tab + row * GD_ROW_STRIDE;
+        /* the PSP row as a buffer: lane offset fixed, chunk offset scalar */
+        const __amdgpu_buffer_rsrc_t psp_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *) (psp_prt + (size_t) row * P.pnp_pad), 0, P.pnp_pad * 4, GD_BUFFER_DWORD3);
+        float hacc = 0.0f;
+        acc.ele = 0.0f;
+        /* chunks [0, kh) hold the hydrophobic-class points (the canonical order
+         * puts them first) and [0, kp) the class-2 points, kp >= kh; beyond kp
+         * every hydrophobic weight and every PSP value is 0, whose terms would
+         * add exact zeros, so that loop leaves the terms (and the PSP read) out */
+        auto chunks = [&](auto read, auto hpc_tag, auto hdb_tag, auto psp_tag, int k0, int k1) {
+          for (int k = k0; k < k1; ++k) {
+            float4 q;
+            float2 eh;
+            read(k, q, eh);
+            const uint32_t bits = __float_as_uint(q.w);
+            /* the PSP read (chunk base uniform, lane offset fixed) is consumed
+             * at the end of the pair; points outside class 2 hold PSP 0 */
+            float pv = 0.0f;
+            if (decltype(psp_tag)::value) {
+              if (XL)
+                pv = psp_lds[row * P.pnp_pad + k * GD_WAVE + lane];
+              else
+                pv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    psp_rsrc, ulane * 4u, k * GD_WAVE * 4, 0));
+            }
+            const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
+            if (decltype(hpc_tag)::value)
+              hacc = gd_hpc_acc2(d2, eh.y, hacc);
+            const gd_pair_param pp =
+                *(const gd_pair_param *) ((const char *) trow + bits);
+            const int near_any = __ballot(d2 < sc_v.dcut2) != 0;
+            const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, decltype(psp_tag)::value,
+                                              pv, decltype(hdb_tag)::value, near_any, &acc);
+            if (mref) {
+              const unsigned long long now = __ballot(contact);
+              const unsigned long long ref = mref[(size_t) l * nchunk + k];
+              tp += __popcll(now & ref);
+              fp += __popcll(now & ~ref);
+              fn += __popcll(~now & ref);
+            }
           }
-          const float d2 = gd_dist2(lx - q.x, ly - q.y, lz - q.z);
-          if (decltype(hpc_tag)::value)
-            hacc = gd_hpc_acc2(d2, eh.y, hacc);
-          const gd_pair_param pp =
-              *(const gd_pair_param *) ((const char *) trow + bits);
-          const int near_any = __ballot(d2 < sc_v.dcut2) != 0;
-          const int contact = gd_pair_terms(d2, eh.x, &pp, &sc_v, decltype(psp_tag)::value,
-                                            pv, decltype(hdb_tag)::value, near_any, &acc);
-          if (mref) {
-            const unsigned long long now = __ballot(contact);
-            const unsigned long long ref = mref[(size_t) l * nchunk + k];
-            tp += __popcll(now & ref);
-            fp += __popcll(now & ~ref);
-            fn += __popcll(~now & ref);
+        };
+        /* rows of ligand types without any H-bond pair leave that test out;
+         * the chunks [kh, kp) (none in what the loaders build) run with the
+         * hydrophobic term, whose points there hold hpp = 0 and add exact
+         * zeros, so an atom needs two loops, not three */
+        auto rows = [&](auto read) {
+          const gd_bool<true> on;
+          const gd_bool<false> off;
+          if (ar.hdb) {
+            chunks(read, on, on, on, 0, kp);
+            chunks(read, off, on, off, kp, nchunk);
+          } else {
+            chunks(read, on, off, on, 0, kp);
+            chunks(read, off, off, off, kp, nchunk);
           }
-        }
-      };
-      /* rows of ligand types without any H-bond pair leave that test out;
-       * the chunks [kh, kp) (none in what the loaders build) run with the
-       * hydrophobic term, whose points there hold hpp = 0 and add exact
-       * zeros, so an atom needs two loops, not three */
-      auto rows = [&](auto read) {
-        const gd_bool<true> on;
-        const gd_bool<false> off;
-        if (ar.hdb) {
-          chunks(read, on, on, on, 0, kp);
-          chunks(read, off, on, off, kp, nchunk);
-        } else {
-          chunks(read, on, off, on, 0, kp);
-          chunks(read, off, off, off, kp, nchunk);
-        }
-      };
-      if (XL || pocket_lds) {
-        rows([&](int k, float4 &q, float2 &eh) {
-          const float2 *__restrict__ c =
-              (const float2 *) (smem + k * GD_LDS_CHUNK_BYTES) + lane;
-          const float2 xy = c[0], zw = c[GD_WAVE];
-          q = make_float4(xy.x, xy.y, zw.x, zw.y);
-          eh = c[2 * GD_WAVE];
-        });
-      } else {
-        /* whole-protein pockets: the conformer's points as raw buffers, lane
-         * offset fixed, chunk offset scalar */
-        rows([&](int k, float4 &q, float2 &eh) {
-          const gd_f4 v = __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(
-                                                        pos_rsrc, ulane * 16u, k * GD_WAVE * 16, 0));
-          const gd_f2 w = __builtin_bit_cast(gd_f2, __builtin_amdgcn_raw_buffer_load_b64(
-                                                        eh_rsrc, ulane * 8u, k * GD_WAVE * 8, 0));
-          q = make_float4(v.x, v.y, v.z, v.w);
-          eh = make_float2(w.x, w.y);
-        });
+        };
+        rows(read);
+        eele = gd_fma(lq, acc.ele, eele);
+        const float hs = 0.5f * gd_wave_sum(hacc);   /* the pairs added twice the kernel */
+        const float z = (hs - ar.hpl0) * ar.inv_hpl1;
+        ehpc += 0.5f * z * z - ar.hpl2;
       }
-      eele = gd_fma(lq, acc.ele, eele);
-      const float hs = 0.5f * gd_wave_sum(hacc);   /* the pairs added twice the kernel */
-      const float z = (hs - ar.hpl0) * ar.inv_hpl1;
-      ehpc += 0.5f * z * z - ar.hpl2;
     }
+  };
+  if (XL || pocket_lds) {
+    atoms([&](int k, float4 &q, float2 &eh) {
+      const float2 *__restrict__ c = (const float2 *) (smem + k * GD_LDS_CHUNK_BYTES) + lane;
+      const float2 xy = c[0], zw = c[GD_WAVE];
+      q = make_float4(xy.x, xy.y, zw.x, zw.y);
+      eh = c[2 * GD_WAVE];
+    });
+  } else {
+    /* whole-protein pockets: the conformer's points as raw buffers, lane
+     * offset fixed, chunk offset scalar */
+    atoms([&](int k, float4 &q, float2 &eh) {
+      const gd_f4 v = __builtin_bit_cast(gd_f4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                    pos_rsrc, ulane * 16u, k * GD_WAVE * 16, 0));
+      const gd_f2 w = __builtin_bit_cast(gd_f2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                    eh_rsrc, ulane * 8u, k * GD_WAVE * 8, 0));
+      q = make_float4(v.x, v.y, v.z, v.w);
+      eh = make_float2(w.x, w.y);
+    });
   }
   blk.vdw = acc.vdw;
   blk.pmf = acc.pmf;
