@@ -273,7 +273,11 @@ int main(int argc, char **argv)
   std::printf("MC sweeps per second\t\t%.3f\n", mcs0);
   const double mcs1 = (double) moves / log.t1;
   std::printf("wall time\t\t\t%.3f seconds\n", log.t1);
-  std::printf("MC sweeps per second\t\t%.3f\n%s", mcs1, bar);
+  std::printf("MC sweeps per second\t\t%.3f\n", mcs1);
+  /* no counterpart in the reference's report: the whole call, with setup,
+   * upload, initial energies, FP64 re-score and read-back */
+  std::printf("call time\t\t\t%.3f seconds\n", log.t2);
+  std::printf("MC sweeps per second\t\t%.3f\n%s", (double) moves / log.t2, bar);
 
   if (!a.csv.empty()) {
     FILE *out = std::fopen(a.csv.c_str(), "w");

@@ -191,7 +191,10 @@ typedef struct gd_mcpara {
 
 /* McLog (dock.h:472-482).  acs_mc / acs_temp_exchg are [n_rep] arrays the
  * caller provides (or NULL); they are accumulated into, as run.cu:284-296
- * does.  t0 = seconds of MC compute, t1 = t0 plus result transfer. */
+ * does.  As the reference takes them (gpusingle.cu:10,31,41): t0 = seconds
+ * of the MC loop (the cycle launches, after setup, upload and the initial
+ * energies), t1 = t0 plus the trajectory dumps; t2 (unused there) = the
+ * whole gd_run call, setup to results on the host (FP64 re-score, copies). */
 typedef struct gd_mclog {
   double t0, t1, t2;
   long long ac_mc;

@@ -67,6 +67,7 @@ def test_cli_run_equals_library(tmp_path):
                       "-seed", "7", "-csv", str(out)])
     assert r.returncode == 0, r.stderr
     assert "AR of MC" in r.stdout and "MC sweeps per second" in r.stdout
+    assert "compute time" in r.stdout and "call time" in r.stdout
     rows = list(csv.DictReader(open(out)))
 
     from test_loaders import load_1a07c1_files

@@ -132,6 +132,9 @@ def test_run_matches_replay_c0_full():
     _compare_runs(g, r)
     assert g["acs_mc"].sum() > 0 and g["acs_exchg"].sum() > 0
     assert log.ac_mc == g["acs_mc"].sum()
+    # clocks as the reference takes them: t0 the MC loop, t1 = t0 without
+    # trajectory dumps, t2 the whole call (setup, upload, re-score, copies)
+    assert 0.0 < log.t0 <= log.t1 < log.t2
     # best-pose FP64 re-score equals the oracle's FP64 of the same pose
     n_lig, n_tmp = 4, 8
     idx = np.arange(len(g["acs_mc"]))
