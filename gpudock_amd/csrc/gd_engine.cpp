@@ -1325,8 +1325,12 @@ extern "C" int gd_run(const gd_ligand *lig, const gd_protein *prt, const gd_psp 
   const int spe = mcpara->steps_per_exchange;
   const int n_cycles = s->total / spe;
   /* the reference's clocks cover its MC loop only (gpusingle.cu:8-41):
-   * setup, upload and the initial energies finish before they start */
-  rc = s->eng.sync();
+   * setup (here also the task lists and the launch shape, else built on
+   * the first cycle), upload and the initial energies finish before they
+   * start */
+  rc = s->eng.ensure_built();
+  if (!rc)
+    rc = s->eng.sync();
   if (rc) {
     gd_session_destroy(s);
     return rc;
