@@ -1855,9 +1855,19 @@ static hipError_t set_attr_wpg(int bytes)
 template <int SPLIT>
 static hipError_t set_attr_split(int bytes)
 {
+  /* a complex whose staging and split area exceed the device's LDS never
+   * runs split (the engine checks the same sum), so it needs no attribute */
+  int dev = 0, max_lds = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess)
+    return e;
+  const int want = bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES;
+  if (want > max_lds)
+    return hipSuccess;
   return hipFuncSetAttribute((const void *) gd_mc_split_kernel<SPLIT>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES);
+                             hipFuncAttributeMaxDynamicSharedMemorySize, want);
 }
 
 hipError_t gd_set_kernel_attrs(int stage_bytes)
