@@ -357,13 +357,21 @@ __device__ __forceinline__ float gd_kde_rounds(const gd_pack &P, const char *sme
   auto rounds = [&](auto read) {
     /* rounds [ra, rb) of one group, whose atoms lanes hold in (gx, gy, gz);
      * padding entries name lane 0 and the zero-weight point */
+    auto entry = [&](int r) -> uint32_t {
+      return XL ? ((const __attribute__((address_space(3))) uint32_t *) (smem + P.off_pairs))
+                      [r * GD_WAVE + ulane]
+                : __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
+                                                       r * GD_WAVE * 4, 0);
+    };
     auto group = [&](int ra, int rb, float gx, float gy, float gz) {
+      if (ra >= rb)
+        return;
+      /* each round's entries are loaded one round ahead (the last round
+       * loads its own again), so the point read does not wait for them */
+      uint32_t e_next = entry(ra);
       for (int r = ra; r < rb; ++r) {
-        const uint32_t e =
-            XL ? ((const __attribute__((address_space(3))) uint32_t *) (smem + P.off_pairs))
-                     [r * GD_WAVE + ulane]
-               : __builtin_amdgcn_raw_buffer_load_b32(pairs_rsrc, ulane * 4u,
-                                                      r * GD_WAVE * 4, 0);
+        const uint32_t e = e_next;
+        e_next = entry(r + 1 < rb ? r + 1 : r);
         const int src = (int) (e & 0xffu);
         const float lx = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gx)));
         const float ly = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(gy)));
