@@ -737,8 +737,8 @@ int gd_engine::autotune(gd_tune_report *rep)
     }
   };
   /* staged search: launch shape (and, for few replicas, wavefronts per
-   * replica); then KDE placement x register budget; then the launch shape
-   * again at the chosen placement and budget */
+   * replica); then KDE placement x register budget; then every launch shape
+   * and register budget at the chosen placement */
   const int wpgs[3] = {4, 8, 16};
   const int kg0 = kde_global, occ0 = lc.occ;
   for (int wi = 0; wi < 3; ++wi)
@@ -758,12 +758,29 @@ int gd_engine::autotune(gd_tune_report *rep)
           trial(c.wpg, c.persistent, any_kde_lds ? kg : kg0, occ, c.split);
   }
   if (best_c.split <= 1) {
-    const gd_launch_cfg c = best_c;
+    /* the best shape of the first stage may not be the best at another
+     * register budget: all of them again (a shape far slower in the first
+     * stage, as 4 waves/group on LDS-bound complexes, is left out) */
     const int kg = best_kg;
-    for (int wi = 0; wi < 3; ++wi)
+    auto tried = [&](int wpg, int pers, int occ) {
+      for (const Cand &c : cands)
+        if (c.wpg == wpg && c.pers == pers && c.kg == kg && c.occ == occ && c.split <= 1)
+          return true;
+      return false;
+    };
+    float first_best[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+    for (const Cand &c : cands)
+      for (int wi = 0; wi < 3; ++wi)
+        if (c.wpg == wpgs[wi] && c.split <= 1 && c.ms < first_best[wi])
+          first_best[wi] = c.ms;
+    for (int wi = 0; wi < 3; ++wi) {
+      if (first_best[wi] > 1.25f * best)
+        continue;
       for (int pers = 0; pers < 2; ++pers)
-        if ((wpgs[wi] != c.wpg || pers != c.persistent) && (c.occ != occ0 || kg != kg0))
-          trial(wpgs[wi], pers, kg, c.occ, 1);
+        for (int occ : {0, 6, 8})
+          if (!tried(wpgs[wi], pers, occ))
+            trial(wpgs[wi], pers, kg, occ, 1);
+    }
   }
   /* the staged trials differ by a few per cent, about what one short trial
    * can resolve: the three fastest distinct configurations are timed again
@@ -784,11 +801,15 @@ int gd_engine::autotune(gd_tune_report *rep)
       if (top.size() == 3)
         break;
     }
+    /* about 100 ms of cycles per configuration and pass (4 to 16 cycles):
+     * the shapes left differ by a few per cent, and shorter windows read
+     * the device's clock changes as well */
+    const int reps = std::max(4, std::min(16, (int) (100.0f / std::max(top[0].ms, 1e-3f))));
     std::vector<float> fastest(top.size(), 3.0e38f);
     for (int pass = 0; pass < 2; ++pass)
       for (size_t i = 0; i < top.size(); ++i) {
         const Cand &c = top[i];
-        const float ms = time_cfg(c.wpg, c.pers, c.kg, c.occ, c.split, 4);
+        const float ms = time_cfg(c.wpg, c.pers, c.kg, c.occ, c.split, reps);
         if (ms >= 0.0f && ms < fastest[i])
           fastest[i] = ms;
       }

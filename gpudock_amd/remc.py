@@ -140,7 +140,7 @@ class EngineConfig(C.Structure):
                 ("staging_bytes", C.c_size_t), ("occ", C.c_int), ("split", C.c_int)]
 
 
-TUNE_MAX = 16
+TUNE_MAX = 32
 
 
 class TuneReport(C.Structure):

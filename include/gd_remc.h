@@ -408,7 +408,7 @@ typedef struct gd_engine_config {
                           kept in HBM */
 } gd_engine_config;
 
-#define GD_TUNE_MAX 16
+#define GD_TUNE_MAX 32
 typedef struct gd_tune_report {
   int n;               /* launch shapes tried */
   int wpg[GD_TUNE_MAX];
