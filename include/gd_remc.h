@@ -494,7 +494,7 @@ int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_config *cfg,
 typedef struct gd_server gd_server;
 
 /* Launch shape: on a mixed stream of small and medium complexes, wpg 16,
- * persistent 0, occ 8 and kde_policy GD_KDE_GLOBAL served a 1,000-job burst
+ * persistent 0, occ 8 and kde_policy GD_KDE_STAGE_NEVER served a 1,000-job burst
  * 3 % faster than wpg 8, persistent 1, occ 6, and at a load below capacity
  * the jobs' p50 latency was 2.4 times lower (fewer, larger workgroup tasks
  * per job). */
