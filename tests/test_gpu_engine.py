@@ -53,7 +53,7 @@ def test_batch_equals_replay_of_each_job(jobs):
     each job's results are those of its replay alone."""
     js, replays = jobs
     res, rep = remc.batch_run(js, autotune=1)
-    assert 6 <= rep["tune"]["n"] <= 16
+    assert 6 <= rep["tune"]["n"] <= remc.TUNE_MAX
     assert rep["layout"]["n_complexes"] == len(js)
     assert rep["layout"]["n_classes"] >= 1
     for i, (g, r) in enumerate(zip(res, replays)):
