@@ -9,6 +9,7 @@ two); the GPU FP64 re-score matches the oracle's FP64 within 1e-9 relative.
 Per run: with the same Philox stream, every accept and exchange decision
 and every final energy equals the replay's (bitwise)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -150,6 +151,20 @@ def test_decision_parity_1e4_steps():
     CPU replay's on the same Philox stream."""
     cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=1, steps=10000)
     opt = remc.default_options(seed=1)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    assert g["acs_exchg"].sum() > 0
+
+
+@pytest.mark.skipif(os.environ.get("GD_LONG_PARITY") != "1",
+                    reason="10^5 steps: about 80 s of CPU replay; set GD_LONG_PARITY=1")
+def test_decision_parity_1e5_steps():
+    """10^5 MC steps on 8 temperatures with exchange every 10 steps (10^4
+    exchange sweeps): every accept and exchange decision and the final state
+    bits equal the CPU replay's."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=1, steps=100000)
+    opt = remc.default_options(seed=3)
     g, _ = remc.run(cx, temps, mc, opt)
     r = O.replay_run(cx, temps, mc, opt)
     _compare_runs(g, r)
