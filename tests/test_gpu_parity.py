@@ -148,13 +148,19 @@ def test_run_matches_replay_c0_full():
 def test_decision_parity_1e4_steps():
     """10^4 MC steps on 8 temperatures with exchange every 10 steps: the GPU
     trajectory (accept counts, exchange counts, final state bits) equals the
-    CPU replay's on the same Philox stream."""
+    CPU replay's on the same Philox stream, split over several wavefronts per
+    replica and with one wavefront per replica."""
     cx, temps, mc, cfg = synth.make_config("c0", n_lig=1, n_prt=1, steps=10000)
     opt = remc.default_options(seed=1)
     g, _ = remc.run(cx, temps, mc, opt)
     r = O.replay_run(cx, temps, mc, opt)
     _compare_runs(g, r)
     assert g["acs_exchg"].sum() > 0
+    # the same run with one wavefront per replica (gd_run splits these 8
+    # replicas over several wavefronts by itself)
+    res, rep = remc.batch_run([remc.JobSpec(cx, temps, mc, opt)], split=1)
+    assert rep["layout"]["split"] == 1
+    _compare_runs(res[0], r)
 
 
 @pytest.mark.skipif(os.environ.get("GD_LONG_PARITY") != "1",
