@@ -351,6 +351,46 @@ def test_whole_protein_pocket_read_through_caches():
     _compare_runs(g, r)
 
 
+def test_reference_maximum_complex():
+    """The reference's largest complex (gpudocksm-rem-1.2/src/size.h:22-79):
+    10,000 protein points (MAXPRO), 100 ligand atoms (MAXLIG), 10,000 KDE
+    points (MAXKDE) and 1,000 MCS restraints (MAXPOS) of 100 fields each
+    (MAXMCS 256): poses equal the replay bit for bit and FP64 within the
+    tolerance; a short run, in gd_run's own layout and with one wavefront per
+    replica, equals the replay."""
+    cx = synth.make_complex(pnp=10000, radius=None, lna=100, n_lig=2, n_prt=2, pnk=10000,
+                            pos=1000, seed=31)
+    rng = np.random.default_rng(31)
+    lc, pc, mv = _poses(rng, 16, 2, 2, tsig=0.4, rsig=0.6)
+    _check_pose_parity(cx, lc, pc, mv)
+    temps = remc.temperatures(2, 0.3, 3.0)
+    mc = remc.mcpara(20, 10, [0.2, 0.2, 0.2, 0.05, 0.05, 0.05])
+    opt = remc.default_options(seed=6, mcc=1)
+    r = O.replay_run(cx, temps, mc, opt)
+    g, _ = remc.run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    res, rep = remc.batch_run([remc.JobSpec(cx, temps, mc, opt)], split=1)
+    assert rep["layout"]["split"] == 1
+    _compare_runs(res[0], r)
+
+
+@pytest.mark.parametrize("n_tmp,n_lig,n_prt", [(40, 17, 3), (1, 70, 20)])
+def test_reference_maximum_replicas(n_tmp, n_lig, n_prt):
+    """The reference's replica limits (gpudocksm-rem-1.2/src/size.h:28-40):
+    40 temperatures (MAXTMP) on 2,040 of at most 2,048 replicas (MAXREP), and
+    70 ligand x 20 protein conformers (MAXEN2, MAXEN1) at one temperature;
+    exchange every 5 steps: every decision equals the replay's."""
+    cx, temps, mc, cfg = synth.make_config("c0", n_lig=n_lig, n_prt=n_prt, n_tmp=n_tmp,
+                                           steps=30, spe=5)
+    opt = remc.default_options(seed=7)
+    g, _ = remc.run(cx, temps, mc, opt)
+    r = O.replay_run(cx, temps, mc, opt)
+    _compare_runs(g, r)
+    assert len(g["acs_mc"]) == n_tmp * n_lig * n_prt
+    if n_tmp > 1:
+        assert g["acs_exchg"].sum() > 0
+
+
 def test_class2_points_outside_the_hydrophobic_class():
     """Class-2 points (the only ones with a PSP value) whose hydrophobic flag
     is clear sort after the hydrophobic chunks, so the chunks that read PSP
