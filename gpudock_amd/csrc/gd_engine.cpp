@@ -211,19 +211,24 @@ int gd_engine::upload_image(GdImage &I, size_t off, GdBlock &b)
  * (the most that keeps the total within that), one replica per workgroup.
  * Complexes with more position restraints than the split area holds keep
  * one wavefront per replica. */
+/* Split launches stage every table, the pair list, the restraint targets
+ * and the PSP rows in LDS beside the area: restraints beyond the area's
+ * slots, a complex whose extras do not fit, or KDE points kept in HBM mean
+ * one wavefront per replica. */
+bool gd_engine::can_split(const GdImage &I) const
+{
+  const gd_pack &P = I.P;
+  return P.pos <= GD_SPLIT_MAX_POS && P.split_extra && !kde_global &&
+         P.split_lds_bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES <= max_lds;
+}
+
 int gd_engine::choose_split() const
 {
   long long reps = 0;
   for (auto *s : slots) {
-    const gd_pack &P = s->img->P;
-    /* split launches stage every table, the pair list, the restraint
-     * targets and the PSP rows in LDS beside the area: restraints beyond the
-     * area's slots, a complex whose extras do not fit, or KDE points kept in
-     * HBM mean one wavefront per replica */
-    if (P.pos > GD_SPLIT_MAX_POS || !P.split_extra || kde_global ||
-        P.split_lds_bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES > max_lds)
+    if (!can_split(*s->img))
       return 1;
-    reps += P.n_rep;
+    reps += s->img->P.n_rep;
   }
   if (lc.split == 1 || lc.split == 4 || lc.split == 8 || lc.split == 16)
     return lc.split;

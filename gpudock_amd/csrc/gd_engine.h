@@ -149,6 +149,8 @@ struct gd_engine {
   /* admit a built image; it is moved into the engine only on success */
   int add_image(std::unique_ptr<GdImage> &img, int *handle);
   int build_image(GdImage &I, const gd_job &job) const;
+  /* a complex this engine could run split over several wavefronts */
+  bool can_split(const GdImage &I) const;
   /* retire; its memory is released once queued work no longer needs it */
   int remove(int handle);
   GdSlot *find(int handle);

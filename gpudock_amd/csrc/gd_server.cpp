@@ -16,6 +16,15 @@
  *   5. wait for the oldest tick, a read-back or a submission.
  * The GPU never waits for the host while work is resident: the next tick
  * is already queued when the current one ends.
+ *
+ * Lanes: a cycle lasts as long as its slowest workgroup task, and a task's
+ * time grows with its complex's lna x pnp, so one engine holds every job to
+ * the cycle of the largest one resident.  With lanes > 1 each lane is an
+ * engine of its own (stream, arena share, pinned read-back area, worker
+ * thread running the loop above), jobs are routed by lna x pnp, and the
+ * lanes' launches run side by side on the device: a lane of small jobs
+ * cycles at its own pace, and its launches fill the CUs that another
+ * lane's launch tail leaves idle.
  */
 #include <hip/hip_runtime.h>
 
@@ -41,6 +50,7 @@ namespace {
 
 struct SrvJob {
   long long ticket = 0;
+  int lane = 0;
   std::unique_ptr<GdImage> img;     /* until admission */
   gd_server_job_info info{};
   int handle = -1;
@@ -67,16 +77,43 @@ double pct(std::vector<double> v, double q)
 
 }  // namespace
 
-struct gd_server {
-  gd_server_config cfg{};
+struct gd_server;
+
+/* One engine and the worker thread that owns it.  Fields marked (mu) are
+ * read or written under the server's mutex; the rest belong to the worker. */
+struct SrvLane {
+  gd_server *srv = nullptr;
+  int index = 0;
   gd_engine eng;
   GdArena rb;                       /* pinned read-back area */
   std::thread worker;
+  std::vector<SrvJob *> running;
+  int admitting = 0;                /* (mu) a job between the queue and the engine */
+  int n_pending = 0;                /* (mu) queued jobs routed here */
+  size_t arena_used = 0;            /* (mu) eng.arena.used() after the last change */
+  long long stat_cycles = 0, stat_rebuilds = 0, n_completed = 0;   /* (mu) */
+  int stat_split = 1;                                               /* (mu) */
+  bool split_lane = false;          /* runs several wavefronts per replica */
+  struct Batch {
+    hipEvent_t done;
+    std::vector<SrvJob *> jobs;
+  };
+  std::vector<Batch> batches;       /* read-backs in flight */
+
+  void loop();
+  bool admit_some();
+  void start_collects(std::vector<SrvJob *> &ready);
+  bool reap_collects();
+  void drop_running(SrvJob *j);
+};
+
+struct gd_server {
+  gd_server_config cfg{};
+  std::vector<std::unique_ptr<SrvLane>> lanes;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<SrvJob *> pending;
   std::unordered_map<long long, SrvJob *> jobs;   /* every job not yet forgotten */
-  std::vector<SrvJob *> running;    /* worker thread only */
   long long next_ticket = 1;
   bool stop = false;
   int worker_rc = GD_OK;
@@ -88,25 +125,48 @@ struct gd_server {
   double pose_evals = 0.0, first_submit = -1.0, last_done = 0.0;
   size_t arena_peak = 0;
   long long resident_reps = 0;
-  int admitting = 0;                /* a job between the queue and the engine */
-  long long stat_cycles = 0, stat_rebuilds = 0;
 
   double now() const
   {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
-  void loop();
-  bool admit_some();
   void finish(SrvJob *j, int state, int status);
-  struct Batch {
-    hipEvent_t done;
-    std::vector<SrvJob *> jobs;
-  };
-  std::vector<Batch> batches;       /* read-backs in flight (worker only) */
-  void start_collects(std::vector<SrvJob *> &ready);
-  bool reap_collects();
-  void drop_running(SrvJob *j);
+  int first_lane(const gd_job &job) const;
+  int admitting() const;
+  void note_arena(SrvLane &l);
 };
+
+/* The first lane a job may take: the first k < lanes - 1 with lna x pnp <=
+ * lane_max_pairs[k], else the last. */
+int gd_server::first_lane(const gd_job &job) const
+{
+  const long long pairs = (long long) job.size.lna * job.size.pnp;
+  const int n = (int) lanes.size();
+  for (int k = 0; k + 1 < n; ++k)
+    if (pairs <= cfg.lane_max_pairs[k])
+      return k;
+  return n - 1;
+}
+
+/* Jobs between the queue and an engine (lock held). */
+int gd_server::admitting() const
+{
+  int n = 0;
+  for (auto &l : lanes)
+    n += l->admitting;
+  return n;
+}
+
+/* Record a lane's arena use after a change, and the peak over all lanes
+ * (lock held; called by the lane's own worker). */
+void gd_server::note_arena(SrvLane &l)
+{
+  l.arena_used = l.eng.arena.used();
+  size_t total = 0;
+  for (auto &x : lanes)
+    total += x->arena_used;
+  arena_peak = std::max(arena_peak, total);
+}
 
 /* Mark a job finished (lock held by the caller). */
 void gd_server::finish(SrvJob *j, int state, int status)
@@ -117,6 +177,7 @@ void gd_server::finish(SrvJob *j, int state, int status)
   j->info.latency = j->info.t_done - j->info.t_submit;
   if (state == GD_JOB_DONE) {
     ++n_completed;
+    ++lanes[j->lane]->n_completed;
     lat.push_back(j->info.latency);
     wait.push_back(j->info.t_admit - j->info.t_submit);
     pose_evals += (double) j->info.n_rep * (double) j->info.steps_total;
@@ -133,70 +194,86 @@ void gd_server::finish(SrvJob *j, int state, int status)
   cv_done.notify_all();
 }
 
-/* Admit queued jobs in FIFO order while they fit.  Returns true if any. */
-bool gd_server::admit_some()
+/* Admit this lane's queued jobs in FIFO order while they fit.  Returns true
+ * if any. */
+bool SrvLane::admit_some()
 {
+  gd_server &S = *srv;
   bool any = false;
   for (;;) {
     SrvJob *j = nullptr;
     {
-      std::lock_guard<std::mutex> lk(mu);
-      while (!pending.empty() && pending.front()->cancel_req) {
-        SrvJob *c = pending.front();
-        pending.pop_front();
-        finish(c, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
-      }
-      if (pending.empty())
+      std::lock_guard<std::mutex> lk(S.mu);
+      if (!n_pending)
         return any;
-      j = pending.front();
-      if (cfg.max_resident_replicas > 0 && !running.empty() &&
-          resident_reps + j->info.n_rep > cfg.max_resident_replicas)
+      /* the first queued job of this lane; cancelled ones before it end here */
+      auto it = S.pending.begin();
+      while (it != S.pending.end()) {
+        if ((*it)->lane != index) {
+          ++it;
+          continue;
+        }
+        if (!(*it)->cancel_req)
+          break;
+        SrvJob *c = *it;
+        it = S.pending.erase(it);
+        --n_pending;
+        S.finish(c, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
+      }
+      if (it == S.pending.end())
+        return any;
+      j = *it;
+      if (S.cfg.max_resident_replicas > 0 && !running.empty() &&
+          S.resident_reps + j->info.n_rep > S.cfg.max_resident_replicas)
         return any;
       /* footprint + its initial-energy tasks must fit beside the residents */
       if (j->info.footprint + GD_ADMIT_MARGIN > eng.free_bytes() && !running.empty())
         return any;
-      pending.pop_front();
+      S.pending.erase(it);
+      --n_pending;
       admitting = 1;
     }
     int h = -1;
     int rc = eng.add_image(j->img, &h);
-    std::lock_guard<std::mutex> lk(mu);
+    std::lock_guard<std::mutex> lk(S.mu);
     admitting = 0;
     if (rc == GD_ERR_NOMEM && !running.empty()) {
-      /* fragmented arena: back to the head of the queue until a resident
-       * job retires */
-      pending.push_front(j);
+      /* fragmented arena: back to the head of the queue (it was this lane's
+       * oldest) until a resident job retires */
+      S.pending.push_front(j);
+      ++n_pending;
       return any;
     }
     if (rc) {
-      finish(j, GD_JOB_FAILED, rc);
+      S.finish(j, GD_JOB_FAILED, rc);
       continue;
     }
     j->handle = h;
     j->info.state = GD_JOB_RUNNING;
-    j->info.t_admit = now();
+    j->info.t_admit = S.now();
     running.push_back(j);
-    resident_reps += j->info.n_rep;
-    ++n_running;
-    running_peak = std::max(running_peak, n_running);
-    arena_peak = std::max(arena_peak, eng.arena.used());
+    S.resident_reps += j->info.n_rep;
+    ++S.n_running;
+    S.running_peak = std::max(S.running_peak, S.n_running);
+    S.note_arena(*this);
     any = true;
   }
 }
 
 /* Remove a running job from the engine and the running list (lock held). */
-void gd_server::drop_running(SrvJob *j)
+void SrvLane::drop_running(SrvJob *j)
 {
   (void) eng.remove(j->handle);
   running.erase(std::find(running.begin(), running.end(), j));
-  resident_reps -= j->info.n_rep;
-  --n_running;
+  srv->resident_reps -= j->info.n_rep;
+  --srv->n_running;
+  srv->note_arena(*this);
 }
 
 /* Queue one read-back (one FP64 re-score launch) for every job in `ready`
  * that finds room in the pinned read-back area; the rest wait for a later
  * round. */
-void gd_server::start_collects(std::vector<SrvJob *> &ready)
+void SrvLane::start_collects(std::vector<SrvJob *> &ready)
 {
   std::vector<GdCollect> cs;
   std::vector<SrvJob *> js;
@@ -226,12 +303,12 @@ void gd_server::start_collects(std::vector<SrvJob *> &ready)
   if (rc) {
     if (done)
       (void) hipEventDestroy(done);
-    std::lock_guard<std::mutex> lk(mu);
+    std::lock_guard<std::mutex> lk(srv->mu);
     for (SrvJob *j : js) {
       if (j->rb_bytes)
         rb.free(j->col.host_off, j->rb_bytes);
       drop_running(j);
-      finish(j, GD_JOB_FAILED, rc);
+      srv->finish(j, GD_JOB_FAILED, rc);
     }
     return;
   }
@@ -242,7 +319,7 @@ void gd_server::start_collects(std::vector<SrvJob *> &ready)
 
 /* Parse read-backs that have landed; retire their jobs.  Returns true if
  * any job finished. */
-bool gd_server::reap_collects()
+bool SrvLane::reap_collects()
 {
   bool any = false;
   for (size_t b = 0; b < batches.size();) {
@@ -252,7 +329,7 @@ bool gd_server::reap_collects()
     }
     for (SrvJob *j : batches[b].jobs) {
       GdSlot *s = eng.find(j->handle);
-      if (s && cfg.keep_results && !j->cancel_req) {
+      if (s && srv->cfg.keep_results && !j->cancel_req) {
         j->res.resize(s->img->P.n_rep);
         gd_mclog lg;
         memset(&lg, 0, sizeof lg);
@@ -266,12 +343,12 @@ bool gd_server::reap_collects()
         rb.free(j->col.host_off, j->rb_bytes);
       j->rb_pageable.clear();
       j->rb_pageable.shrink_to_fit();
-      std::lock_guard<std::mutex> lk(mu);
+      std::lock_guard<std::mutex> lk(srv->mu);
       drop_running(j);
       if (j->cancel_req)
-        finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
+        srv->finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
       else
-        finish(j, GD_JOB_DONE, GD_OK);
+        srv->finish(j, GD_JOB_DONE, GD_OK);
       any = true;
     }
     (void) hipEventDestroy(batches[b].done);
@@ -280,20 +357,24 @@ bool gd_server::reap_collects()
   return any;
 }
 
-void gd_server::loop()
+void SrvLane::loop()
 {
-  if (hipSetDevice(cfg.device) != hipSuccess) {
-    std::lock_guard<std::mutex> lk(mu);
-    worker_rc = GD_ERR_HIP;
+  gd_server &S = *srv;
+  if (hipSetDevice(S.cfg.device) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(S.mu);
+    S.worker_rc = GD_ERR_HIP;
+    S.stop = true;
+    S.cv_work.notify_all();
+    S.cv_done.notify_all();
     return;
   }
   std::deque<hipEvent_t> ticks;
-  const int per_tick = cfg.cycles_per_tick > 0 ? cfg.cycles_per_tick : 1;
-  const int max_flight = cfg.max_ticks_in_flight > 0 ? cfg.max_ticks_in_flight : 2;
+  const int per_tick = S.cfg.cycles_per_tick > 0 ? S.cfg.cycles_per_tick : 1;
+  const int max_flight = S.cfg.max_ticks_in_flight > 0 ? S.cfg.max_ticks_in_flight : 2;
   for (;;) {
     {
-      std::lock_guard<std::mutex> lk(mu);
-      if (stop)
+      std::lock_guard<std::mutex> lk(S.mu);
+      if (S.stop)
         break;
     }
     admit_some();
@@ -304,16 +385,16 @@ void gd_server::loop()
       SrvJob *j = running[i];
       bool c;
       {
-        std::lock_guard<std::mutex> lk(mu);
+        std::lock_guard<std::mutex> lk(S.mu);
         c = j->cancel_req && !j->collecting;
       }
       if (!c) {
         ++i;
         continue;
       }
-      std::lock_guard<std::mutex> lk(mu);
+      std::lock_guard<std::mutex> lk(S.mu);
       drop_running(j);
-      finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
+      S.finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
     }
 
     /* queue a tick if any resident job still has steps */
@@ -326,9 +407,10 @@ void gd_server::loop()
     if (work && (int) ticks.size() < max_flight) {
       int rc = eng.cycles(per_tick);
       {
-        std::lock_guard<std::mutex> lk(mu);
+        std::lock_guard<std::mutex> lk(S.mu);
         stat_cycles = eng.n_cycles;
         stat_rebuilds = eng.n_rebuilds;
+        stat_split = eng.split_eff;
       }
       hipEvent_t ev = nullptr;
       if (!rc && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
@@ -338,9 +420,10 @@ void gd_server::loop()
           (void) hipEventDestroy(ev);
       }
       if (rc) {
-        std::lock_guard<std::mutex> lk(mu);
-        worker_rc = rc;
-        stop = true;
+        std::lock_guard<std::mutex> lk(S.mu);
+        S.worker_rc = rc;
+        S.stop = true;
+        S.cv_work.notify_all();
         break;
       }
     }
@@ -371,9 +454,9 @@ void gd_server::loop()
     } else if (!work && !batches.empty()) {
       (void) hipEventSynchronize(batches.front().done);
     } else if (!work && !collecting) {
-      std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait_for(lk, std::chrono::milliseconds(20),
-                       [&] { return stop || !pending.empty(); });
+      std::unique_lock<std::mutex> lk(S.mu);
+      S.cv_work.wait_for(lk, std::chrono::milliseconds(20),
+                         [&] { return S.stop || n_pending > 0; });
     }
   }
   (void) hipStreamSynchronize(eng.stream);
@@ -382,18 +465,23 @@ void gd_server::loop()
   for (auto &b : batches)
     (void) hipEventDestroy(b.done);
   batches.clear();
-  std::lock_guard<std::mutex> lk(mu);
+  std::lock_guard<std::mutex> lk(S.mu);
   for (SrvJob *j : running) {
-    --n_running;
-    finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
+    --S.n_running;
+    S.finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
   }
   running.clear();
-  while (!pending.empty()) {
-    SrvJob *j = pending.front();
-    pending.pop_front();
-    finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
+  for (auto it = S.pending.begin(); it != S.pending.end();) {
+    if ((*it)->lane != index) {
+      ++it;
+      continue;
+    }
+    SrvJob *j = *it;
+    it = S.pending.erase(it);
+    --n_pending;
+    S.finish(j, GD_JOB_CANCELLED, GD_ERR_CANCELLED);
   }
-  cv_done.notify_all();
+  S.cv_done.notify_all();
 }
 
 /* ------------------------------------------------------------------------ */
@@ -409,24 +497,51 @@ extern "C" int gd_server_create(const gd_server_config *cfg, gd_server **out)
     s->cfg = *cfg;
   else
     s->cfg.keep_results = 1;
-  gd_engine_config ec;
-  memset(&ec, 0, sizeof ec);
-  ec.device = s->cfg.device;
-  ec.arena_bytes = s->cfg.arena_bytes ? s->cfg.arena_bytes : ((size_t) 4 << 30);
-  ec.wpg = s->cfg.wpg;
-  ec.persistent = s->cfg.persistent;
-  ec.use_graph = s->cfg.use_graph;
-  ec.kde_policy = s->cfg.kde_policy;
-  ec.occ = s->cfg.occ;
-  ec.split = s->cfg.split;
-  int rc = s->eng.init(&ec);
-  if (rc)
-    return rc;
-  GD_HIP_TRY(s->rb.init(s->cfg.readback_bytes ? s->cfg.readback_bytes : ((size_t) 64 << 20),
-                        true));
+  const int n_lanes = s->cfg.lanes > 1 ? s->cfg.lanes : 1;
+  if (n_lanes > GD_SERVER_MAX_LANES)
+    return gd_fail(GD_ERR_ARG, "lanes %d: at most %d", s->cfg.lanes, GD_SERVER_MAX_LANES);
+  for (int k = 0; k + 1 < n_lanes; ++k) {
+    if (s->cfg.lane_max_pairs[k] <= 0 ||
+        (k > 0 && s->cfg.lane_max_pairs[k] <= s->cfg.lane_max_pairs[k - 1]))
+      return gd_fail(GD_ERR_ARG, "lane_max_pairs must be positive and increasing");
+  }
+  const size_t arena = s->cfg.arena_bytes ? s->cfg.arena_bytes : ((size_t) 4 << 30);
+  const size_t rb = s->cfg.readback_bytes ? s->cfg.readback_bytes : ((size_t) 64 << 20);
+  for (int k = 0; k < n_lanes; ++k) {
+    std::unique_ptr<SrvLane> l(new SrvLane());
+    l->srv = s.get();
+    l->index = k;
+    gd_engine_config ec;
+    memset(&ec, 0, sizeof ec);
+    ec.device = s->cfg.device;
+    ec.arena_bytes = arena / (size_t) n_lanes;
+    ec.wpg = s->cfg.wpg;
+    ec.persistent = s->cfg.persistent;
+    ec.use_graph = s->cfg.use_graph;
+    ec.kde_policy = s->cfg.kde_policy;
+    ec.occ = s->cfg.occ;
+    ec.split = s->cfg.split;
+    if (s->cfg.lane_split[k] > 1) {
+      if (s->cfg.lane_split[k] != 4 && s->cfg.lane_split[k] != 8 && s->cfg.lane_split[k] != 16)
+        return gd_fail(GD_ERR_ARG, "lane_split[%d] = %d: 0, 1, 4, 8 or 16", k,
+                       s->cfg.lane_split[k]);
+      ec.split = s->cfg.lane_split[k];
+      ec.kde_policy = GD_KDE_STAGE_AUTO;   /* split launches read KDE points from LDS */
+      l->split_lane = true;
+    } else if (s->cfg.lane_split[k] == 1) {
+      ec.split = 1;
+    }
+    int rc = l->eng.init(&ec);
+    if (rc)
+      return rc;
+    GD_HIP_TRY(l->rb.init(rb / (size_t) n_lanes, true));
+    s->lanes.push_back(std::move(l));
+  }
   s->t0 = std::chrono::steady_clock::now();
-  gd_server *raw = s.get();
-  s->worker = std::thread([raw] { raw->loop(); });
+  for (auto &l : s->lanes) {
+    SrvLane *raw = l.get();
+    l->worker = std::thread([raw] { raw->loop(); });
+  }
   *out = s.release();
   return GD_OK;
 }
@@ -436,14 +551,24 @@ extern "C" int gd_server_submit(gd_server *s, const gd_job *job, long long *tick
   if (!s || !job || !ticket)
     return gd_fail(GD_ERR_ARG, "null argument");
   std::unique_ptr<SrvJob> j(new SrvJob());
-  j->img.reset(new GdImage());
-  int rc = s->eng.build_image(*j->img, *job);
-  if (rc)
-    return rc;
+  /* the image follows its lane's KDE placement; a split lane passes on a
+   * job it cannot split */
+  const int n_lanes = (int) s->lanes.size();
+  for (int k = s->first_lane(*job);; ++k) {
+    j->img.reset(new GdImage());
+    int rc = s->lanes[k]->eng.build_image(*j->img, *job);
+    if (rc)
+      return rc;
+    j->lane = k;
+    if (k + 1 == n_lanes || !s->lanes[k]->split_lane || s->lanes[k]->eng.can_split(*j->img))
+      break;
+  }
+  j->info.lane = j->lane;
+  const gd_engine &eng = s->lanes[j->lane]->eng;
   j->info.footprint = gd_image_footprint(*j->img);
-  if (j->info.footprint + GD_ADMIT_MARGIN > s->eng.arena.size())
-    return gd_fail(GD_ERR_NOMEM, "job needs %zu B of HBM; the server arena has %zu B",
-                   j->info.footprint, s->eng.arena.size());
+  if (j->info.footprint + GD_ADMIT_MARGIN > eng.arena.size())
+    return gd_fail(GD_ERR_NOMEM, "job needs %zu B of HBM; the server arena%s has %zu B",
+                   j->info.footprint, s->lanes.size() > 1 ? " lane" : "", eng.arena.size());
   j->info.n_rep = j->img->P.n_rep;
   j->info.steps_total = j->img->P.steps_run;
   j->info.state = GD_JOB_QUEUED;
@@ -458,8 +583,9 @@ extern "C" int gd_server_submit(gd_server *s, const gd_job *job, long long *tick
   ++s->n_submitted;
   *ticket = j->ticket;
   s->jobs[j->ticket] = j.get();
+  ++s->lanes[j->lane]->n_pending;
   s->pending.push_back(j.release());
-  s->cv_work.notify_one();
+  s->cv_work.notify_all();
   return GD_OK;
 }
 
@@ -540,7 +666,7 @@ extern "C" int gd_server_cancel(gd_server *s, long long ticket)
     return gd_fail(GD_ERR_ARG, "unknown ticket %lld", ticket);
   if (!finished(it->second->info.state))
     it->second->cancel_req = true;
-  s->cv_work.notify_one();
+  s->cv_work.notify_all();
   return GD_OK;
 }
 
@@ -550,7 +676,7 @@ extern "C" int gd_server_drain(gd_server *s)
     return gd_fail(GD_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv_done.wait(lk, [&] {
-    return s->stop || (s->pending.empty() && s->n_running == 0 && !s->admitting);
+    return s->stop || (s->pending.empty() && s->n_running == 0 && !s->admitting());
   });
   if (s->worker_rc)
     return gd_fail(s->worker_rc, "serving loop stopped on an error");
@@ -585,11 +711,18 @@ extern "C" int gd_server_get_stats(gd_server *s, gd_server_stats *st)
   st->last_done = s->last_done;
   const double span = st->last_done - st->first_submit;
   st->pose_evals_per_s = span > 0.0 ? st->pose_evals / span : 0.0;
-  st->arena_size = s->eng.arena.size();
   st->arena_peak = s->arena_peak;
   st->running_peak = s->running_peak;
-  st->cycles = s->stat_cycles;
-  st->rebuilds = s->stat_rebuilds;
+  st->lanes = (int) s->lanes.size();
+  for (size_t k = 0; k < s->lanes.size(); ++k) {
+    const SrvLane &l = *s->lanes[k];
+    st->arena_size += l.eng.arena.size();
+    st->cycles += l.stat_cycles;
+    st->rebuilds += l.stat_rebuilds;
+    st->lane_cycles[k] = l.stat_cycles;
+    st->lane_completed[k] = l.n_completed;
+    st->lane_split[k] = l.stat_split;
+  }
   return GD_OK;
 }
 
@@ -602,8 +735,9 @@ extern "C" void gd_server_destroy(gd_server *s)
     s->stop = true;
   }
   s->cv_work.notify_all();
-  if (s->worker.joinable())
-    s->worker.join();
+  for (auto &l : s->lanes)
+    if (l->worker.joinable())
+      l->worker.join();
   for (auto &kv : s->jobs)
     delete kv.second;
   s->jobs.clear();

@@ -174,19 +174,24 @@ class BatchReport(C.Structure):
                 ("layout", EngineLayout)]
 
 
+SERVER_MAX_LANES = 4   # GD_SERVER_MAX_LANES
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("arena_bytes", C.c_size_t), ("wpg", C.c_int),
                 ("persistent", C.c_int), ("use_graph", C.c_int), ("kde_policy", C.c_int),
                 ("max_resident_replicas", C.c_int), ("cycles_per_tick", C.c_int),
                 ("max_ticks_in_flight", C.c_int), ("keep_results", C.c_int),
-                ("readback_bytes", C.c_size_t), ("occ", C.c_int), ("split", C.c_int)]
+                ("readback_bytes", C.c_size_t), ("occ", C.c_int), ("split", C.c_int),
+                ("lanes", C.c_int), ("lane_max_pairs", C.c_int * (SERVER_MAX_LANES - 1)),
+                ("lane_split", C.c_int * SERVER_MAX_LANES)]
 
 
 class ServerJobInfo(C.Structure):
     _fields_ = [("ticket", C.c_longlong), ("state", C.c_int), ("status", C.c_int),
                 ("n_rep", C.c_int), ("steps_total", C.c_int), ("t_submit", C.c_double),
                 ("t_admit", C.c_double), ("t_done", C.c_double), ("latency", C.c_double),
-                ("footprint", C.c_size_t)]
+                ("footprint", C.c_size_t), ("lane", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -199,7 +204,10 @@ class ServerStats(C.Structure):
                 ("first_submit", C.c_double), ("last_done", C.c_double),
                 ("pose_evals_per_s", C.c_double), ("arena_size", C.c_size_t),
                 ("arena_peak", C.c_size_t), ("running_peak", C.c_int),
-                ("cycles", C.c_longlong), ("rebuilds", C.c_longlong)]
+                ("cycles", C.c_longlong), ("rebuilds", C.c_longlong), ("lanes", C.c_int),
+                ("lane_cycles", C.c_longlong * SERVER_MAX_LANES),
+                ("lane_completed", C.c_longlong * SERVER_MAX_LANES),
+                ("lane_split", C.c_int * SERVER_MAX_LANES)]
 
 
 class InputFiles(C.Structure):
@@ -811,10 +819,19 @@ class Server:
 
     def __init__(self, device=0, arena_bytes=0, wpg=8, persistent=0, use_graph=0,
                  kde_policy=0, max_resident_replicas=0, cycles_per_tick=1,
-                 max_ticks_in_flight=2, keep_results=1, readback_bytes=0, occ=0, split=0):
+                 max_ticks_in_flight=2, keep_results=1, readback_bytes=0, occ=0, split=0,
+                 lane_max_pairs=(), lane_split=()):
+        """lane_max_pairs: upper bounds of lna x pnp for all lanes but the
+        last (len + 1 lanes; empty = one engine); lane_split: wavefronts per
+        replica of each lane (0 = as split)."""
+        bounds = list(lane_max_pairs)
+        if len(bounds) > SERVER_MAX_LANES - 1 or len(lane_split) > SERVER_MAX_LANES:
+            raise GDError(f"at most {SERVER_MAX_LANES} lanes")
         cfg = ServerConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy,
                            max_resident_replicas, cycles_per_tick, max_ticks_in_flight,
-                           keep_results, readback_bytes, occ, split)
+                           keep_results, readback_bytes, occ, split, len(bounds) + 1,
+                           (C.c_int * (SERVER_MAX_LANES - 1))(*bounds),
+                           (C.c_int * SERVER_MAX_LANES)(*lane_split))
         h = C.c_void_p()
         check(lib().gd_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -493,6 +493,8 @@ int gd_batch_run(int n_jobs, const gd_job *jobs, const gd_batch_config *cfg,
 /* ---------------------------------------------------------------------- */
 typedef struct gd_server gd_server;
 
+#define GD_SERVER_MAX_LANES 4
+
 /* Launch shape: on a mixed stream of small and medium complexes, wpg 16,
  * persistent 0, occ 8 and kde_policy GD_KDE_STAGE_NEVER served a 1,000-job burst
  * 3 % faster than wpg 8, persistent 1, occ 6, and at a load below capacity
@@ -513,6 +515,24 @@ typedef struct gd_server_config {
   size_t readback_bytes;       /* pinned host memory for read-back; 0 = 64 MiB */
   int occ;                     /* MC kernel register budget: 0, 6 or 8 */
   int split;                   /* wavefronts per replica: as gd_engine_config.split */
+  /* Lanes: every job advances steps_per_exchange steps per cycle, and a
+   * cycle lasts as long as the slowest workgroup task in it, which is set by
+   * the largest complex resident (one wavefront walks lna x pnp pairs per
+   * step).  With lanes > 1 the server runs that many engines, each with its
+   * own stream, worker thread, 1/lanes of arena_bytes and readback_bytes,
+   * and its own cycles, side by side on the device; a job goes to the first
+   * lane k < lanes - 1 with lna x pnp <= lane_max_pairs[k], else to the
+   * last.  Admission stays FIFO within a lane; max_resident_replicas counts
+   * every lane.  0 or 1 = one engine.
+   * lane_split[k] > 1 runs lane k with that many wavefronts per replica (4,
+   * 8 or 16) and KDE points staged in LDS: a few small jobs then fill the
+   * device, so they are done in a few cycles of their own instead of waiting
+   * in a large resident set.  A job that lane cannot split (more than 64
+   * restraints, or its tables do not fit in LDS beside the split area) goes
+   * to the next lane.  0 = split as above. */
+  int lanes;
+  int lane_max_pairs[GD_SERVER_MAX_LANES - 1];
+  int lane_split[GD_SERVER_MAX_LANES];
 } gd_server_config;
 
 enum gd_job_state {
@@ -528,6 +548,7 @@ typedef struct gd_server_job_info {
   double t_submit, t_admit, t_done;   /* seconds since gd_server_create */
   double latency;              /* t_done - t_submit */
   size_t footprint;            /* device bytes while resident */
+  int lane;                    /* the lane the job was routed to */
 } gd_server_job_info;
 
 typedef struct gd_server_stats {
@@ -538,9 +559,14 @@ typedef struct gd_server_stats {
   double pose_evals;           /* MC pose evaluations of completed jobs */
   double first_submit, last_done;
   double pose_evals_per_s;     /* pose_evals / (last_done - first_submit) */
-  size_t arena_size, arena_peak;
+  size_t arena_size, arena_peak;   /* summed over lanes */
   int running_peak;
-  long long cycles, rebuilds;
+  long long cycles, rebuilds;      /* summed over lanes */
+  int lanes;
+  long long lane_cycles[GD_SERVER_MAX_LANES];
+  long long lane_completed[GD_SERVER_MAX_LANES];
+  int lane_split[GD_SERVER_MAX_LANES];   /* wavefronts per replica of each
+                                            lane's last cycle */
 } gd_server_stats;
 
 int gd_server_create(const gd_server_config *cfg, gd_server **out);

@@ -214,6 +214,65 @@ def test_server_results_equal_alone_under_continuous_batching(jobs):
     assert st["pose_evals"] == sum(j.n_rep * r for j, r in zip(js + extra, runs))
 
 
+def test_server_lanes_route_by_size(jobs):
+    """Three lanes (bounds on ligand atoms x pocket points): each job goes to
+    the first lane whose bound holds its size, the lanes cycle on their own,
+    and every result equals the replay of the job alone."""
+    js, replays = jobs
+    pairs = [j.cx.lig_xyz.shape[1] * j.cx.prt_xyz.shape[1] for j in js]
+    lo, hi = sorted(pairs)[1], sorted(pairs)[3]
+    srv = remc.Server(arena_bytes=96 << 20, wpg=8, lane_max_pairs=(lo, hi))
+    tickets = [srv.submit(j) for j in js]
+    srv.drain()
+    lanes = []
+    for i, t in enumerate(tickets):
+        g, _, info = srv.fetch(t)
+        _same(g, replays[i], f"lane-served job {i}")
+        lanes.append(info["lane"])
+    want = [0 if p <= lo else 1 if p <= hi else 2 for p in pairs]
+    assert lanes == want and set(want) == {0, 1, 2}
+    st = srv.stats()
+    srv.close()
+    assert st["lanes"] == 3 and st["completed"] == len(js)
+    assert st["lane_completed"][:3] == [want.count(k) for k in range(3)]
+    assert all(c > 0 for c in st["lane_cycles"][:3])
+    assert st["cycles"] == sum(st["lane_cycles"])
+    with pytest.raises(remc.GDError):
+        remc.Server(lane_max_pairs=(hi, lo))
+
+
+def test_server_split_lane(jobs):
+    """A first lane run with 16 wavefronts per replica (KDE points in LDS)
+    beside a lane held to one wavefront per replica: small jobs go to it, a small job with more
+    than 64 restraints (which no split launch holds) passes on to the next
+    lane, and every result equals the replay of the job alone."""
+    js, replays = jobs
+    pairs = [j.cx.lig_xyz.shape[1] * j.cx.prt_xyz.shape[1] for j in js]
+    bound = sorted(pairs)[3]
+    cx, temps, mc, _ = synth.make_config("c0", seed=77, lna=12, pnp=90, pnk=40, pos=80,
+                                         n_lig=1, n_prt=1, n_tmp=2, steps=40, spe=5)
+    many = remc.JobSpec(cx, temps, mc, remc.default_options(seed=77, complex_id=40))
+    assert 12 * 90 <= bound
+    srv = remc.Server(arena_bytes=96 << 20, wpg=8, lane_max_pairs=(bound,), lane_split=(16, 1))
+    tickets = [srv.submit(j) for j in js]
+    t_many = srv.submit(many)
+    srv.drain()
+    lanes = []
+    for i, t in enumerate(tickets):
+        g, _, info = srv.fetch(t)
+        _same(g, replays[i], f"split-lane job {i}")
+        lanes.append(info["lane"])
+    g, _, info = srv.fetch(t_many)
+    _same(g, O.replay_run(many.cx, many.temps, many.mc, many.opt), "passed-on job")
+    assert info["lane"] == 1
+    assert lanes == [0 if p <= bound else 1 for p in pairs]
+    st = srv.stats()
+    srv.close()
+    assert st["lane_split"][:2] == [16, 1]
+    with pytest.raises(remc.GDError):
+        remc.Server(lane_max_pairs=(bound,), lane_split=(3, 0))
+
+
 def test_server_admission_by_footprint_and_cancel(jobs):
     """An arena that holds one job at a time serialises admission (FIFO);
     a cancelled queued job never runs; a job larger than the arena is
