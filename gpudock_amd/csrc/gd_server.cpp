@@ -90,6 +90,7 @@ struct SrvLane {
   std::vector<SrvJob *> running;
   int admitting = 0;                /* (mu) a job between the queue and the engine */
   int n_pending = 0;                /* (mu) queued jobs routed here */
+  int n_resident = 0;               /* (mu) jobs admitted and not yet retired */
   size_t arena_used = 0;            /* (mu) eng.arena.used() after the last change */
   long long stat_cycles = 0, stat_rebuilds = 0, n_completed = 0;   /* (mu) */
   int stat_split = 1;                                               /* (mu) */
@@ -252,6 +253,7 @@ bool SrvLane::admit_some()
     j->info.state = GD_JOB_RUNNING;
     j->info.t_admit = S.now();
     running.push_back(j);
+    ++n_resident;
     S.resident_reps += j->info.n_rep;
     ++S.n_running;
     S.running_peak = std::max(S.running_peak, S.n_running);
@@ -265,6 +267,7 @@ void SrvLane::drop_running(SrvJob *j)
 {
   (void) eng.remove(j->handle);
   running.erase(std::find(running.begin(), running.end(), j));
+  --n_resident;
   srv->resident_reps -= j->info.n_rep;
   --srv->n_running;
   srv->note_arena(*this);
@@ -551,10 +554,16 @@ extern "C" int gd_server_submit(gd_server *s, const gd_job *job, long long *tick
   if (!s || !job || !ticket)
     return gd_fail(GD_ERR_ARG, "null argument");
   std::unique_ptr<SrvJob> j(new SrvJob());
-  /* the image follows its lane's KDE placement; a split lane passes on a
-   * job it cannot split */
+  /* the image follows its lane's KDE placement; a lane passes a job on when
+   * it holds lane_max_jobs already, and a split lane one it cannot split */
   const int n_lanes = (int) s->lanes.size();
   for (int k = s->first_lane(*job);; ++k) {
+    if (k + 1 < n_lanes && s->cfg.lane_max_jobs[k] > 0) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      const SrvLane &l = *s->lanes[k];
+      if (l.n_pending + l.n_resident + l.admitting >= s->cfg.lane_max_jobs[k])
+        continue;
+    }
     j->img.reset(new GdImage());
     int rc = s->lanes[k]->eng.build_image(*j->img, *job);
     if (rc)

@@ -523,16 +523,25 @@ typedef struct gd_server_config {
    * and its own cycles, side by side on the device; a job goes to the first
    * lane k < lanes - 1 with lna x pnp <= lane_max_pairs[k], else to the
    * last.  Admission stays FIFO within a lane; max_resident_replicas counts
-   * every lane.  0 or 1 = one engine.
+   * every lane.  0 or 1 = one engine.  The lanes' streams share the
+   * process's hardware queues (HIP's default is 4): with four lanes two of
+   * them shared a queue and their launches ran in turn, so three lanes is
+   * the practical number.
    * lane_split[k] > 1 runs lane k with that many wavefronts per replica (4,
    * 8 or 16) and KDE points staged in LDS: a few small jobs then fill the
    * device, so they are done in a few cycles of their own instead of waiting
    * in a large resident set.  A job that lane cannot split (more than 64
    * restraints, or its tables do not fit in LDS beside the split area) goes
-   * to the next lane.  0 = split as above. */
+   * to the next lane.  0 = split as above.
+   * lane_max_jobs[k] > 0: lane k takes a job only while fewer jobs than
+   * that are queued or resident on it; the others go to the next lane.  A
+   * split lane holds a few jobs at a time at low latency and hands the rest
+   * of a heavy load to the one-wavefront lanes, which carry more work per
+   * second.  0 = no limit; the last lane takes every job routed to it. */
   int lanes;
   int lane_max_pairs[GD_SERVER_MAX_LANES - 1];
   int lane_split[GD_SERVER_MAX_LANES];
+  int lane_max_jobs[GD_SERVER_MAX_LANES];
 } gd_server_config;
 
 enum gd_job_state {

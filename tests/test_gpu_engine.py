@@ -273,6 +273,30 @@ def test_server_split_lane(jobs):
         remc.Server(lane_max_pairs=(bound,), lane_split=(3, 0))
 
 
+def test_server_lane_overflow(jobs):
+    """A split lane that holds at most one job passes the small jobs that
+    arrive while it is busy to the next lane; every result still equals the
+    replay of the job alone."""
+    js, replays = jobs
+    pairs = [j.cx.lig_xyz.shape[1] * j.cx.prt_xyz.shape[1] for j in js]
+    bound = sorted(pairs)[3]
+    srv = remc.Server(arena_bytes=96 << 20, wpg=8, lane_max_pairs=(bound,), lane_split=(16, 1),
+                      lane_max_jobs=(1, 0))
+    tickets = [srv.submit(j) for j in js]
+    srv.drain()
+    lanes = []
+    for i, t in enumerate(tickets):
+        g, _, info = srv.fetch(t)
+        _same(g, replays[i], f"overflow job {i}")
+        lanes.append(info["lane"])
+    st = srv.stats()
+    srv.close()
+    first_small = next(i for i, p in enumerate(pairs) if p <= bound)
+    assert lanes[first_small] == 0
+    assert all(ln == 1 for ln, p in zip(lanes, pairs) if p > bound)
+    assert st["lane_completed"][:2] == [lanes.count(0), lanes.count(1)]
+
+
 def test_server_admission_by_footprint_and_cancel(jobs):
     """An arena that holds one job at a time serialises admission (FIFO);
     a cancelled queued job never runs; a job larger than the arena is
