@@ -261,7 +261,8 @@ def bench_batch(args):
 
 
 def serve_leg(rate, duration, steps, seed, wpg=16, persistent=0, occ=8, kde_policy=1,
-              arena_gib=16, tag="serve", lanes=(), lane_split=(), lane_max_jobs=()):
+              arena_gib=16, tag="serve", lanes=(), lane_split=(), lane_max_jobs=(),
+              lane_prio=()):
     """Serving loop: Poisson arrivals at `rate` jobs/s for `duration` s, each
     job a complex drawn from a pool of 256 (ligand atoms lognormal median 32
     in [10, 120], pocket points lognormal median 600 in [150, 2000]) with 16
@@ -283,7 +284,7 @@ def serve_leg(rate, duration, steps, seed, wpg=16, persistent=0, occ=8, kde_poli
              for i in range(len(arrivals))]
     srv = remc.Server(arena_bytes=arena_gib << 30, wpg=wpg, persistent=persistent,
                       keep_results=0, occ=occ, kde_policy=kde_policy, lane_max_pairs=lanes,
-                      lane_split=lane_split, lane_max_jobs=lane_max_jobs)
+                      lane_split=lane_split, lane_max_jobs=lane_max_jobs, lane_prio=lane_prio)
     print(f"{tag}: {len(specs)} jobs at {rate:.1f}/s over {duration} s", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
@@ -327,7 +328,7 @@ def serve_leg(rate, duration, steps, seed, wpg=16, persistent=0, occ=8, kde_poli
         "submit_lag_max_s": behind, "wall_s": wall,
         "server": {"wpg": wpg, "persistent": persistent, "occ": occ, "kde_policy": kde_policy,
                    "lane_max_pairs": list(lanes), "lane_split_asked": list(lane_split),
-                   "lane_max_jobs": list(lane_max_jobs),
+                   "lane_max_jobs": list(lane_max_jobs), "lane_prio": list(lane_prio),
                    "lane_split": st["lane_split"][:st["lanes"]],
                    "lane_cycles": st["lane_cycles"][:st["lanes"]],
                    "lane_completed": st["lane_completed"][:st["lanes"]]},
@@ -343,7 +344,7 @@ def lane_bounds(text):
 
 
 def burst_rate(n, steps, seed, wpg=16, persistent=0, occ=8, kde_policy=1, arena_gib=16,
-               lanes=(), lane_split=(), lane_max_jobs=()):
+               lanes=(), lane_split=(), lane_max_jobs=(), lane_prio=()):
     """n jobs submitted at once: the completion rate the server sustains."""
     from gpudock_amd import remc, synth
     import numpy as np
@@ -355,7 +356,7 @@ def burst_rate(n, steps, seed, wpg=16, persistent=0, occ=8, kde_policy=1, arena_
              for i in range(n)]
     srv = remc.Server(arena_bytes=arena_gib << 30, wpg=wpg, persistent=persistent,
                       keep_results=0, occ=occ, kde_policy=kde_policy, lane_max_pairs=lanes,
-                      lane_split=lane_split, lane_max_jobs=lane_max_jobs)
+                      lane_split=lane_split, lane_max_jobs=lane_max_jobs, lane_prio=lane_prio)
     tb = time.perf_counter()
     for sp in specs:
         srv.submit(sp)
@@ -376,7 +377,8 @@ def bench_serve(args):
     kw = dict(wpg=args.wpg, persistent=args.persistent, occ=args.occ,
               kde_policy=args.kde_policy, arena_gib=args.arena_gib,
               lanes=lane_bounds(args.lane_pairs), lane_split=lane_bounds(args.lane_split),
-              lane_max_jobs=lane_bounds(args.lane_max_jobs))
+              lane_max_jobs=lane_bounds(args.lane_max_jobs),
+              lane_prio=lane_bounds(args.lane_prio))
     leg = serve_leg(args.rate, args.duration, args.serve_steps, args.seed, **kw)
     burst = burst_rate(args.burst, args.serve_steps, args.seed, **kw) if args.burst else None
     at_max = None
@@ -467,6 +469,9 @@ def main():
     ap.add_argument("--lane-max-jobs", default="",
                     help="serving: comma-separated jobs each lane holds (queued or resident) "
                          "before it passes new ones to the next lane (0 = no limit)")
+    ap.add_argument("--lane-prio", default="",
+                    help="serving: comma-separated issue priority (0-3) of each lane's "
+                         "wavefronts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -623,7 +628,8 @@ def main():
                                    arena_gib=args.arena_gib,
                                    lanes=lane_bounds(args.lane_pairs),
                                    lane_split=lane_bounds(args.lane_split),
-                                   lane_max_jobs=lane_bounds(args.lane_max_jobs))
+                                   lane_max_jobs=lane_bounds(args.lane_max_jobs),
+                                   lane_prio=lane_bounds(args.lane_prio))
         out["latency_s"] = out["serving"]["latency_s"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cx, temps, list(mc.move_scale), cfg, args.seed,

@@ -59,6 +59,7 @@ int gd_engine::init(const gd_engine_config *cfg)
   lc.persistent = c.persistent ? 1 : 0;
   lc.occ = (c.occ == 6 || c.occ == 8) ? c.occ : 0;
   lc.split = (c.split == 1 || c.split == 4 || c.split == 8 || c.split == 16) ? c.split : 0;
+  lc.prio = c.prio > 0 ? std::min(c.prio, 3) : 0;
   lc.stage_bytes = 16;
   GD_HIP_TRY(hipSetDevice(c.device));
   GD_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,

@@ -1355,6 +1355,20 @@ gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue, int n
     gd_advance_all(packs, n_cx, queue, n_queue);
 }
 
+/* Issue priority of this wavefront in its SIMD (s_setprio takes an
+ * immediate): with launches of several streams sharing the device, the
+ * higher one issues first whenever both have an instruction ready.  It
+ * changes no result. */
+__device__ __forceinline__ void gd_set_prio(int p)
+{
+  if (p == 1)
+    __builtin_amdgcn_s_setprio(1);
+  else if (p == 2)
+    __builtin_amdgcn_s_setprio(2);
+  else if (p >= 3)
+    __builtin_amdgcn_s_setprio(3);
+}
+
 /* ------------------------------------------------------------------------ */
 /* MC kernel                                                                 */
 /* mode 0: one cycle (steps from each complex's device counter)              */
@@ -1369,9 +1383,10 @@ gd_advance_kernel(const gd_pack *__restrict__ packs, int n_cx, int *queue, int n
 template <int WPG, int MINW>
 __global__ void __launch_bounds__(GD_WAVE *WPG) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ tasks,
-             int n_tasks, int *queue, int mode, int wslot_off)
+             int n_tasks, int *queue, int mode, int wslot_off, int prio)
 {
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  gd_set_prio(prio);
   __shared__ int s_task;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / GD_WAVE);
   gd_lds_f32 *ws = (gd_lds_f32 *) (smem + wslot_off) + w * GD_WSLOT_FLOATS;
@@ -1428,9 +1443,10 @@ gd_mc_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ task
 template <int SPLIT>
 __global__ void __launch_bounds__(GD_WAVE *SPLIT)
 gd_mc_split_kernel(const gd_pack *__restrict__ packs, const gd_task *__restrict__ tasks,
-                   int n_tasks, int *queue, int wslot_off)
+                   int n_tasks, int *queue, int wslot_off, int prio)
 {
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  gd_set_prio(prio);
   __shared__ int s_task;
   const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / GD_WAVE);
   gd_lds_f32 *ws = (gd_lds_f32 *) (smem + wslot_off);
@@ -1735,12 +1751,12 @@ gd_rescore64_best_kernel(const gd_pack *__restrict__ packs, const int2 *__restri
 /* ------------------------------------------------------------------------ */
 template <int WPG, int MINW>
 static hipError_t launch_mc_var(const gd_pack *packs, const gd_task *tasks, int n_tasks,
-                                int *queue, int grid, int mode, int stage_bytes,
+                                int *queue, int grid, int mode, int stage_bytes, int prio,
                                 hipStream_t st)
 {
   const int lds = stage_bytes + WPG * GD_WSLOT_FLOATS * 4;
   hipLaunchKernelGGL((gd_mc_kernel<WPG, MINW>), dim3(grid), dim3(GD_WAVE * WPG), lds, st,
-                     packs, tasks, n_tasks, queue, mode, stage_bytes);
+                     packs, tasks, n_tasks, queue, mode, stage_bytes, prio);
   return hipGetLastError();
 }
 
@@ -1750,19 +1766,25 @@ static hipError_t launch_mc_wpg(const gd_launch_cfg &C, const gd_pack *packs,
                                 int mode, hipStream_t st)
 {
   switch (C.occ) {
-  case 6: return launch_mc_var<WPG, 6>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
-  case 8: return launch_mc_var<WPG, 8>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
-  default: return launch_mc_var<WPG, 1>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes, st);
+  case 6:
+    return launch_mc_var<WPG, 6>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes,
+                                 C.prio, st);
+  case 8:
+    return launch_mc_var<WPG, 8>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes,
+                                 C.prio, st);
+  default:
+    return launch_mc_var<WPG, 1>(packs, tasks, n_tasks, queue, grid, mode, C.stage_bytes,
+                                 C.prio, st);
   }
 }
 
 template <int SPLIT>
 static hipError_t launch_split(const gd_pack *packs, const gd_task *tasks, int n_tasks,
-                               int *queue, int grid, int stage_bytes, hipStream_t st)
+                               int *queue, int grid, int stage_bytes, int prio, hipStream_t st)
 {
   const int lds = stage_bytes + GD_WSLOT_FLOATS * 4 + (int) GD_SPLIT_AREA_BYTES;
   hipLaunchKernelGGL(gd_mc_split_kernel<SPLIT>, dim3(grid), dim3(GD_WAVE * SPLIT), lds, st,
-                     packs, tasks, n_tasks, queue, stage_bytes);
+                     packs, tasks, n_tasks, queue, stage_bytes, prio);
   return hipGetLastError();
 }
 
@@ -1773,9 +1795,10 @@ hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_t
     return hipSuccess;
   if (mode == 0 && C.split > 1) {
     switch (C.split) {
-    case 4: return launch_split<4>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, st);
-    case 8: return launch_split<8>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, st);
-    case 16: return launch_split<16>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, st);
+    case 4: return launch_split<4>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, C.prio, st);
+    case 8: return launch_split<8>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, C.prio, st);
+    case 16:
+      return launch_split<16>(packs, tasks, n_tasks, queue, grid, C.stage_bytes, C.prio, st);
     default: return hipErrorInvalidValue;
     }
   }

@@ -15,6 +15,8 @@ typedef struct gd_launch_cfg {
   int occ;          /* register budget for waves per SIMD: 0 (compiler), 6 or 8 */
   int split;        /* wavefronts per replica in MC cycles: 1, or 4, 8, 16 with one
                        replica per workgroup (tasks of one replica) */
+  int prio;         /* issue priority of the MC launches' wavefronts in their SIMD
+                       against other launches' (s_setprio 0-3) */
 } gd_launch_cfg;
 
 hipError_t gd_launch_mc(const gd_launch_cfg &C, const gd_pack *packs, const gd_task *tasks,

@@ -524,6 +524,7 @@ extern "C" int gd_server_create(const gd_server_config *cfg, gd_server **out)
     ec.kde_policy = s->cfg.kde_policy;
     ec.occ = s->cfg.occ;
     ec.split = s->cfg.split;
+    ec.prio = s->cfg.lane_prio[k];
     if (s->cfg.lane_split[k] > 1) {
       if (s->cfg.lane_split[k] != 4 && s->cfg.lane_split[k] != 8 && s->cfg.lane_split[k] != 16)
         return gd_fail(GD_ERR_ARG, "lane_split[%d] = %d: 0, 1, 4, 8 or 16", k,

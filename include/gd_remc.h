@@ -406,6 +406,9 @@ typedef struct gd_engine_config {
                           pair list, restraint targets and PSP rows do not fit
                           in LDS beside the split area, or the KDE points are
                           kept in HBM */
+  int prio;            /* 0-3: issue priority of the MC launches' wavefronts in
+                          their SIMD against other streams' launches on the
+                          device (s_setprio); 0 = the default */
 } gd_engine_config;
 
 #define GD_TUNE_MAX 32
@@ -542,6 +545,9 @@ typedef struct gd_server_config {
   int lane_max_pairs[GD_SERVER_MAX_LANES - 1];
   int lane_split[GD_SERVER_MAX_LANES];
   int lane_max_jobs[GD_SERVER_MAX_LANES];
+  /* lane_prio[k]: gd_engine_config.prio of lane k (a lane of small jobs
+   * above the others: its wavefronts issue first) */
+  int lane_prio[GD_SERVER_MAX_LANES];
 } gd_server_config;
 
 enum gd_job_state {

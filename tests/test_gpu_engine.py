@@ -215,13 +215,14 @@ def test_server_results_equal_alone_under_continuous_batching(jobs):
 
 
 def test_server_lanes_route_by_size(jobs):
-    """Three lanes (bounds on ligand atoms x pocket points): each job goes to
-    the first lane whose bound holds its size, the lanes cycle on their own,
-    and every result equals the replay of the job alone."""
+    """Three lanes (bounds on ligand atoms x pocket points), the first two
+    with raised wavefront priority: each job goes to the first lane whose
+    bound holds its size, the lanes cycle on their own, and every result
+    equals the replay of the job alone."""
     js, replays = jobs
     pairs = [j.cx.lig_xyz.shape[1] * j.cx.prt_xyz.shape[1] for j in js]
     lo, hi = sorted(pairs)[1], sorted(pairs)[3]
-    srv = remc.Server(arena_bytes=96 << 20, wpg=8, lane_max_pairs=(lo, hi))
+    srv = remc.Server(arena_bytes=96 << 20, wpg=8, lane_max_pairs=(lo, hi), lane_prio=(3, 1, 0))
     tickets = [srv.submit(j) for j in js]
     srv.drain()
     lanes = []
