@@ -298,6 +298,37 @@ def test_server_lane_overflow(jobs):
     assert st["lane_completed"][:2] == [lanes.count(0), lanes.count(1)]
 
 
+def test_server_lanes_cancel_and_close(jobs):
+    """Lanes whose arenas hold one job each: a queued job cancelled in one
+    lane never runs while the others complete; closing the server with jobs
+    resident in several lanes and more queued ends every job as cancelled
+    (no hang)."""
+    js, replays = jobs
+    pairs = [j.cx.lig_xyz.shape[1] * j.cx.prt_xyz.shape[1] for j in js]
+    bound = sorted(pairs)[2]
+    need = max(j.footprint() for j in js)
+    srv = remc.Server(arena_bytes=2 * (need + need // 2), wpg=8, lane_max_pairs=(bound,))
+    small = [i for i, p in enumerate(pairs) if p <= bound]
+    t = [srv.submit(js[i]) for i in small]
+    t_cancel = srv.submit(js[small[0]])
+    srv.cancel(t_cancel)
+    srv.drain()
+    for i, x in zip(small, t):
+        g, _, info = srv.fetch(x)
+        _same(g, replays[i], f"lane 0 job {i}")
+        assert info["lane"] == 0
+    assert srv.wait(t_cancel, 10.0)["state"] == remc.JOB_CANCELLED
+    # a long job in each lane, more queued behind them, then close
+    long_jobs = []
+    for i in (small[0], next(i for i, p in enumerate(pairs) if p > bound)):
+        cx, temps, mc = js[i].cx, js[i].temps, remc.mcpara(200000, 10, list(js[i].mc.move_scale))
+        long_jobs.append(remc.JobSpec(cx, temps, mc, js[i].opt))
+    tl = [srv.submit(j) for j in long_jobs + long_jobs]
+    st = srv.stats()
+    assert st["submitted"] == len(small) + 1 + 4
+    srv.close()
+
+
 def test_server_admission_by_footprint_and_cancel(jobs):
     """An arena that holds one job at a time serialises admission (FIFO);
     a cancelled queued job never runs; a job larger than the arena is
