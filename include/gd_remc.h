@@ -527,9 +527,9 @@ typedef struct gd_server_config {
    * lane k < lanes - 1 with lna x pnp <= lane_max_pairs[k], else to the
    * last.  Admission stays FIFO within a lane; max_resident_replicas counts
    * every lane.  0 or 1 = one engine.  The lanes' streams share the
-   * process's hardware queues (HIP's default is 4): with four lanes two of
-   * them shared a queue and their launches ran in turn, so three lanes is
-   * the practical number.
+   * process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default): with four
+   * lanes two of them shared a queue and their launches ran in turn; with 8
+   * queues they did not, but four lanes served no better than three.
    * lane_split[k] > 1 runs lane k with that many wavefronts per replica (4,
    * 8 or 16) and KDE points staged in LDS: a few small jobs then fill the
    * device, so they are done in a few cycles of their own instead of waiting
