@@ -198,6 +198,20 @@ def test_no_gpu_is_an_error_not_a_fallback():
         remc.score_poses(cx, [0], [0], np.zeros((1, 6)))
 
 
+def test_server_lane_settings_checked_before_device_work():
+    """Lane bounds must be positive and increasing, at most four lanes, and
+    a lane's split factor one of 0, 1, 4, 8, 16: refused before any engine
+    touches a device (so this runs without a GPU)."""
+    for kw, msg in [(dict(lane_max_pairs=(5000, 3000)), "increasing"),
+                    (dict(lane_max_pairs=(0,)), "increasing"),
+                    (dict(lane_max_pairs=(1, 2, 3, 4)), "lanes"),
+                    (dict(lane_max_pairs=(1000,), lane_split=(3, 0)), "lane_split")]:
+        with pytest.raises(remc.GDError, match=msg):
+            remc.Server(**kw)
+    import bench
+    assert bench.lane_bounds("20000, 40000") == (20000, 40000) and bench.lane_bounds("") == ()
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py measures one GPU; --gpus N > 1 exits with a message."""
     import subprocess
