@@ -28,6 +28,15 @@ import time
 
 import numpy as np
 
+# The serving lanes launch on a stream each; HIP maps a process's streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 by default), and two lanes that share a
+# queue run their launches in turn.  The runtime reads it when it starts, so it
+# is raised here to 8, before anything touches the device (a larger value set
+# by the caller stays).
+_queues = os.environ.get("GPU_MAX_HW_QUEUES", "")
+if not _queues.isdigit() or int(_queues) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -262,7 +271,7 @@ def bench_batch(args):
 
 def serve_leg(rate, duration, steps, seed, wpg=16, persistent=0, occ=8, kde_policy=1,
               arena_gib=16, tag="serve", lanes=(), lane_split=(), lane_max_jobs=(),
-              lane_prio=()):
+              lane_prio=(), lane_max_total=()):
     """Serving loop: Poisson arrivals at `rate` jobs/s for `duration` s, each
     job a complex drawn from a pool of 256 (ligand atoms lognormal median 32
     in [10, 120], pocket points lognormal median 600 in [150, 2000]) with 16
@@ -284,7 +293,8 @@ def serve_leg(rate, duration, steps, seed, wpg=16, persistent=0, occ=8, kde_poli
              for i in range(len(arrivals))]
     srv = remc.Server(arena_bytes=arena_gib << 30, wpg=wpg, persistent=persistent,
                       keep_results=0, occ=occ, kde_policy=kde_policy, lane_max_pairs=lanes,
-                      lane_split=lane_split, lane_max_jobs=lane_max_jobs, lane_prio=lane_prio)
+                      lane_split=lane_split, lane_max_jobs=lane_max_jobs, lane_prio=lane_prio,
+                      lane_max_total=lane_max_total)
     print(f"{tag}: {len(specs)} jobs at {rate:.1f}/s over {duration} s", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
@@ -329,6 +339,7 @@ def serve_leg(rate, duration, steps, seed, wpg=16, persistent=0, occ=8, kde_poli
         "server": {"wpg": wpg, "persistent": persistent, "occ": occ, "kde_policy": kde_policy,
                    "lane_max_pairs": list(lanes), "lane_split_asked": list(lane_split),
                    "lane_max_jobs": list(lane_max_jobs), "lane_prio": list(lane_prio),
+                   "lane_max_total": list(lane_max_total),
                    "lane_split": st["lane_split"][:st["lanes"]],
                    "lane_cycles": st["lane_cycles"][:st["lanes"]],
                    "lane_completed": st["lane_completed"][:st["lanes"]]},
@@ -344,7 +355,7 @@ def lane_bounds(text):
 
 
 def burst_rate(n, steps, seed, wpg=16, persistent=0, occ=8, kde_policy=1, arena_gib=16,
-               lanes=(), lane_split=(), lane_max_jobs=(), lane_prio=()):
+               lanes=(), lane_split=(), lane_max_jobs=(), lane_prio=(), lane_max_total=()):
     """n jobs submitted at once: the completion rate the server sustains."""
     from gpudock_amd import remc, synth
     import numpy as np
@@ -356,7 +367,8 @@ def burst_rate(n, steps, seed, wpg=16, persistent=0, occ=8, kde_policy=1, arena_
              for i in range(n)]
     srv = remc.Server(arena_bytes=arena_gib << 30, wpg=wpg, persistent=persistent,
                       keep_results=0, occ=occ, kde_policy=kde_policy, lane_max_pairs=lanes,
-                      lane_split=lane_split, lane_max_jobs=lane_max_jobs, lane_prio=lane_prio)
+                      lane_split=lane_split, lane_max_jobs=lane_max_jobs, lane_prio=lane_prio,
+                      lane_max_total=lane_max_total)
     tb = time.perf_counter()
     for sp in specs:
         srv.submit(sp)
@@ -378,7 +390,8 @@ def bench_serve(args):
               kde_policy=args.kde_policy, arena_gib=args.arena_gib,
               lanes=lane_bounds(args.lane_pairs), lane_split=lane_bounds(args.lane_split),
               lane_max_jobs=lane_bounds(args.lane_max_jobs),
-              lane_prio=lane_bounds(args.lane_prio))
+              lane_prio=lane_bounds(args.lane_prio),
+              lane_max_total=lane_bounds(args.lane_max_total))
     leg = serve_leg(args.rate, args.duration, args.serve_steps, args.seed, **kw)
     burst = burst_rate(args.burst, args.serve_steps, args.seed, **kw) if args.burst else None
     at_max = None
@@ -457,21 +470,26 @@ def main():
                     help="serving: MC kernel register budget (0, 6 or 8 waves per SIMD)")
     ap.add_argument("--kde-policy", type=int, default=1,
                     help="serving: 1 = KDE points read from HBM, 0 = staged in LDS when they fit")
-    # serving lanes: at 200 jobs/s, jobs up to 20,000 and 40,000 pairs per pose
-    # in lanes of their own cut the p50 latency from 0.66 s to 0.43 s at the
-    # same p99 and throughput (two lanes at 24,000: 0.42 s, p99 0.85 s)
-    ap.add_argument("--lane-pairs", default="20000,40000",
+    # serving lanes: a split lane (16 wavefronts per replica) for jobs up to
+    # 20,000 pairs per pose while the server holds fewer than 32 jobs (at most 4
+    # in the lane), then plain lanes up to 20,000 and 40,000 pairs and one for
+    # the rest.  p50 / p99 against one engine: 0.41 / 0.77 s against 0.66-0.68 /
+    # 0.78-0.93 s at 200 jobs/s; 0.033 / 0.53 s against 0.40 / 0.54 s at 50 jobs/s
+    ap.add_argument("--lane-pairs", default="20000,20001,40000",
                     help="serving: comma-separated upper bounds of ligand atoms x pocket "
                          "points for all lanes but the last (empty = one engine)")
-    ap.add_argument("--lane-split", default="",
+    ap.add_argument("--lane-split", default="16,1,1,1",
                     help="serving: comma-separated wavefronts per replica of each lane "
                          "(4, 8 or 16 = split, KDE points in LDS; 0 = chosen from the resident count)")
-    ap.add_argument("--lane-max-jobs", default="",
+    ap.add_argument("--lane-max-jobs", default="4,0,0,0",
                     help="serving: comma-separated jobs each lane holds (queued or resident) "
                          "before it passes new ones to the next lane (0 = no limit)")
     ap.add_argument("--lane-prio", default="",
                     help="serving: comma-separated issue priority (0-3) of each lane's "
                          "wavefronts")
+    ap.add_argument("--lane-max-total", default="32,0,0,0",
+                    help="serving: comma-separated jobs in the whole server above which "
+                         "each lane passes new ones on (0 = no limit)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -629,7 +647,8 @@ def main():
                                    lanes=lane_bounds(args.lane_pairs),
                                    lane_split=lane_bounds(args.lane_split),
                                    lane_max_jobs=lane_bounds(args.lane_max_jobs),
-                                   lane_prio=lane_bounds(args.lane_prio))
+                                   lane_prio=lane_bounds(args.lane_prio),
+                                   lane_max_total=lane_bounds(args.lane_max_total))
         out["latency_s"] = out["serving"]["latency_s"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cx, temps, list(mc.move_scale), cfg, args.seed,

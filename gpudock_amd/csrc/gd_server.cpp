@@ -556,13 +556,18 @@ extern "C" int gd_server_submit(gd_server *s, const gd_job *job, long long *tick
     return gd_fail(GD_ERR_ARG, "null argument");
   std::unique_ptr<SrvJob> j(new SrvJob());
   /* the image follows its lane's KDE placement; a lane passes a job on when
-   * it holds lane_max_jobs already, and a split lane one it cannot split */
+   * it holds lane_max_jobs already or the server lane_max_total, and a split
+   * lane one it cannot split */
   const int n_lanes = (int) s->lanes.size();
   for (int k = s->first_lane(*job);; ++k) {
-    if (k + 1 < n_lanes && s->cfg.lane_max_jobs[k] > 0) {
+    if (k + 1 < n_lanes && (s->cfg.lane_max_jobs[k] > 0 || s->cfg.lane_max_total[k] > 0)) {
       std::lock_guard<std::mutex> lk(s->mu);
       const SrvLane &l = *s->lanes[k];
-      if (l.n_pending + l.n_resident + l.admitting >= s->cfg.lane_max_jobs[k])
+      if (s->cfg.lane_max_jobs[k] > 0 &&
+          l.n_pending + l.n_resident + l.admitting >= s->cfg.lane_max_jobs[k])
+        continue;
+      if (s->cfg.lane_max_total[k] > 0 &&
+          (int) s->pending.size() + s->n_running + s->admitting() >= s->cfg.lane_max_total[k])
         continue;
     }
     j->img.reset(new GdImage());

@@ -187,7 +187,8 @@ class ServerConfig(C.Structure):
                 ("lanes", C.c_int), ("lane_max_pairs", C.c_int * (SERVER_MAX_LANES - 1)),
                 ("lane_split", C.c_int * SERVER_MAX_LANES),
                 ("lane_max_jobs", C.c_int * SERVER_MAX_LANES),
-                ("lane_prio", C.c_int * SERVER_MAX_LANES)]
+                ("lane_prio", C.c_int * SERVER_MAX_LANES),
+                ("lane_max_total", C.c_int * SERVER_MAX_LANES)]
 
 
 class ServerJobInfo(C.Structure):
@@ -823,15 +824,19 @@ class Server:
     def __init__(self, device=0, arena_bytes=0, wpg=8, persistent=0, use_graph=0,
                  kde_policy=0, max_resident_replicas=0, cycles_per_tick=1,
                  max_ticks_in_flight=2, keep_results=1, readback_bytes=0, occ=0, split=0,
-                 lane_max_pairs=(), lane_split=(), lane_max_jobs=(), lane_prio=()):
+                 lane_max_pairs=(), lane_split=(), lane_max_jobs=(), lane_prio=(),
+                 lane_max_total=()):
         """lane_max_pairs: upper bounds of lna x pnp for all lanes but the
         last (len + 1 lanes; empty = one engine); lane_split: wavefronts per
         replica of each lane (0 = as split); lane_max_jobs: jobs a lane holds
         queued or resident before it passes new ones on (0 = no limit);
-        lane_prio: issue priority (0-3) of each lane's wavefronts."""
+        lane_prio: issue priority (0-3) of each lane's wavefronts;
+        lane_max_total: jobs in the whole server above which a lane passes
+        new ones on (0 = no limit)."""
         bounds = list(lane_max_pairs)
         if (len(bounds) > SERVER_MAX_LANES - 1 or len(lane_split) > SERVER_MAX_LANES
-                or len(lane_max_jobs) > SERVER_MAX_LANES or len(lane_prio) > SERVER_MAX_LANES):
+                or len(lane_max_jobs) > SERVER_MAX_LANES or len(lane_prio) > SERVER_MAX_LANES
+                or len(lane_max_total) > SERVER_MAX_LANES):
             raise GDError(f"at most {SERVER_MAX_LANES} lanes")
         cfg = ServerConfig(device, arena_bytes, wpg, persistent, use_graph, kde_policy,
                            max_resident_replicas, cycles_per_tick, max_ticks_in_flight,
@@ -839,7 +844,8 @@ class Server:
                            (C.c_int * (SERVER_MAX_LANES - 1))(*bounds),
                            (C.c_int * SERVER_MAX_LANES)(*lane_split),
                            (C.c_int * SERVER_MAX_LANES)(*lane_max_jobs),
-                           (C.c_int * SERVER_MAX_LANES)(*lane_prio))
+                           (C.c_int * SERVER_MAX_LANES)(*lane_prio),
+                           (C.c_int * SERVER_MAX_LANES)(*lane_max_total))
         h = C.c_void_p()
         check(lib().gd_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

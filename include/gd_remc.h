@@ -548,6 +548,10 @@ typedef struct gd_server_config {
   /* lane_prio[k]: gd_engine_config.prio of lane k (a lane of small jobs
    * above the others: its wavefronts issue first) */
   int lane_prio[GD_SERVER_MAX_LANES];
+  /* lane_max_total[k] > 0: lane k takes a job only while fewer jobs than
+   * that are queued or resident in the whole server, so a split lane serves
+   * light load and steps aside under heavy load.  0 = no limit. */
+  int lane_max_total[GD_SERVER_MAX_LANES];
 } gd_server_config;
 
 enum gd_job_state {

@@ -276,8 +276,9 @@ def test_server_split_lane(jobs):
 
 def test_server_lane_overflow(jobs):
     """A split lane that holds at most one job passes the small jobs that
-    arrive while it is busy to the next lane; every result still equals the
-    replay of the job alone."""
+    arrive while it is busy to the next lane, and one gated on the whole
+    server passes them on while any job is in it; every result still equals
+    the replay of the job alone."""
     js, replays = jobs
     pairs = [j.cx.lig_xyz.shape[1] * j.cx.prt_xyz.shape[1] for j in js]
     bound = sorted(pairs)[3]
@@ -296,6 +297,30 @@ def test_server_lane_overflow(jobs):
     assert lanes[first_small] == 0
     assert all(ln == 1 for ln, p in zip(lanes, pairs) if p > bound)
     assert st["lane_completed"][:2] == [lanes.count(0), lanes.count(1)]
+    # gated on the whole server instead: while a long job is in the server
+    # the split lane is closed to small jobs; in an empty server it is open
+    small = [i for i, p in enumerate(pairs) if p <= bound]
+    big = next(i for i, p in enumerate(pairs) if p > bound)
+    long_job = remc.JobSpec(js[big].cx, js[big].temps,
+                            remc.mcpara(200000, 10, list(js[big].mc.move_scale)), js[big].opt)
+    srv = remc.Server(arena_bytes=96 << 20, wpg=8, lane_max_pairs=(bound,), lane_split=(16, 1),
+                      lane_max_total=(1, 0))
+    t_long = srv.submit(long_job)
+    tickets = {i: srv.submit(js[i]) for i in small}
+    for i, t in tickets.items():
+        info = srv.wait(t, 60.0)
+        assert info["state"] == remc.JOB_DONE and info["lane"] == 1
+        g, _, _ = srv.fetch(t)
+        _same(g, replays[i], f"gated job {i}")
+    srv.cancel(t_long)
+    srv.drain()
+    assert srv.wait(t_long, 10.0)["state"] == remc.JOB_CANCELLED
+    t = srv.submit(js[small[0]])
+    assert srv.wait(t, 60.0)["state"] == remc.JOB_DONE
+    g, _, info = srv.fetch(t)
+    assert info["lane"] == 0
+    _same(g, replays[small[0]], "gated job in an empty server")
+    srv.close()
 
 
 def test_server_lanes_cancel_and_close(jobs):
